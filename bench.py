@@ -1,0 +1,135 @@
+"""Benchmark of the synthesis hot path on one MI355X.
+
+One step = one full engine run (imageSynth on device-resident buffers: adaptation,
+point lists, sorted offsets, target ordering, every refinement pass, write-back) over
+config 2 of the benchmark configurations: a 2048x2048 RGBA value-noise texture with a
+union of 12 seeded ellipses as the hole (531,632 target pixels), patchSize 30,
+maxProbeCount 200, sensitivity 0.117, random order, batched mode.
+
+Prints one JSON line.  `value` is target pixels per second of the whole job with the
+inputs already in HBM.  `roofline` prices the dominant kernel (the batched visit kernel)
+by its algorithmic gather bytes: one corpus pixel word per patch neighbour per scored
+candidate.  `cpu_baseline` is the CPU oracle in the reference's threaded mode (8 threads,
+lib/refinerThreaded.h) on a bounded sample of the same workload.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-quality]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "target px/s at equal mean match error: learned-proposal vs plain, 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(threads=8):
+    """Oracle in threaded-reference mode on config 2 scaled to 1024x1024 (same generators,
+    same ellipse count, semi-axes scaled by 1/2): px/s of a whole engine run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    from resynthesizer_amd import workloads as W
+    img, mask = W.config2(size=1024)
+    t = time.time()
+    r, _ = O.image_synth(img, mask, 1, O.default_params(), threads=threads)
+    dt = time.time() - t
+    return {"value": r.nTargets / dt, "unit": "target px/s", "cores": threads, "kind": "port",
+            "sample": "config 2 generators at 1024x1024 (%d target px), one whole engine run, %.1f s, "
+                      "8-thread reference mode of the CPU oracle" % (r.nTargets, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-quality", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import resynthesizer_amd as R
+    from resynthesizer_amd import workloads as W
+
+    img, mask = W.config2(size=args.size)
+    h, w, c = img.shape
+    d_src = torch.from_numpy(img).cuda()
+    d_img = torch.empty_like(d_src)
+    d_mask = torch.from_numpy(mask).cuda()
+    params = R.default_params()
+    torch.cuda.synchronize()
+
+    def step(timing=False):
+        d_img.copy_(d_src)
+        torch.cuda.synchronize()
+        r = R.image_synth_device(d_img.data_ptr(), w, h, w * c, d_mask.data_ptr(), w, R.T_RGBA, params,
+                                 mode=R.MODE_BATCHED, batch=args.batch, timing=timing)
+        assert r.err == 0, r.err
+        return r
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    n_t = r.nTargets
+    value = n_t * args.steps / elapsed
+
+    # kernel timing of the dominant kernel (HIP events recorded by the library on its stream)
+    rt = step(timing=True)
+    kern_ms = rt.synthKernelMs
+    launches = rt.launches
+    alg_bytes = rt.patchTerms * rt.pixelWordBytes
+    achieved = (alg_bytes / launches) / (kern_ms / launches / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "synth_batched_config2_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "target px/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded value noise + grating, 12 seeded ellipses)",
+        "config": {"workload": "config 2: plain heal, %dx%d RGBA, %d target px, patch 30, 200 probes, "
+                               "sensitivity 0.117, batched mode" % (w, h, n_t),
+                   "proposal": "plain (200 random probes)", "target_px": n_t,
+                   "passes": rt.passes, "visits": rt.visits, "candidates": rt.candidates,
+                   "mean_match_error": round(rt.meanError, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_synth_batched", "launches": launches,
+                     "avg_launch_ms": round(kern_ms / max(launches, 1), 4),
+                     "algorithmic_bytes_per_launch": int(alg_bytes / max(launches, 1))},
+    }
+    if not args.no_quality:
+        # the exact serial engine on the same input: the reference's own mean match error
+        d_img.copy_(d_src)
+        torch.cuda.synchronize()
+        t = time.time()
+        rs = R.image_synth_device(d_img.data_ptr(), w, h, w * c, d_mask.data_ptr(), w, R.T_RGBA, params,
+                                  mode=R.MODE_SERIAL)
+        out["config"]["serial_mean_match_error"] = round(rs.meanError, 1)
+        out["config"]["error_ratio_batched_vs_serial"] = round(rt.meanError / rs.meanError, 4)
+        out["config"]["serial_seconds"] = round(time.time() - t, 2)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

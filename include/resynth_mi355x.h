@@ -1,0 +1,230 @@
+/*
+ * resynth_mi355x.h -- C ABI of libresynth_mi355x.so, an MI355X (gfx950) drop-in for
+ * the resynthesizer synthesis library (reference: lib/ of bootchk/resynthesizer).
+ *
+ * Part 1 re-declares the reference's public types and entry points with identical
+ * layouts, names, argument meaning and error codes, so that a caller compiled against
+ * the reference's headers links against this library unchanged.
+ * Part 2 declares extensions with no counterpart in the reference: mode selection,
+ * device-resident inputs (for callers whose images already live in HBM), run
+ * statistics, and the frozen-state scorer used by the parity tests.
+ *
+ * All entry points are synchronous and reentrant per call; calls on one process share
+ * one HIP stream per device and are serialized by the library.
+ */
+#ifndef RESYNTH_MI355X_H
+#define RESYNTH_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================= Part 1 */
+
+/* lib/imageBuffer.h:12-18 -- rows padded to rowBytes, pixels unpadded */
+typedef struct _ImageBuffer {
+  unsigned char *data;
+  unsigned int width;
+  unsigned int height;
+  size_t rowBytes;
+} ImageBuffer;
+
+/* lib/imageFormat.h:36-42 */
+typedef enum ImageFormat { T_RGB, T_RGBA, T_Gray, T_GrayA } TImageFormat;
+
+/* lib/engineParams.h:13-28 */
+typedef enum ImageSynthError {
+  IMAGE_SYNTH_SUCCESS,
+  IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT,
+  IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH,
+  IMAGE_SYNTH_ERROR_PATCH_SIZE_EXCEEDED,
+  IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE,
+  IMAGE_SYNTH_ERROR_EMPTY_TARGET,
+  IMAGE_SYNTH_ERROR_EMPTY_CORPUS
+} TImageSynthError;
+
+/* lib/engineParams.h:31-86 */
+typedef struct ImageSynthParametersStruct {
+  int isMakeSeamlesslyTileableHorizontally;
+  int isMakeSeamlesslyTileableVertically;
+  int matchContextType;
+  double mapWeight;
+  double sensitivityToOutliers;
+  unsigned int patchSize;
+  unsigned int maxProbeCount;
+} TImageSynthParameters;
+
+/* lib/imageFormatIndicies.h:39-58 */
+typedef unsigned char TPixelelIndex;
+typedef struct indicesStruct {
+  TPixelelIndex colorEndBip;
+  TPixelelIndex alpha_bip;
+  TPixelelIndex map_start_bip;
+  TPixelelIndex map_end_bip;
+  TPixelelIndex img_match_bpp;
+  TPixelelIndex map_match_bpp;
+  TPixelelIndex total_bpp;
+  int isAlphaTarget;
+  int isAlphaSource;
+} TFormatIndices;
+
+/* lib/map.h:28-46.  GArray is GLib's public prefix {data, len} (lib/glibProxy.h:86-92);
+ * callers that include <glib.h> first get GLib's own declaration. */
+#ifndef __G_ARRAY_H__
+typedef struct _GArray {
+  char *data;
+  unsigned int len;
+} GArray;
+#endif
+typedef struct {
+  unsigned int width;
+  unsigned int height;
+  unsigned int depth;
+  GArray *data;
+} Map;
+typedef struct {
+  int x;
+  int y;
+} Coordinates;
+typedef unsigned char Pixelel;
+
+/* lib/imageSynth.h:32-40 (replaces lib/imageSynth.c:62-137) */
+int imageSynth(ImageBuffer *imageBuffer, ImageBuffer *mask, TImageFormat imageFormat,
+               TImageSynthParameters *parameters, void (*progressCallback)(int, void *),
+               void *contextInfo, int *cancelFlag);
+
+/* lib/imageSynth.h:43-52 (replaces lib/imageSynth.c:140-217) */
+int imageSynth2(ImageBuffer *imageBuffer, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat imageFormat,
+                TImageSynthParameters *parameters, void (*progressCallback)(int, void *),
+                void *contextInfo, int *cancelFlag);
+
+/* lib/engine.h:4-12 (replaces lib/engine.c:539-690) */
+int engine(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+           void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag);
+
+/* lib/engineParams.h:93-95 (replaces lib/engineParams.c:9-18) */
+void setDefaultParams(TImageSynthParameters *param);
+
+/* lib/imageFormatIndicies.h:60-84 (replace lib/imageFormat.c:40-207) */
+unsigned int countPixelelsPerPixelForFormat(TImageFormat format);
+int prepareImageFormatIndicesFromFormatType(TFormatIndices *indices, TImageFormat format);
+void prepareImageFormatIndices(TFormatIndices *indices, unsigned int count_color_channels_target,
+                               unsigned int count_color_channels_map, int is_alpha_target,
+                               int is_alpha_source, int isMap);
+void prepareDefaultFormatIndices(TFormatIndices *formatIndices);
+
+/* lib/map.h:49-99 (replace lib/mapOps.h:45-168) -- maps for engine() callers */
+void free_map(Map *map);
+void new_pixmap(Map *map, unsigned int width, unsigned int height, unsigned int depth);
+void new_bytemap(Map *map, unsigned int width, unsigned int height);
+void new_intmap(Map *map, unsigned int width, unsigned int height);
+void new_coordmap(Map *map, unsigned int width, unsigned int height);
+void set_bytemap(Map *map, unsigned char value);
+void invert_bytemap(Map *map);
+void interleave_mask(Map *pixmap, Map *mask);
+
+/* ======================================================================= Part 2 */
+
+/* Synthesis modes.  No counterpart in the reference (which has one serial build and
+ * one nondeterministic threaded build, lib/buildSwitches.h:79).
+ *   RESYNTH_MODE_SERIAL:  the reference's serial visiting order and PRNG stream,
+ *                         byte-identical output to the reference's serial build.
+ *   RESYNTH_MODE_BATCHED: targets in conflict-tolerant batches, one wavefront per
+ *                         target, per-target counter-based random probes; judged by
+ *                         mean best-match error, like the reference's threaded build.
+ * The default for Part 1 entry points is RESYNTH_MODE_BATCHED unless the environment
+ * variable RESYNTH_MODE is "serial", or resynth_set_default_mode() was called. */
+enum { RESYNTH_MODE_SERIAL = 0, RESYNTH_MODE_BATCHED = 1 };
+
+/* PRNG of the serial mode: GLib GRand (the GIMP build) or the C library's rand()
+ * (the stand-alone build through lib/glibProxy.c). */
+enum { RESYNTH_PRNG_GRAND = 0, RESYNTH_PRNG_LIBC = 1 };
+
+typedef struct {
+  int mode;            /* RESYNTH_MODE_* */
+  int prng;            /* RESYNTH_PRNG_* (serial mode) */
+  unsigned int seed;   /* 0: the engine's seed 1198472.  Batched mode: seed of the per-target
+                          probe streams.  Serial mode: a nonzero seed replaces the engine's
+                          seed (for statistics over seeds; not a reference behaviour) */
+  unsigned int batch;  /* batched mode: largest number of concurrent target visits
+                          (0 = automatic: nT/32 clamped to [64, 16384]) */
+  int timing;          /* nonzero: time every synthesis kernel with HIP events */
+} ResynthOptions;
+
+typedef struct {
+  unsigned int nTargets, nCorpus, nOffsets;
+  unsigned int passes;
+  unsigned int passSize[6];
+  unsigned int betters[6];
+  unsigned long long visits;          /* target visits */
+  unsigned long long candidates;      /* scorer invocations (computeBestFit calls) */
+  unsigned long long patchTerms;      /* sum over scorer invocations of the patch size */
+  double meanError;                   /* mean best-match sum of each target's last visit */
+  unsigned int launches;              /* synthesis kernel launches */
+  double synthKernelMs;               /* sum of synthesis kernel durations (timing != 0) */
+  double prepMs, synthMs, totalMs;    /* host wall clock of the phases */
+  unsigned int pixelWordBytes;        /* 4 or 8: bytes gathered per corpus pixel */
+} ResynthStats;
+
+void resynth_set_default_mode(int mode);
+void resynth_default_options(ResynthOptions *opt);
+
+/* imageSynth / imageSynth2 with explicit options and statistics (stats may be NULL). */
+int imageSynthEx(ImageBuffer *imageBuffer, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat imageFormat,
+                 TImageSynthParameters *parameters, void (*progressCallback)(int, void *), void *contextInfo,
+                 int *cancelFlag, const ResynthOptions *options, ResynthStats *stats);
+
+/* engine() with explicit options and statistics. */
+int engineEx(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+             void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
+             const ResynthOptions *options, ResynthStats *stats);
+
+/* imageSynth on device-resident buffers: image (rows of rowBytes, synthesized in place),
+ * mask (rows of maskRowBytes), optional mask2 (corpus mask as in imageSynth2, else NULL),
+ * all device pointers on the library's device.  Same semantics and error codes as
+ * imageSynth2/imageSynth.  For callers whose images already live in HBM. */
+int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsigned int height, size_t rowBytes,
+                               const unsigned char *d_mask, size_t maskRowBytes,
+                               const unsigned char *d_mask2, size_t mask2RowBytes,
+                               TImageFormat imageFormat, const TImageSynthParameters *parameters,
+                               void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
+                               const ResynthOptions *options, ResynthStats *stats);
+
+/* Frozen-state scorer (the computeBestFit loop of lib/synthesize.h:266-400 over an
+ * ordered candidate list, strict-less update, stop at the first perfect match) for
+ * nProblems independent problems, evaluated on the GPU.  Host pointers:
+ *   corpus   : pixmap [ch][cw][depth] in the engine's layout (mask byte first)
+ *   T, Tm    : the 512-entry metric tables (lib/matchWeighting.h:195-204)
+ *   nbCount  : [nProblems] neighbour counts (<= 64)
+ *   nbOffset : [nProblems][64][2] neighbour offsets
+ *   nbPixel  : [nProblems][64][8] neighbour pixels in the engine layout
+ *   candCount: [nProblems]; cands: [nProblems][maxCands][2]
+ * Outputs: winner[nProblems] (-1 if none accepted), bestSum[nProblems],
+ *          stopped[nProblems] (candidates consumed before the stop).
+ * Returns 0, or -1 on invalid arguments. */
+int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned int ch,
+                         const TFormatIndices *indices, const uint16_t *T, const uint32_t *Tm,
+                         unsigned int nProblems, const unsigned int *nbCount, const int32_t *nbOffset,
+                         const unsigned char *nbPixel, const unsigned int *candCount, unsigned int maxCands,
+                         const int32_t *cands, int32_t *winner, uint32_t *bestSum, int32_t *stopped);
+
+/* Host-side pieces of the engine, exposed so that they can be checked without a GPU:
+ * raw PRNG outputs and int_range draws of the engine's PRNG (seeded like lib/engine.c:643
+ * when seed is 1198472), the metric tables (lib/matchWeighting.h:195-204), and the
+ * target ordering of lib/orderTarget.h:269-343 applied in place to n (x,y) points. */
+void resynth_prng_sample(int prng, unsigned int seed, unsigned int n, uint32_t *out);
+void resynth_prng_ranges(int prng, unsigned int seed, unsigned int n, const uint32_t *dists, uint32_t *out);
+void resynth_metric_tables(double sensitivity, double mapWeight, uint16_t *T, uint32_t *Tm);
+int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsigned int n);
+
+/* Library/device information: returns 0 when a gfx950 device is usable. */
+int resynth_device_info(char *name, size_t nameLen, int *computeUnits);
+const char *resynth_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

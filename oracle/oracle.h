@@ -61,6 +61,14 @@ enum { ORC_PRNG_GRAND = 0, ORC_PRNG_LIBC = 1 };
 typedef struct {
   int threads;   /* 0 or 1: serial refiner; >1: threaded refiner with that many threads */
   int prng;      /* ORC_PRNG_GRAND (GLib build) or ORC_PRNG_LIBC (stand-alone lib build) */
+  int batch;     /* >0: model of batched synthesis (not a reference mode): each pass runs in
+                    chunks of this many targets that all read the state at the chunk start;
+                    pass 0 ramps its chunks up from 256 */
+  int conflictRadius; /* batch model: >0 defers a target to a later chunk while a target of
+                         the current chunk lies within this Chebyshev distance */
+  int rampPermille;   /* batch model: >0 sizes pass-0 chunks as max(64, k*rampPermille/1000)
+                         where k is the number of targets already visited in the pass */
+  unsigned seed;      /* 0: the engine's seed 1198472 (lib/engine.c:643); else a study seed */
 } OOptions;
 
 typedef struct {

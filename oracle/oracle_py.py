@@ -40,7 +40,9 @@ class OPixmap(ctypes.Structure):
 
 
 class OOptions(ctypes.Structure):
-    _fields_ = [("threads", ctypes.c_int), ("prng", ctypes.c_int)]
+    _fields_ = [("threads", ctypes.c_int), ("prng", ctypes.c_int), ("batch", ctypes.c_int),
+                ("conflictRadius", ctypes.c_int), ("rampPermille", ctypes.c_int),
+                ("seed", ctypes.c_uint)]
 
 
 class OStats(ctypes.Structure):
@@ -236,7 +238,7 @@ def _buffer(arr2d_bytes, width, height, row_bytes):
 
 
 def image_synth(image, mask, fmt, params=None, threads=1, prng=PRNG_GRAND, cancel=0, want_maps=False,
-                mask2=None, width=None, mask_width=None, mask2_width=None):
+                mask2=None, width=None, mask_width=None, mask2_width=None, batch=0, radius=0, ramp=0, seed=0):
     """image: uint8 (H, W*bpp + pad) row buffer or (H, W, C); mask: uint8 (H, W[+pad]).
     Returns (result, image_out) with image_out the same shape as image."""
     img = np.array(image, dtype=np.uint8, copy=True, order="C")
@@ -256,7 +258,7 @@ def image_synth(image, mask, fmt, params=None, threads=1, prng=PRNG_GRAND, cance
     log = []
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
-    opt = OOptions(threads, prng)
+    opt = OOptions(threads, prng, batch, radius, ramp, seed)
     pp = ctypes.byref(_params(params)) if params is not None else None
     if mask2 is None:
         err = lib().oracle_imageSynth(ctypes.byref(ib), ctypes.byref(mb), fmt, pp, cb, None, ctypes.byref(cf),
@@ -270,7 +272,8 @@ def image_synth(image, mask, fmt, params=None, threads=1, prng=PRNG_GRAND, cance
     return _result(err, st, keep, log), img
 
 
-def engine(params, ix, target_pixmap, corpus_pixmap, threads=1, prng=PRNG_GRAND, cancel=0, want_maps=False):
+def engine(params, ix, target_pixmap, corpus_pixmap, threads=1, prng=PRNG_GRAND, cancel=0, want_maps=False,
+           batch=0, radius=0, ramp=0, seed=0):
     """target/corpus pixmaps: uint8 (H, W, depth) interleaved [mask, colours, alpha?, maps].
     Returns (result, target_out)."""
     t = np.array(target_pixmap, dtype=np.uint8, copy=True, order="C")
@@ -281,7 +284,7 @@ def engine(params, ix, target_pixmap, corpus_pixmap, threads=1, prng=PRNG_GRAND,
     log = []
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
-    opt = OOptions(threads, prng)
+    opt = OOptions(threads, prng, batch, radius, ramp, seed)
     err = lib().oracle_engine(_params(params), ctypes.byref(ix), ctypes.byref(tp), ctypes.byref(cp), cb, None,
                               ctypes.byref(cf), ctypes.byref(opt), ctypes.byref(st))
     return _result(err, st, keep, log), t

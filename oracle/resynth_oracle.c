@@ -302,6 +302,11 @@ typedef struct {
   uint16_t T[512];
   uint32_t Tm[512];
   uint32_t *lastBest;     /* target W*H, may be NULL */
+  /* what neighbour gathers read: the live maps, or chunk-start snapshots (batch model) */
+  const uint8_t *rdHas;
+  const uint8_t *rdTgt;
+  const int32_t *rdSrc;
+  const unsigned *kmap;   /* batch model: order index of each visited slot, else NULL */
   /* progress (lib/progress.c:53-65) */
   void (*progress)(int, void *);
   void *ctx;
@@ -348,10 +353,10 @@ static void add_neighbor(Engine *E, Neighbor *nb, int32_t ox, int32_t oy, int32_
   size_t p = (size_t)px + (size_t)py * E->tgt->width;
   nb->ox = ox; nb->oy = oy;
   pthread_mutex_lock(&E->mutex);
-  nb->sx = E->sourceOf[2 * p];
-  nb->sy = E->sourceOf[2 * p + 1];
+  nb->sx = E->rdSrc[2 * p];
+  nb->sy = E->rdSrc[2 * p + 1];
   memset(nb->pix, 0, 8);
-  memcpy(nb->pix, E->tgt->data + p * E->tgt->depth, E->ix->total_bpp);
+  memcpy(nb->pix, E->rdTgt + p * E->tgt->depth, E->ix->total_bpp);
   pthread_mutex_unlock(&E->mutex);
 }
 
@@ -363,7 +368,7 @@ static unsigned prepare_neighbors(Engine *E, int32_t tx, int32_t ty, Neighbor *n
   for (unsigned j = 1; j < E->nO; j++) {
     int32_t ox = E->offsets[2 * j], oy = E->offsets[2 * j + 1];
     int32_t px = tx + ox, py = ty + oy;
-    if (clip_or_wrap(E, &px, &py) && E->hasValue[(size_t)px + (size_t)py * E->tgt->width]) {
+    if (clip_or_wrap(E, &px, &py) && E->rdHas[(size_t)px + (size_t)py * E->tgt->width]) {
       add_neighbor(E, &nb[count++], ox, oy, px, py);
       if (count >= E->P.patchSize) break;
     }
@@ -403,13 +408,14 @@ static int compute_best_fit(Engine *E, int32_t cx, int32_t cy, uint32_t *best, i
 }
 
 /* lib/synthesize.h:426-642 synthesize (one thread's stride over a pass prefix) */
-static unsigned synthesize(Engine *E, unsigned threadIndex, unsigned stride, unsigned end, Prng *prng)
+static unsigned synthesize(Engine *E, unsigned start, unsigned threadIndex, unsigned stride, unsigned end,
+                           Prng *prng)
 {
   Neighbor nb[MAX_NEIGHBORS];
   unsigned betters = 0;
   const OFormatIndices *ix = E->ix;
   size_t W = E->tgt->width;
-  for (unsigned k = threadIndex % stride; k < end; k += stride) {
+  for (unsigned k = start + threadIndex % stride; k < end; k += stride) {
     if ((k & CALLBACK_MASK) == 0) {
       deep_progress(E);
       if (*E->cancel) break;
@@ -427,10 +433,11 @@ static unsigned synthesize(Engine *E, unsigned threadIndex, unsigned stride, uns
       int32_t cx = nb[j].sx - nb[j].ox, cy = nb[j].sy - nb[j].oy;
       if (clipped_or_masked(E, cx, cy)) continue;
       uint32_t *rp = &E->prober[(size_t)cx + (size_t)cy * E->cor->width];
-      if (*rp == k) continue;
+      unsigned tidx = E->kmap ? E->kmap[k] : k;
+      if (*rp == tidx) continue;
       perfect = compute_best_fit(E, cx, cy, &best, &bx, &by, count, nb, &bettered);
       if (perfect) break;
-      *rp = k;
+      *rp = tidx;
     }
     /* random corpus probes (:583-604) */
     if (!perfect) {
@@ -466,7 +473,7 @@ typedef struct { Engine *E; unsigned idx, stride, end; Prng prng; unsigned bette
 static void *thread_main(void *a)
 {
   ThreadArg *t = (ThreadArg *)a;
-  t->betters = synthesize(t->E, t->idx, t->stride, t->end, &t->prng);
+  t->betters = synthesize(t->E, 0, t->idx, t->stride, t->end, &t->prng);
   return NULL;
 }
 
@@ -486,8 +493,69 @@ static void refiner(Engine *E, Prng *prng, const OOptions *opt, OStats *st)
   for (unsigned pass = 0; pass < 6; pass++) {
     unsigned end = sizes[pass];
     unsigned long betters = 0;
-    if (threads == 1) {
-      betters = synthesize(E, 0, 1, end, prng);
+    if (opt && opt->batch > 0) {
+      size_t np = (size_t)E->tgt->width * E->tgt->height;
+      unsigned len = pass == 0 ? (256u < (unsigned)opt->batch ? 256u : (unsigned)opt->batch) : (unsigned)opt->batch;
+      if (pass == 0 && opt->rampPermille > 0) len = 64;
+      int R = opt->conflictRadius;
+      int TW = (int)E->tgt->width, TH = (int)E->tgt->height;
+      /* pending targets in synthesis order; each chunk takes up to len of them, skipping
+         (deferring) those within R of a target already taken */
+      unsigned *queue = (unsigned *)malloc((end ? end : 1) * sizeof(unsigned));
+      unsigned *next = (unsigned *)malloc((end ? end : 1) * sizeof(unsigned));
+      unsigned *chunk = (unsigned *)malloc((end ? end : 1) * sizeof(unsigned));
+      unsigned *stamp = (unsigned *)calloc(np, sizeof(unsigned));
+      unsigned nq = end, chunkId = 0;
+      for (unsigned i = 0; i < end; i++) queue[i] = i;
+      int32_t *savedTargets = E->targets;
+      int32_t *tmpTargets = (int32_t *)malloc((end ? end : 1) * 2 * sizeof(int32_t));
+      while (nq > 0) {
+        unsigned nc = 0, nn = 0;
+        chunkId++;
+        for (unsigned q = 0; q < nq; q++) {
+          unsigned k = queue[q];
+          int tx = savedTargets[2 * k], ty = savedTargets[2 * k + 1];
+          int ok = nc < len;
+          if (ok && R > 0) {
+            for (int dy = -R; dy <= R && ok; dy++)
+              for (int dx = -R; dx <= R && ok; dx++) {
+                int x = tx + dx, y = ty + dy;
+                if (x >= 0 && y >= 0 && x < TW && y < TH && stamp[(size_t)y * TW + x] == chunkId) ok = 0;
+              }
+          }
+          if (ok) {
+            chunk[nc++] = k;
+            stamp[(size_t)ty * TW + tx] = chunkId;
+          } else {
+            next[nn++] = k;
+          }
+        }
+        memcpy((void *)E->rdHas, E->hasValue, np);
+        memcpy((void *)E->rdTgt, E->tgt->data, np * E->tgt->depth);
+        memcpy((void *)E->rdSrc, E->sourceOf, np * 2 * sizeof(int32_t));
+        /* visit the chunk: lay its targets out contiguously; kmap keeps their order index
+           for the prober heuristic */
+        for (unsigned c = 0; c < nc; c++) {
+          tmpTargets[2 * c] = savedTargets[2 * chunk[c]];
+          tmpTargets[2 * c + 1] = savedTargets[2 * chunk[c] + 1];
+        }
+        E->targets = tmpTargets;
+        E->kmap = chunk;
+        betters += synthesize(E, 0, 0, 1, nc, prng);
+        E->targets = savedTargets;
+        E->kmap = NULL;
+        unsigned *t = queue; queue = next; next = t;
+        nq = nn;
+        if (pass == 0 && opt->rampPermille > 0) {
+          unsigned long long done = end - nq;
+          unsigned long long l = done * (unsigned long long)opt->rampPermille / 1000ull;
+          len = (unsigned)(l < 64 ? 64 : (l > (unsigned long long)opt->batch ? (unsigned long long)opt->batch : l));
+        } else if (len < (unsigned)opt->batch) len *= 2;
+        if (*E->cancel) break;
+      }
+      free(queue); free(next); free(chunk); free(stamp); free(tmpTargets);
+    } else if (threads == 1) {
+      betters = synthesize(E, 0, 0, 1, end, prng);
     } else {
       pthread_t th[64];
       ThreadArg args[64];
@@ -681,7 +749,7 @@ int oracle_engine(OParams P, const OFormatIndices *ix, OPixmap *tgt, const OPixm
   orc_quantize(P.sensitivity, P.mapWeight, E.T, E.Tm);
 
   Prng prng;
-  prng_init(&prng, opt ? opt->prng : ORC_PRNG_GRAND, ENGINE_SEED);
+  prng_init(&prng, opt ? opt->prng : ORC_PRNG_GRAND, (opt && opt->seed) ? opt->seed : ENGINE_SEED);
   int err = order_targets(P.matchContextType, E.targets, nT, &prng);
   if (err) {
     free(E.targets); free(E.hasValue); free(E.sourceOf); free(E.corpusPts); free(E.offsets);
@@ -692,6 +760,19 @@ int oracle_engine(OParams P, const OFormatIndices *ix, OPixmap *tgt, const OPixm
   E.prober = (uint32_t *)malloc(CW * CH * sizeof(uint32_t));
   for (size_t i = 0; i < CW * CH; i++) E.prober[i] = 0xFFFFFFFFu;
 
+  E.rdHas = E.hasValue;
+  E.rdTgt = E.tgt->data;
+  E.rdSrc = E.sourceOf;
+  uint8_t *snapHas = NULL, *snapTgt = NULL;
+  int32_t *snapSrc = NULL;
+  if (opt && opt->batch > 0) {
+    snapHas = (uint8_t *)malloc(TW * TH);
+    snapTgt = (uint8_t *)malloc(TW * TH * tgt->depth);
+    snapSrc = (int32_t *)malloc(TW * TH * 2 * sizeof(int32_t));
+    E.rdHas = snapHas;
+    E.rdTgt = snapTgt;
+    E.rdSrc = snapSrc;
+  }
   if (st && st->lastBest) {
     E.lastBest = st->lastBest;
     for (size_t i = 0; i < TW * TH; i++) E.lastBest[i] = UINT32_MAX;
@@ -714,6 +795,7 @@ int oracle_engine(OParams P, const OFormatIndices *ix, OPixmap *tgt, const OPixm
     }
   }
   free(E.prober); free(E.targets); free(E.hasValue); free(E.sourceOf); free(E.corpusPts); free(E.offsets);
+  free(snapHas); free(snapTgt); free(snapSrc);
   pthread_mutex_destroy(&E.mutex);
   return 0;
 }

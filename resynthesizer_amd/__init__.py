@@ -1,0 +1,326 @@
+"""Python host interface of libresynth_mi355x.so (the MI355X resynthesizer engine).
+
+Mirrors the reference library's entry points (lib/imageSynth.h:32-52, lib/engine.h:4-12):
+same argument meaning, error codes and in-place semantics, on numpy arrays.  Every call
+runs on the GPU through the C ABI declared in include/resynth_mi355x.h; there is no CPU
+path, and importing this package fails when the HIP library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libresynth_mi355x.so")
+
+# TImageFormat (lib/imageFormat.h:36-42)
+T_RGB, T_RGBA, T_Gray, T_GrayA = 0, 1, 2, 3
+# TImageSynthError (lib/engineParams.h:13-28)
+IMAGE_SYNTH_SUCCESS = 0
+IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT = 1
+IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH = 2
+IMAGE_SYNTH_ERROR_PATCH_SIZE_EXCEEDED = 3
+IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE = 4
+IMAGE_SYNTH_ERROR_EMPTY_TARGET = 5
+IMAGE_SYNTH_ERROR_EMPTY_CORPUS = 6
+
+MODE_SERIAL, MODE_BATCHED = 0, 1
+PRNG_GRAND, PRNG_LIBC = 0, 1
+
+
+class ImageBuffer(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_uint), ("height", ctypes.c_uint),
+                ("rowBytes", ctypes.c_size_t)]
+
+
+class TImageSynthParameters(ctypes.Structure):
+    _fields_ = [("isMakeSeamlesslyTileableHorizontally", ctypes.c_int),
+                ("isMakeSeamlesslyTileableVertically", ctypes.c_int),
+                ("matchContextType", ctypes.c_int), ("mapWeight", ctypes.c_double),
+                ("sensitivityToOutliers", ctypes.c_double), ("patchSize", ctypes.c_uint),
+                ("maxProbeCount", ctypes.c_uint)]
+
+
+class TFormatIndices(ctypes.Structure):
+    _fields_ = [("colorEndBip", ctypes.c_uint8), ("alpha_bip", ctypes.c_uint8),
+                ("map_start_bip", ctypes.c_uint8), ("map_end_bip", ctypes.c_uint8),
+                ("img_match_bpp", ctypes.c_uint8), ("map_match_bpp", ctypes.c_uint8),
+                ("total_bpp", ctypes.c_uint8), ("isAlphaTarget", ctypes.c_int), ("isAlphaSource", ctypes.c_int)]
+
+
+class GArray(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("len", ctypes.c_uint)]
+
+
+class Map(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint), ("height", ctypes.c_uint), ("depth", ctypes.c_uint),
+                ("data", ctypes.POINTER(GArray))]
+
+
+class ResynthOptions(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("prng", ctypes.c_int), ("seed", ctypes.c_uint), ("batch", ctypes.c_uint),
+                ("timing", ctypes.c_int)]
+
+
+class ResynthStats(ctypes.Structure):
+    _fields_ = [("nTargets", ctypes.c_uint), ("nCorpus", ctypes.c_uint), ("nOffsets", ctypes.c_uint),
+                ("passes", ctypes.c_uint), ("passSize", ctypes.c_uint * 6), ("betters", ctypes.c_uint * 6),
+                ("visits", ctypes.c_ulonglong), ("candidates", ctypes.c_ulonglong),
+                ("patchTerms", ctypes.c_ulonglong), ("meanError", ctypes.c_double), ("launches", ctypes.c_uint),
+                ("synthKernelMs", ctypes.c_double), ("prepMs", ctypes.c_double), ("synthMs", ctypes.c_double),
+                ("totalMs", ctypes.c_double), ("pixelWordBytes", ctypes.c_uint)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["passSize"] = list(self.passSize)[: self.passes]
+        d["betters"] = list(self.betters)[: self.passes]
+        return d
+
+
+PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p)
+
+# Every function declared in include/resynth_mi355x.h, with its ctypes signature.
+_P = ctypes.POINTER
+SIGNATURES = {
+    "imageSynth": (ctypes.c_int, [_P(ImageBuffer), _P(ImageBuffer), ctypes.c_int, _P(TImageSynthParameters),
+                                  PROGRESS, ctypes.c_void_p, _P(ctypes.c_int)]),
+    "imageSynth2": (ctypes.c_int, [_P(ImageBuffer), _P(ImageBuffer), _P(ImageBuffer), ctypes.c_int,
+                                   _P(TImageSynthParameters), PROGRESS, ctypes.c_void_p, _P(ctypes.c_int)]),
+    "engine": (ctypes.c_int, [TImageSynthParameters, _P(TFormatIndices), _P(Map), _P(Map), PROGRESS,
+                              ctypes.c_void_p, _P(ctypes.c_int)]),
+    "setDefaultParams": (None, [_P(TImageSynthParameters)]),
+    "countPixelelsPerPixelForFormat": (ctypes.c_uint, [ctypes.c_int]),
+    "prepareImageFormatIndicesFromFormatType": (ctypes.c_int, [_P(TFormatIndices), ctypes.c_int]),
+    "prepareImageFormatIndices": (None, [_P(TFormatIndices), ctypes.c_uint, ctypes.c_uint, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]),
+    "prepareDefaultFormatIndices": (None, [_P(TFormatIndices)]),
+    "free_map": (None, [_P(Map)]),
+    "new_pixmap": (None, [_P(Map), ctypes.c_uint, ctypes.c_uint, ctypes.c_uint]),
+    "new_bytemap": (None, [_P(Map), ctypes.c_uint, ctypes.c_uint]),
+    "new_intmap": (None, [_P(Map), ctypes.c_uint, ctypes.c_uint]),
+    "new_coordmap": (None, [_P(Map), ctypes.c_uint, ctypes.c_uint]),
+    "set_bytemap": (None, [_P(Map), ctypes.c_ubyte]),
+    "invert_bytemap": (None, [_P(Map)]),
+    "interleave_mask": (None, [_P(Map), _P(Map)]),
+    "resynth_set_default_mode": (None, [ctypes.c_int]),
+    "resynth_default_options": (None, [_P(ResynthOptions)]),
+    "imageSynthEx": (ctypes.c_int, [_P(ImageBuffer), _P(ImageBuffer), _P(ImageBuffer), ctypes.c_int,
+                                    _P(TImageSynthParameters), PROGRESS, ctypes.c_void_p, _P(ctypes.c_int),
+                                    _P(ResynthOptions), _P(ResynthStats)]),
+    "engineEx": (ctypes.c_int, [TImageSynthParameters, _P(TFormatIndices), _P(Map), _P(Map), PROGRESS,
+                                ctypes.c_void_p, _P(ctypes.c_int), _P(ResynthOptions), _P(ResynthStats)]),
+    "resynth_image_synth_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_size_t,
+                                                  ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                                  ctypes.c_int, _P(TImageSynthParameters), PROGRESS,
+                                                  ctypes.c_void_p, _P(ctypes.c_int), _P(ResynthOptions),
+                                                  _P(ResynthStats)]),
+    "resynth_score_frozen": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, _P(TFormatIndices),
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_prng_sample": (None, [ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p]),
+    "resynth_prng_ranges": (None, [ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_metric_tables": (None, [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_order_targets": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint]),
+    "resynth_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_int)]),
+    "resynth_version": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library.  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("resynthesizer_amd: %s is not built (run __graft_entry__.build() or "
+                               "python -m resynthesizer_amd.build); there is no CPU fallback" % LIB_PATH)
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7 under the same
+        # soname, and torch fails to initialize if another copy was loaded first.  Loading
+        # torch first makes this library bind to torch's copy.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_params(**kw):
+    p = TImageSynthParameters()
+    lib().setDefaultParams(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, _PARAM_ALIASES.get(k, k), v)
+    return p
+
+
+_PARAM_ALIASES = {"tileH": "isMakeSeamlesslyTileableHorizontally", "tileV": "isMakeSeamlesslyTileableVertically",
+                  "sensitivity": "sensitivityToOutliers"}
+
+
+def params_from_dict(d):
+    return default_params(**d)
+
+
+def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False):
+    return ResynthOptions(mode, prng, seed, batch, int(bool(timing)))
+
+
+def format_indices(fmt):
+    ix = TFormatIndices()
+    err = lib().prepareImageFormatIndicesFromFormatType(ctypes.byref(ix), fmt)
+    return err, ix
+
+
+def prepare_indices(colors, maps, alpha_t, alpha_s, is_map):
+    ix = TFormatIndices()
+    lib().prepareImageFormatIndices(ctypes.byref(ix), colors, maps, int(alpha_t), int(alpha_s), int(is_map))
+    return ix
+
+
+class Result:
+    def __init__(self, err, stats, progress):
+        self.err = err
+        self.stats = stats.as_dict()
+        self.progress = progress
+
+    def __getattr__(self, k):
+        try:
+            return self.__dict__["stats"][k]
+        except KeyError:
+            raise AttributeError(k)
+
+
+def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, mask2=None,
+                width=None, mask_width=None, mask2_width=None, seed=0, batch=0, timing=False):
+    """imageSynth / imageSynth2 (lib/imageSynth.c:62-217) on the GPU.
+
+    image: uint8 (H, W, C) or a padded row buffer (H, rowBytes) with `width` given;
+    mask (and mask2): uint8 (H, W) or padded rows with the width given.
+    Returns (Result, image_out)."""
+    img = np.array(image, dtype=np.uint8, copy=True, order="C")
+    bpp = int(lib().countPixelelsPerPixelForFormat(fmt)) or 1
+    if img.ndim == 3:
+        h, w = img.shape[0], img.shape[1]
+        row = img.shape[1] * img.shape[2]
+    else:
+        h, row = img.shape
+        w = width if width is not None else row // bpp
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    ib = ImageBuffer(img.ctypes.data, w, h, row)
+    mb = ImageBuffer(m.ctypes.data, mask_width if mask_width is not None else m.shape[1], m.shape[0], m.shape[1])
+    mb2p = None
+    if mask2 is not None:
+        m2 = np.ascontiguousarray(mask2, dtype=np.uint8)
+        mb2 = ImageBuffer(m2.ctypes.data, mask2_width if mask2_width is not None else m2.shape[1], m2.shape[0],
+                          m2.shape[1])
+        mb2p = ctypes.byref(mb2)
+    log = []
+    cb = PROGRESS(lambda pct, ctx: log.append(pct))
+    cf = ctypes.c_int(cancel)
+    st = ResynthStats()
+    opt = options(mode, prng, seed, batch, timing)
+    pp = ctypes.byref(params) if params is not None else None
+    err = lib().imageSynthEx(ctypes.byref(ib), ctypes.byref(mb), mb2p, fmt, pp, cb, None, ctypes.byref(cf),
+                             ctypes.byref(opt), ctypes.byref(st))
+    return Result(err, st, log), img
+
+
+def engine(params, ix, target_pixmap, corpus_pixmap, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, seed=0,
+           batch=0, timing=False):
+    """engine() (lib/engine.c:539-690) on engine-layout pixmaps uint8 (H, W, depth).
+    Returns (Result, target_out)."""
+    t = np.array(target_pixmap, dtype=np.uint8, copy=True, order="C")
+    c = np.ascontiguousarray(corpus_pixmap, dtype=np.uint8)
+    ta = GArray(t.ctypes.data, 0)
+    ca = GArray(c.ctypes.data, 0)
+    tm = Map(t.shape[1], t.shape[0], t.shape[2], ctypes.pointer(ta))
+    cm = Map(c.shape[1], c.shape[0], c.shape[2], ctypes.pointer(ca))
+    log = []
+    cb = PROGRESS(lambda pct, ctx: log.append(pct))
+    cf = ctypes.c_int(cancel)
+    st = ResynthStats()
+    opt = options(mode, prng, seed, batch, timing)
+    err = lib().engineEx(params, ctypes.byref(ix), ctypes.byref(tm), ctypes.byref(cm), cb, None, ctypes.byref(cf),
+                         ctypes.byref(opt), ctypes.byref(st))
+    return Result(err, st, log), t
+
+
+def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, fmt, params=None,
+                       mode=MODE_BATCHED, prng=PRNG_GRAND, d_mask2=None, mask2_row_bytes=0, seed=0, batch=0,
+                       timing=False, cancel=0):
+    """imageSynth on device-resident buffers (integer device addresses, e.g. torch data_ptr()).
+    Synthesizes d_image in place.  Returns a Result."""
+    log = []
+    cb = PROGRESS(lambda pct, ctx: log.append(pct))
+    cf = ctypes.c_int(cancel)
+    st = ResynthStats()
+    opt = options(mode, prng, seed, batch, timing)
+    pp = ctypes.byref(params) if params is not None else None
+    err = lib().resynth_image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, d_mask2,
+                                           mask2_row_bytes, fmt, pp, cb, None, ctypes.byref(cf), ctypes.byref(opt),
+                                           ctypes.byref(st))
+    return Result(err, st, log)
+
+
+def score_frozen(corpus_pixmap, ix, T, Tm, nb_counts, nb_offsets, nb_pixels, cand_counts, cands):
+    """GPU frozen-state scorer.  corpus_pixmap uint8 (H, W, depth); nb_offsets int32 (P, 64, 2);
+    nb_pixels uint8 (P, 64, 8); cands int32 (P, M, 2).  Returns (winner, bestSum, stopped)."""
+    cp = np.ascontiguousarray(corpus_pixmap, dtype=np.uint8)
+    P = len(nb_counts)
+    nbc = np.ascontiguousarray(nb_counts, dtype=np.uint32)
+    nbo = np.ascontiguousarray(nb_offsets, dtype=np.int32)
+    nbp = np.ascontiguousarray(nb_pixels, dtype=np.uint8)
+    cc = np.ascontiguousarray(cand_counts, dtype=np.uint32)
+    cd = np.ascontiguousarray(cands, dtype=np.int32)
+    T = np.ascontiguousarray(T, dtype=np.uint16)
+    Tm = np.ascontiguousarray(Tm, dtype=np.uint32)
+    win = np.zeros(P, np.int32)
+    best = np.zeros(P, np.uint32)
+    stop = np.zeros(P, np.int32)
+    rc = lib().resynth_score_frozen(cp.ctypes.data, cp.shape[1], cp.shape[0], ctypes.byref(ix), T.ctypes.data,
+                                    Tm.ctypes.data, P, nbc.ctypes.data, nbo.ctypes.data, nbp.ctypes.data,
+                                    cc.ctypes.data, cd.shape[1], cd.ctypes.data, win.ctypes.data, best.ctypes.data,
+                                    stop.ctypes.data)
+    if rc != 0:
+        raise ValueError("resynth_score_frozen rejected its arguments")
+    return win, best, stop
+
+
+def prng_sample(kind, seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().resynth_prng_sample(kind, seed, n, out.ctypes.data)
+    return out
+
+
+def prng_ranges(kind, seed, dists):
+    d = np.ascontiguousarray(dists, dtype=np.uint32)
+    out = np.zeros(len(d), np.uint32)
+    lib().resynth_prng_ranges(kind, seed, len(d), d.ctypes.data, out.ctypes.data)
+    return out
+
+
+def metric_tables(sensitivity=0.117, map_weight=0.5):
+    T = np.zeros(512, np.uint16)
+    Tm = np.zeros(512, np.uint32)
+    lib().resynth_metric_tables(sensitivity, map_weight, T.ctypes.data, Tm.ctypes.data)
+    return T, Tm
+
+
+def order_targets(match_type, points, prng=PRNG_GRAND):
+    pts = np.ascontiguousarray(points, dtype=np.int32).copy()
+    err = lib().resynth_order_targets(match_type, prng, pts.ctypes.data, len(pts))
+    return err, pts
+
+
+def device_info():
+    name = ctypes.create_string_buffer(64)
+    cus = ctypes.c_int(0)
+    rc = lib().resynth_device_info(name, 64, ctypes.byref(cus))
+    return rc, name.value.decode(), cus.value

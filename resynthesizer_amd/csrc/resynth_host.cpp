@@ -1,0 +1,1146 @@
+// Host side of libresynth_mi355x.so: parameter checks, the reference's data contract,
+// metric tables, the GLib / libc PRNG streams, target ordering, the pass loop with the
+// reference's progress and cancel semantics, and the C ABI (include/resynth_mi355x.h).
+// All image-sized work runs on the GPU (resynth_kernels.hip).
+#include "../../include/resynth_mi355x.h"
+#include "resynth_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace rs;
+
+namespace {
+
+constexpr unsigned kMaxNeighbors = 64;        // lib/imageSynthConstants.h:26
+constexpr uint32_t kEngineSeed = 1198472u;     // lib/engine.c:643
+constexpr unsigned kCallbackMask = 4095u;      // lib/imageSynthConstants.h:99
+
+double now_ms()
+{
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIP_TRY(expr)                                                                                 \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) {                                                                           \
+      std::fprintf(stderr, "resynth_mi355x: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_),  \
+                   __FILE__, __LINE__);                                                               \
+      std::abort();                                                                                   \
+    }                                                                                                 \
+  } while (0)
+
+// ------------------------------------------------------------------ PRNG streams
+
+// GLib GRand: MT19937 seeded by g_rand_set_seed, tempered g_rand_int, and the rejection
+// rule of g_rand_int_range (GLib >= 2.2).
+struct MT19937 {
+  uint32_t s[624];
+  int i = 624;
+  explicit MT19937(uint32_t seed)
+  {
+    s[0] = seed;
+    for (int k = 1; k < 624; k++) s[k] = 1812433253u * (s[k - 1] ^ (s[k - 1] >> 30)) + (uint32_t)k;
+  }
+  void regenerate()
+  {
+    auto mixbits = [](uint32_t hi, uint32_t lo) {
+      uint32_t y = (hi & 0x80000000u) | (lo & 0x7FFFFFFFu);
+      return (y >> 1) ^ ((y & 1u) ? 0x9908B0DFu : 0u);
+    };
+    for (int k = 0; k < 624; k++) s[k] = s[(k + 397) % 624] ^ mixbits(s[k], s[(k + 1) % 624]);
+    i = 0;
+  }
+  uint32_t next()
+  {
+    if (i >= 624) regenerate();
+    uint32_t y = s[i++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9D2C5680u;
+    y ^= (y << 15) & 0xEFC60000u;
+    y ^= y >> 18;
+    return y;
+  }
+};
+
+// The C library's rand() of glibc (TYPE_3 additive feedback, r[i] = r[i-3] + r[i-31]),
+// the PRNG of the stand-alone build (lib/glibProxy.c:30-49).
+struct LibcRand {
+  // ring of the last 31 raw words o[n-31..n-1]; the next word is o[n-31] + o[n-3]
+  int32_t r[31];
+  unsigned n = 0;
+  explicit LibcRand(uint32_t seed)
+  {
+    if (seed == 0) seed = 1;
+    int32_t o[344];
+    o[0] = (int32_t)seed;
+    for (int k = 1; k < 31; k++) {
+      long long v = (16807LL * o[k - 1]) % 2147483647LL;
+      if (v < 0) v += 2147483647LL;
+      o[k] = (int32_t)v;
+    }
+    for (int k = 31; k < 34; k++) o[k] = o[k - 31];
+    for (int k = 34; k < 344; k++) o[k] = (int32_t)((uint32_t)o[k - 31] + (uint32_t)o[k - 3]);
+    for (int k = 0; k < 31; k++) r[k] = o[313 + k];
+  }
+  uint32_t next()
+  {
+    int32_t v = (int32_t)((uint32_t)r[n % 31] + (uint32_t)r[(n + 28) % 31]);
+    r[n % 31] = v;
+    n++;
+    return (uint32_t)v >> 1;
+  }
+};
+
+struct Prng {
+  int kind;
+  MT19937 mt;
+  LibcRand lc;
+  Prng(int k, uint32_t seed) : kind(k), mt(seed), lc(seed) {}
+  uint32_t raw() { return kind == RESYNTH_PRNG_LIBC ? lc.next() : mt.next(); }
+  // int_range(0, n) as the engine's PRNG computes it
+  uint32_t range(uint32_t n)
+  {
+    if (kind == RESYNTH_PRNG_LIBC) {
+      if (n < 1) return 0;
+      return lc.next() / (2147483647u / n + 1u);
+    }
+    if (n == 0) return 0;
+    uint32_t maxvalue;
+    if (n <= 0x80000000u) {
+      uint32_t leftover = (0x80000000u % n) * 2u;
+      if (leftover >= n) leftover -= n;
+      maxvalue = 0xFFFFFFFFu - leftover;
+    } else {
+      maxvalue = n - 1u;
+    }
+    uint32_t v;
+    do v = mt.next(); while (v > maxvalue);
+    return v % n;
+  }
+};
+
+// ------------------------------------------------------------------ tables
+
+// lib/matchWeighting.h:50-60, 143-159, 167-178: the parameters arrive as float.
+void quantize_tables(double sensitivity, double mapWeight, Tables &t)
+{
+  const float c = (float)sensitivity;
+  const float mw = (float)mapWeight;
+  const float scale = c * 256.0f;
+  const double top = std::log(((double)256 / scale) * ((double)256 / scale) + 1.0);
+  for (int i = -256; i < 256; i++) {
+    const double num = std::log(((double)i / scale) * ((double)i / scale) + 1.0);
+    t.T[256 + i] = (uint16_t)(num / top * (double)(float)65535);
+    t.Tm[256 + i] = (uint32_t)((float)(i * i) * mw * 4.0);
+  }
+}
+
+// ------------------------------------------------------------------ ordering
+
+// Top-down merge sort taking the left element iff cmp(left, right) <= 0: the sort behind
+// g_array_sort, whose order matters here because the engine's comparators never return 0.
+template <typename T, typename Cmp>
+void merge_sort(std::vector<T> &v, Cmp cmp)
+{
+  std::vector<T> tmp(v.size());
+  struct Rec {
+    static void run(T *b, size_t n, T *t, Cmp &cmp)
+    {
+      if (n <= 1) return;
+      size_t n1 = n / 2, n2 = n - n1;
+      run(b, n1, t, cmp);
+      run(b + n1, n2, t, cmp);
+      T *l = b, *r = b + n1, *o = t;
+      while (n1 && n2) {
+        if (cmp(*l, *r) <= 0) { *o++ = *l++; n1--; }
+        else { *o++ = *r++; n2--; }
+      }
+      while (n1) { *o++ = *l++; n1--; }
+      std::copy(t, o, b);
+    }
+  };
+  if (!v.empty()) Rec::run(v.data(), v.size(), tmp.data(), cmp);
+}
+
+struct Pt {
+  int32_t x, y;
+};
+struct SortPt {
+  int32_t x, y;
+  float prop;
+};
+
+// lib/orderTarget.h:58-79
+void randomize_bands(std::vector<Pt> &pts, Prng &prng)
+{
+  const int n = (int)pts.size();
+  const int last = n - 1;
+  const int half = (int)(n * 0.1);
+  for (int i = 0; i <= last; i++) {
+    int start = std::max(i - half, 0);
+    int end = std::min(i + half, last);
+    int j = start + (int)prng.range((uint32_t)(end - start));
+    std::swap(pts[i], pts[j]);
+  }
+}
+
+// lib/orderTarget.h:269-343 for types 2..8 (types 0/1 are a plain index permutation,
+// see order_random).  Returns the error of an out-of-range type.
+int order_directional(int type, std::vector<Pt> &pts, Prng &prng)
+{
+  int32_t ulx = INT32_MAX, uly = INT32_MAX, lrx = 0, lry = 0;   // lib/engineTypes.h:208-226
+  for (auto &p : pts) {
+    ulx = std::min(ulx, p.x); uly = std::min(uly, p.y);
+    lrx = std::max(lrx, p.x); lry = std::max(lry, p.y);
+  }
+  const int32_t cx = (lrx - ulx) / 2 + ulx, cy = (lry - uly) / 2 + uly;
+  for (auto &p : pts) { p.x -= cx; p.y -= cy; }
+  auto lessOf = [](bool less) { return less ? -1 : 1; };
+  auto invert = [](int r) { return r == 1 ? -1 : 1; };
+  if (type == 2 || type == 5 || type == 8) {
+    // lib/brushfire.h:63-148: proportion of the distance to the farthest point on the ray
+    auto grad = [](int32_t x, int32_t y) {
+      return (unsigned)(std::atan2((double)(float)y, (double)(float)x) * 200 / 3.14159265358979323846 + 200);
+    };
+    unsigned maxc[401] = {0};
+    for (auto &p : pts) {
+      unsigned c = (unsigned)(p.x * p.x + p.y * p.y);
+      unsigned r = grad(p.x, p.y);
+      if (c > maxc[r]) maxc[r] = c;
+    }
+    std::vector<SortPt> s(pts.size());
+    for (size_t i = 0; i < pts.size(); i++)
+      s[i] = {pts[i].x, pts[i].y, (float)(pts[i].y * pts[i].y + pts[i].x * pts[i].x) / (float)maxc[grad(pts[i].x, pts[i].y)]};
+    if (type == 5) merge_sort(s, [&](const SortPt &a, const SortPt &b) { return lessOf(a.prop < b.prop); });
+    else merge_sort(s, [&](const SortPt &a, const SortPt &b) { return invert(lessOf(a.prop < b.prop)); });
+    // type 8 (lib/orderTarget.h:209-262) overwrites its interleaving with this same order
+    for (size_t i = 0; i < pts.size(); i++) pts[i] = {s[i].x, s[i].y};
+  } else {
+    bool horiz = (type == 3 || type == 6);
+    bool more = (type == 3 || type == 4);
+    merge_sort(pts, [&](const Pt &a, const Pt &b) {
+      int r = horiz ? lessOf(a.x * a.x < b.x * b.x) : lessOf(a.y * a.y < b.y * b.y);
+      return more ? invert(r) : r;
+    });
+  }
+  for (auto &p : pts) { p.x += cx; p.y += cy; }
+  randomize_bands(pts, prng);
+  return 0;
+}
+
+// ------------------------------------------------------------------ device state
+
+struct Buf {
+  void *p = nullptr;
+  size_t cap = 0;
+  void *get(size_t bytes)
+  {
+    if (bytes > cap) {
+      if (p) HIP_TRY(hipFree(p));
+      size_t c = std::max(bytes, cap + cap / 2);
+      HIP_TRY(hipMalloc(&p, c));
+      cap = c;
+    }
+    return p;
+  }
+};
+
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  void *get(size_t bytes)
+  {
+    if (bytes > cap) {
+      if (p) HIP_TRY(hipHostFree(p));
+      size_t c = std::max(bytes, cap + cap / 2);
+      HIP_TRY(hipHostMalloc(&p, c, hipHostMallocDefault));
+      cap = c;
+    }
+    return p;
+  }
+};
+
+struct DeviceState {
+  bool ready = false;
+  bool ok = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // workspaces
+  Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
+      tables, counters, compactTemp, perm, streamBuf, streamState, meanSum, image, mask, mask2, tpix, cpix, tileHas,
+      firstIdx;
+  PinnedBuf pinCounts, pinTables, pinPerm, pinStream;
+  std::vector<hipEvent_t> events;
+};
+
+std::mutex g_mutex;
+DeviceState g_dev;
+int g_defaultMode = -1;
+
+bool ensure_device()
+{
+  if (g_dev.ready) return g_dev.ok;
+  g_dev.ready = true;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    std::fprintf(stderr, "resynth_mi355x: no HIP device is available\n");
+    return false;
+  }
+  HIP_TRY(hipGetDevice(&g_dev.device));
+  HIP_TRY(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+  g_dev.ok = true;
+  return true;
+}
+
+int default_mode()
+{
+  if (g_defaultMode >= 0) return g_defaultMode;
+  const char *m = std::getenv("RESYNTH_MODE");
+  if (m && (std::strcmp(m, "serial") == 0 || std::strcmp(m, "0") == 0)) return RESYNTH_MODE_SERIAL;
+  return RESYNTH_MODE_BATCHED;
+}
+
+// ------------------------------------------------------------------ engine job
+
+// Where the two images come from.
+struct Source {
+  // SimpleAPI form (device pointers)
+  bool simple = false;
+  const unsigned char *image = nullptr;   // device
+  size_t rowBytes = 0;
+  unsigned bpp = 0;
+  const unsigned char *mask = nullptr;    // device, image dims
+  size_t maskRowBytes = 0;
+  const unsigned char *mask2 = nullptr;   // device or null (inverse of mask)
+  size_t mask2RowBytes = 0;
+  int alphaIdx = -1;                      // index of alpha within a pixel, -1 if none
+  // engine-layout form (device pixmaps)
+  const unsigned char *tpix = nullptr;
+  const unsigned char *cpix = nullptr;
+  int tw = 0, th = 0, cw = 0, ch = 0;
+};
+
+struct Progress {
+  void (*cb)(int, void *) = nullptr;
+  void *ctx = nullptr;
+  int *cancel = nullptr;
+  unsigned estimated = 0, completed = 0, prior = 0;
+  // lib/progress.c:53-65
+  void tick()
+  {
+    completed += kCallbackMask;
+    unsigned pct = (unsigned)(((float)completed / (float)estimated) * 100);
+    if (pct > prior) {
+      if (cb) cb((int)pct, ctx);
+      prior = pct;
+    }
+  }
+  bool cancelled() const { return cancel && *cancel; }
+};
+
+struct Job {
+  TImageSynthParameters P;
+  TFormatIndices ix;
+  Source src;
+  ResynthOptions opt;
+  ResynthStats st;
+  Progress prog;
+  bool wide = false;
+  int nColors = 0, nMaps = 0;
+};
+
+void record_event(Job &job, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev, bool start)
+{
+  if (!job.opt.timing) return;
+  hipEvent_t e;
+  HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventRecord(e, g_dev.stream));
+  if (start) ev.push_back({e, nullptr});
+  else ev.back().second = e;
+}
+
+// The engine (lib/engine.c:539-690) on device buffers.  Returns a TImageSynthError;
+// the synthesized colours are in the target words (g_dev.tword) on success.
+int run_engine(Job &job, unsigned &nTargetsOut)
+{
+  hipStream_t s = g_dev.stream;
+  const TImageSynthParameters &P = job.P;
+  const TFormatIndices &ix = job.ix;
+  const double t0 = now_ms();
+  if (P.patchSize > kMaxNeighbors) return IMAGE_SYNTH_ERROR_PATCH_SIZE_EXCEEDED;
+
+  Source &S = job.src;
+  const int tw = S.tw, th = S.th, cw = S.cw, ch = S.ch;
+  const size_t nTpix = (size_t)tw * th, nCpix = (size_t)cw * ch;
+  job.nColors = ix.img_match_bpp;
+  job.nMaps = ix.map_match_bpp;
+  job.wide = job.nColors + job.nMaps > 3;
+  const size_t wb = job.wide ? 8 : 4;
+  const int useContext = P.matchContextType != 0;
+  const bool serialMode = job.opt.mode == RESYNTH_MODE_SERIAL;
+
+  void *tword = g_dev.tword.get(nTpix * wb);
+  void *cword = g_dev.cword.get(std::max(nCpix, nTpix) * wb);
+  uint8_t *tflag = (uint8_t *)g_dev.tflag.get(nTpix + 64);
+  uint8_t *cflag = (uint8_t *)g_dev.cflag.get(std::max(nCpix, nTpix) + 64);
+  if (S.simple) {
+    HIP_TRY(launch_adapt_simple(s, job.wide, S.image, S.rowBytes, S.bpp, S.mask, S.maskRowBytes, S.mask2,
+                                S.mask2RowBytes, tw, th, job.nColors, S.alphaIdx, useContext, tword, cword, tflag,
+                                cflag));
+  } else {
+    HIP_TRY(launch_adapt_maps(s, job.wide, S.tpix, tw, th, ix.total_bpp, ix.colorEndBip, ix.map_start_bip,
+                              ix.map_end_bip, ix.isAlphaTarget ? ix.alpha_bip : -1, useContext, 1, tword, tflag));
+    HIP_TRY(launch_adapt_maps(s, job.wide, S.cpix, cw, ch, ix.total_bpp, ix.colorEndBip, ix.map_start_bip,
+                              ix.map_end_bip, ix.isAlphaSource ? ix.alpha_bip : -1, useContext, 0, cword, cflag));
+  }
+  // row-major target and corpus point lists (lib/engine.c:338-431)
+  int2 *targetsRM = (int2 *)g_dev.targetsRM.get(nTpix * sizeof(int2) + 64);
+  int2 *corpusPts = (int2 *)g_dev.corpusPts.get(nCpix * sizeof(int2) + 64);
+  size_t ctb = std::max(compact_temp_bytes(nTpix), compact_temp_bytes(nCpix));
+  char *ctemp = (char *)g_dev.compactTemp.get(2 * ctb + 256);
+  unsigned *dCounts = (unsigned *)(ctemp + 2 * ctb);
+  HIP_TRY(launch_compact(s, tflag, tw, th, targetsRM, dCounts + 0, ctemp));
+  HIP_TRY(launch_compact(s, cflag, cw, ch, corpusPts, dCounts + 1, ctemp + ctb));
+  unsigned *hCounts = (unsigned *)g_dev.pinCounts.get(64);
+  HIP_TRY(hipMemcpyAsync(hCounts, dCounts, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  hipEvent_t countsReady;
+  HIP_TRY(hipEventCreateWithFlags(&countsReady, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(countsReady, s));
+
+  // Work that does not depend on the counts goes to the device now and overlaps the
+  // host's target ordering: the sorted offsets (lib/engine.c:465-497), the metric tables
+  // and the state maps.
+  const int ow = std::min(cw, tw), oh = std::min(ch, th);
+  const size_t nO = (size_t)(2 * ow - 1) * (size_t)(2 * oh - 1);
+  job.st.nOffsets = (unsigned)nO;
+  int2 *offsets = (int2 *)g_dev.offsets.get(nO * sizeof(int2));
+  size_t otb = offsets_temp_bytes(nO);
+  HIP_TRY(launch_sorted_offsets(s, ow, oh, offsets, g_dev.offTemp.get(otb), otb));
+  Tables *hT = (Tables *)g_dev.pinTables.get(sizeof(Tables));
+  quantize_tables(P.sensitivityToOutliers, P.mapWeight, *hT);
+  Tables *dT = (Tables *)g_dev.tables.get(sizeof(Tables));
+  HIP_TRY(hipMemcpyAsync(dT, hT, sizeof(Tables), hipMemcpyHostToDevice, s));
+  int2 *sourceOf = (int2 *)g_dev.sourceOf.get(nTpix * sizeof(int2));
+  uint32_t *prober = (uint32_t *)g_dev.prober.get(nCpix * sizeof(uint32_t));
+  uint32_t *lastBest = (uint32_t *)g_dev.lastBest.get(nTpix * sizeof(uint32_t));
+  Counters *cnt = (Counters *)g_dev.counters.get(sizeof(Counters));
+  HIP_TRY(launch_fill_u32(s, (uint32_t *)sourceOf, 0xFFFFFFFFu, nTpix * 2));
+  HIP_TRY(launch_fill_u32(s, prober, 0xFFFFFFFFu, nCpix));
+  HIP_TRY(launch_fill_u32(s, lastBest, 0xFFFFFFFFu, nTpix));
+  HIP_TRY(launch_fill_u32(s, (uint32_t *)cnt, 0u, sizeof(Counters) / 4));
+  // occupancy of 8x8 tiles and the start of each distance in the sorted offsets, used to
+  // skip provably empty offsets in pass-0 scans
+  const int tilesW = (tw + kTile - 1) / kTile, tilesH = (th + kTile - 1) / kTile;
+  const size_t nTiles = (size_t)tilesW * tilesH;
+  uint8_t *tileHas = (uint8_t *)g_dev.tileHas.get((nTiles + 3) / 4 * 4);
+  HIP_TRY(launch_fill_u32(s, (uint32_t *)tileHas, 0u, (nTiles + 3) / 4));
+  HIP_TRY(launch_tile_init(s, job.wide, tword, tw, th, tileHas));
+  const unsigned rhoMax = (unsigned)std::ceil(std::sqrt((double)(ow - 1) * (ow - 1) + (double)(oh - 1) * (oh - 1))) + 1;
+  uint32_t *firstIdx = (uint32_t *)g_dev.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
+  HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
+
+  HIP_TRY(hipEventSynchronize(countsReady));
+  HIP_TRY(hipEventDestroy(countsReady));
+  const unsigned nT = hCounts[0], nC = hCounts[1];
+  job.st.nTargets = nT;
+  job.st.nCorpus = nC;
+  nTargetsOut = nT;
+  if (!nT || !nC) {
+    HIP_TRY(hipStreamSynchronize(s));
+    return !nT ? IMAGE_SYNTH_ERROR_EMPTY_TARGET : IMAGE_SYNTH_ERROR_EMPTY_CORPUS;
+  }
+
+  // target order (lib/orderTarget.h:269-343) with the engine's PRNG
+  // the engine's seed (lib/engine.c:643); a nonzero option seed gives the serial mode
+  // another stream, for statistics over seeds
+  Prng prng(job.opt.prng, (serialMode && job.opt.seed) ? job.opt.seed : kEngineSeed);
+  int2 *targets = (int2 *)g_dev.targets.get((size_t)nT * sizeof(int2) + 64);
+  const int type = P.matchContextType;
+  if (type == 0 || type == 1) {
+    std::vector<uint32_t> perm(nT);
+    for (unsigned i = 0; i < nT; i++) perm[i] = i;
+    for (unsigned i = 0; i < nT; i++) {
+      uint32_t j = prng.range(nT);
+      std::swap(perm[i], perm[j]);
+    }
+    uint32_t *hPerm = (uint32_t *)g_dev.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
+    std::memcpy(hPerm, perm.data(), (size_t)nT * sizeof(uint32_t));
+    uint32_t *dPerm = (uint32_t *)g_dev.perm.get((size_t)nT * sizeof(uint32_t) + 64);
+    HIP_TRY(hipMemcpyAsync(dPerm, hPerm, (size_t)nT * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_gather_targets(s, targetsRM, dPerm, targets, nT));
+  } else if (type >= 2 && type <= 8) {
+    std::vector<Pt> pts(nT);
+    HIP_TRY(hipMemcpyAsync(pts.data(), targetsRM, (size_t)nT * sizeof(int2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    order_directional(type, pts, prng);
+    Pt *hp = (Pt *)g_dev.pinPerm.get((size_t)nT * sizeof(Pt) + 64);
+    std::copy(pts.begin(), pts.end(), hp);
+    HIP_TRY(hipMemcpyAsync(targets, hp, (size_t)nT * sizeof(int2), hipMemcpyHostToDevice, s));
+  } else {
+    HIP_TRY(hipStreamSynchronize(s));
+    return IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE;
+  }
+
+  Geometry g = {};
+  g.tw = tw; g.th = th; g.cw = cw; g.ch = ch;
+  g.nColors = job.nColors;
+  g.nMaps = job.nMaps;
+  g.tileH = P.isMakeSeamlesslyTileableHorizontally != 0;
+  g.tileV = P.isMakeSeamlesslyTileableVertically != 0;
+  g.patchLimit = std::max(P.patchSize, 2u);
+  g.maxProbe = P.maxProbeCount;
+  g.nT = nT; g.nC = nC; g.nO = (unsigned)nO;
+  g.penalty = 65535u * (uint32_t)job.nColors + hT->Tm[0] * (uint32_t)job.nMaps;
+  g.tilesW = tilesW;
+  g.tilesH = tilesH;
+  g.rhoMax = rhoMax;
+  g.skipEmpty = 0;
+
+  SynthBuffers b = {};
+  b.tword = tword;
+  b.sourceOf = sourceOf;
+  b.cword = cword;
+  b.prober = prober;
+  b.targets = targets;
+  b.corpusPts = corpusPts;
+  b.offsets = offsets;
+  b.lastBest = lastBest;
+  b.tables = dT;
+  b.counters = cnt;
+  b.tileHas = tileHas;
+  b.firstIdx = firstIdx;
+
+  // pass schedule (lib/passes.h:68-93) and progress estimate (lib/progress.c:31-37)
+  unsigned sizes[6];
+  {
+    unsigned n = nT;
+    sizes[0] = n;
+    for (int i = 1; i < 6; i++) { sizes[i] = n; n = (unsigned)n * 3 / 4; }
+  }
+  job.prog.estimated = 0;
+  for (int i = 0; i < 6; i++) job.prog.estimated += sizes[i];
+  HIP_TRY(hipStreamSynchronize(s));
+  const double t1 = now_ms();
+  job.st.prepMs = t1 - t0;
+
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  unsigned launches = 0;
+  const bool serial = job.opt.mode == RESYNTH_MODE_SERIAL;
+
+  // serial mode: raw PRNG outputs after the ordering draws, streamed to the device
+  // raw outputs buffered on the device; RESYNTH_SERIAL_STREAM_CAP shrinks the buffer so
+  // that tests exercise the refill path
+  uint32_t streamCap = 1u << 22;
+  if (const char *capEnv = std::getenv("RESYNTH_SERIAL_STREAM_CAP")) {
+    long v = std::atol(capEnv);
+    if (v >= 1024) streamCap = (uint32_t)std::min<long>(v, 1L << 30);
+  }
+  uint32_t *dStream = nullptr, *hStream = nullptr;
+  StreamState *dState = nullptr, *hState = nullptr;
+  uint32_t streamLen = 0, streamPos = 0;
+  if (serial) {
+    dStream = (uint32_t *)g_dev.streamBuf.get((size_t)streamCap * 4);
+    hStream = (uint32_t *)g_dev.pinStream.get((size_t)streamCap * 4 + sizeof(StreamState) * 2);
+    hState = (StreamState *)(hStream + streamCap);
+    dState = (StreamState *)g_dev.streamState.get(sizeof(StreamState));
+  }
+  auto refill = [&]() {
+    // keep the unconsumed tail, append fresh outputs, upload
+    uint32_t tail = streamLen - streamPos;
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memmove(hStream, hStream + streamPos, (size_t)tail * 4);
+    for (uint32_t i = tail; i < streamCap; i++) hStream[i] = prng.raw();
+    streamLen = streamCap;
+    streamPos = 0;
+    HIP_TRY(hipMemcpyAsync(dStream, hStream, (size_t)streamCap * 4, hipMemcpyHostToDevice, s));
+    hState->pos = 0;
+    hState->kDone = 0;
+    hState->aborted = 0;
+    HIP_TRY(hipMemcpyAsync(dState, hState, sizeof(StreamState), hipMemcpyHostToDevice, s));
+  };
+  if (serial) refill();
+
+  // Batched schedule.  Concurrent visits do not see each other's results, so a chunk is
+  // kept small relative to what the pass has already written: in pass 0 a chunk holds 3%
+  // of the targets visited so far (at least 1), later passes use the cap, nT/32 clamped
+  // to [64, 16384] unless the caller sets it.
+  const unsigned cap = job.opt.batch ? job.opt.batch : std::min(16384u, std::max(64u, nT / 32u));
+  unsigned pass;
+  unsigned passesRun = 0;
+  unsigned bettersBefore = 0;
+  for (pass = 0; pass < 6; pass++) {
+    const unsigned end = sizes[pass];
+    // every target has a value after pass 0; scans can skip empty rings only in pass 0,
+    // and wrapped neighbourhoods (tiling) are not covered by the tile walk
+    g.skipEmpty = (pass == 0 && !g.tileH && !g.tileV) ? 1 : 0;
+    // lib/synthesize.h:493-497: progress + cancel poll before target k when k & 4095 == 0
+    unsigned k = 0;
+    bool stop = false;
+    while (k < end && !stop) {
+      unsigned k1;
+      if (serial) {
+        k1 = std::min(end, (k & ~kCallbackMask) + 4096u);
+      } else {
+        unsigned len = (pass == 0) ? std::max(1u, (unsigned)((unsigned long long)k * 30u / 1000u)) : cap;
+        len = std::min(len, cap);
+        k1 = std::min(end, k + len);
+      }
+      // progress points inside [k, k1)
+      unsigned firstPoint = (k + kCallbackMask) & ~kCallbackMask;
+      for (unsigned pt = firstPoint; pt < k1; pt += 4096u) {
+        job.prog.tick();
+        if (job.prog.cancelled()) {
+          k1 = pt;
+          stop = true;
+          break;
+        }
+      }
+      if (k1 > k) {
+        if (serial) {
+          unsigned kk = k;
+          while (kk < k1) {
+            if (streamLen - streamPos < streamCap / 2) refill();
+            record_event(job, ev, true);
+            HIP_TRY(launch_synth_serial(s, job.wide, g, b, kk, k1, job.opt.prng, dStream, streamLen, dState));
+            record_event(job, ev, false);
+            launches++;
+            HIP_TRY(hipMemcpyAsync(hState, dState, sizeof(StreamState), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            streamPos = hState->pos;
+            if (hState->kDone == kk && streamPos == 0) {
+              // one visit needs more draws than the stream holds: grow it
+              if (streamCap >= (1u << 30)) {
+                std::fprintf(stderr, "resynth_mi355x: maxProbeCount too large for the serial stream\n");
+                std::abort();
+              }
+              std::vector<uint32_t> keep(hStream, hStream + streamLen);
+              streamCap *= 2;
+              dStream = (uint32_t *)g_dev.streamBuf.get((size_t)streamCap * 4);
+              hStream = (uint32_t *)g_dev.pinStream.get((size_t)streamCap * 4 + sizeof(StreamState) * 2);
+              hState = (StreamState *)(hStream + streamCap);
+              std::copy(keep.begin(), keep.end(), hStream);
+            }
+            kk = hState->kDone;
+            if (kk < k1) refill();   // stopped for want of PRNG outputs
+          }
+        } else {
+          record_event(job, ev, true);
+          HIP_TRY(launch_synth_batched(s, job.wide, g, b, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
+          record_event(job, ev, false);
+          launches++;
+        }
+      }
+      k = k1;
+    }
+    Counters hc;
+    HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    unsigned bettersNow = 0;
+    for (int i = 0; i < CNT_SLOTS; i++) bettersNow += hc.betters[i];
+    const unsigned betters = bettersNow - bettersBefore;
+    bettersBefore = bettersNow;
+    job.st.passSize[pass] = end;
+    job.st.betters[pass] = betters;
+    passesRun = pass + 1;
+    job.st.visits += k;
+    // lib/refiner.h:111-115
+    if ((double)((float)betters / (float)nT) < 0.1) break;
+  }
+  job.st.passes = passesRun;
+  job.st.launches = launches;
+
+  // statistics
+  {
+    Counters hc;
+    unsigned long long *dSum = (unsigned long long *)g_dev.meanSum.get(16);
+    HIP_TRY(launch_fill_u32(s, (uint32_t *)dSum, 0u, 4));
+    HIP_TRY(launch_mean_error(s, lastBest, targets, nT, tw, dSum));
+    unsigned long long hs[2];
+    HIP_TRY(hipMemcpyAsync(hs, dSum, 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    job.st.meanError = hs[1] ? (double)hs[0] / (double)hs[1] : 0.0;
+    for (int i = 0; i < CNT_SLOTS; i++) {
+      job.st.candidates += hc.candidates[i];
+      job.st.patchTerms += hc.terms[i];
+    }
+    double ms = 0;
+    for (auto &e : ev) {
+      float f = 0;
+      HIP_TRY(hipEventElapsedTime(&f, e.first, e.second));
+      ms += f;
+      HIP_TRY(hipEventDestroy(e.first));
+      HIP_TRY(hipEventDestroy(e.second));
+    }
+    job.st.synthKernelMs = ms;
+    job.st.pixelWordBytes = (unsigned)wb;
+  }
+  job.st.synthMs = now_ms() - t1;
+  return 0;
+}
+
+void init_job(Job &job, const TImageSynthParameters *params, const ResynthOptions *opt, void (*cb)(int, void *),
+              void *ctx, int *cancel)
+{
+  if (params) job.P = *params;
+  else setDefaultParams(&job.P);
+  if (opt) job.opt = *opt;
+  else resynth_default_options(&job.opt);
+  std::memset(&job.st, 0, sizeof job.st);
+  job.prog.cb = cb;
+  job.prog.ctx = ctx;
+  job.prog.cancel = cancel;
+}
+
+// Builds the corpus mask of imageSynth2 as lib/adaptSimple.h:48-96 fills it: mask2's own
+// width x height bytes, in its own row order, written consecutively into an image-sized
+// bytemap that starts out all zero.
+bool remap_mask2(const ImageBuffer *image, const ImageBuffer *mask2, std::vector<unsigned char> &out)
+{
+  const size_t n = (size_t)image->width * image->height;
+  if ((size_t)mask2->width * mask2->height > n) return false;
+  out.assign(n, 0);
+  size_t d = 0;
+  for (unsigned r = 0; r < mask2->height; r++)
+    for (unsigned c = 0; c < mask2->width; c++) out[d++] = mask2->data[(size_t)r * mask2->rowBytes + c];
+  return true;
+}
+
+int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat format,
+                     TImageSynthParameters *params, void (*cb)(int, void *), void *ctx, int *cancelFlag,
+                     const ResynthOptions *opt, ResynthStats *stats)
+{
+  // lib/imageSynth.c:62-137 / 140-217
+  if (image->width != mask->width || image->height != mask->height) return IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
+  TFormatIndices ix;
+  if (prepareImageFormatIndicesFromFormatType(&ix, format)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+  std::vector<unsigned char> m2;
+  if (mask2 && !remap_mask2(image, mask2, m2)) return IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  const double t0 = now_ms();
+  hipStream_t s = g_dev.stream;
+  const unsigned w = image->width, h = image->height;
+  const unsigned bpp = countPixelelsPerPixelForFormat(format);
+  Job job;
+  init_job(job, params, opt, cb, ctx, cancelFlag);
+  job.ix = ix;
+  Source &S = job.src;
+  S.simple = true;
+  S.tw = S.cw = (int)w;
+  S.th = S.ch = (int)h;
+  S.bpp = bpp;
+  S.rowBytes = (size_t)w * bpp;
+  S.maskRowBytes = w;
+  S.mask2RowBytes = w;
+  S.alphaIdx = (format == T_RGBA) ? 3 : (format == T_GrayA) ? 1 : -1;
+  const size_t npx = (size_t)w * h;
+  unsigned char *dImg = (unsigned char *)g_dev.image.get(npx * bpp + 64);
+  unsigned char *dMask = (unsigned char *)g_dev.mask.get(npx + 64);
+  if (npx) {
+    HIP_TRY(hipMemcpy2DAsync(dImg, S.rowBytes, image->data, image->rowBytes, S.rowBytes, h, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy2DAsync(dMask, w, mask->data, mask->rowBytes, w, h, hipMemcpyHostToDevice, s));
+  }
+  S.image = dImg;
+  S.mask = dMask;
+  if (mask2) {
+    unsigned char *dM2 = (unsigned char *)g_dev.mask2.get(npx + 64);
+    HIP_TRY(hipMemcpyAsync(dM2, m2.data(), npx, hipMemcpyHostToDevice, s));
+    S.mask2 = dM2;
+  }
+  unsigned nT = 0;
+  int err = npx ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
+  if (!err && !(cancelFlag && *cancelFlag)) {
+    HIP_TRY(launch_writeback_simple(s, job.wide, g_dev.tword.p, (const int2 *)g_dev.targets.p, nT, dImg,
+                                    S.rowBytes, bpp, job.nColors, (int)w));
+    HIP_TRY(hipMemcpy2DAsync(image->data, image->rowBytes, dImg, S.rowBytes, S.rowBytes, h, hipMemcpyDeviceToHost,
+                             s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  job.st.totalMs = now_ms() - t0;
+  if (stats) *stats = job.st;
+  return err;
+}
+
+int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+                void (*cb)(int, void *), void *ctx, int *cancelFlag, const ResynthOptions *opt, ResynthStats *stats)
+{
+  if (parameters.patchSize > kMaxNeighbors) return IMAGE_SYNTH_ERROR_PATCH_SIZE_EXCEEDED;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  const double t0 = now_ms();
+  hipStream_t s = g_dev.stream;
+  Job job;
+  init_job(job, &parameters, opt, cb, ctx, cancelFlag);
+  job.ix = *indices;
+  Source &S = job.src;
+  S.tw = (int)targetMap->width;
+  S.th = (int)targetMap->height;
+  S.cw = (int)corpusMap->width;
+  S.ch = (int)corpusMap->height;
+  const size_t depth = indices->total_bpp;
+  const size_t tbytes = (size_t)S.tw * S.th * depth, cbytes = (size_t)S.cw * S.ch * depth;
+  unsigned char *dT = (unsigned char *)g_dev.tpix.get(tbytes + 64);
+  unsigned char *dC = (unsigned char *)g_dev.cpix.get(cbytes + 64);
+  if (tbytes) HIP_TRY(hipMemcpyAsync(dT, targetMap->data->data, tbytes, hipMemcpyHostToDevice, s));
+  if (cbytes) HIP_TRY(hipMemcpyAsync(dC, corpusMap->data->data, cbytes, hipMemcpyHostToDevice, s));
+  S.tpix = dT;
+  S.cpix = dC;
+  unsigned nT = 0;
+  int err = (tbytes && cbytes) ? run_engine(job, nT)
+                               : (tbytes ? IMAGE_SYNTH_ERROR_EMPTY_CORPUS : IMAGE_SYNTH_ERROR_EMPTY_TARGET);
+  if (!err) {
+    // lib/engine.c:675-676: results are left in targetMap
+    HIP_TRY(launch_writeback_pixmap(s, job.wide, g_dev.tword.p, (const int2 *)g_dev.targets.p, nT, dT, (int)depth,
+                                    job.nColors, S.tw));
+    HIP_TRY(hipMemcpyAsync(targetMap->data->data, dT, tbytes, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  job.st.totalMs = now_ms() - t0;
+  if (stats) *stats = job.st;
+  return err;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+void setDefaultParams(TImageSynthParameters *param)
+{
+  // lib/engineParams.c:9-18
+  param->isMakeSeamlesslyTileableHorizontally = 0;
+  param->isMakeSeamlesslyTileableVertically = 0;
+  param->matchContextType = 1;
+  param->mapWeight = 0.5;
+  param->sensitivityToOutliers = 0.117;
+  param->patchSize = 30;
+  param->maxProbeCount = 200;
+}
+
+unsigned int countPixelelsPerPixelForFormat(TImageFormat format)
+{
+  switch (format) {
+  case T_RGB: return 3;
+  case T_RGBA: return 4;
+  case T_Gray: return 1;
+  case T_GrayA: return 2;
+  default: return 0;
+  }
+}
+
+void prepareImageFormatIndices(TFormatIndices *ix, unsigned int colors, unsigned int maps, int alphaTarget,
+                               int alphaSource, int isMap)
+{
+  // lib/imageFormat.c:117-207: [mask][colours][alpha if either image has one][maps]
+  ix->img_match_bpp = (TPixelelIndex)colors;
+  ix->colorEndBip = (TPixelelIndex)(1 + colors);
+  if (alphaTarget || alphaSource) {
+    ix->alpha_bip = ix->colorEndBip;
+    ix->map_start_bip = (TPixelelIndex)(ix->colorEndBip + 1);
+  } else {
+    ix->alpha_bip = 0;
+    ix->map_start_bip = ix->colorEndBip;
+  }
+  ix->map_match_bpp = isMap ? (TPixelelIndex)maps : 0;
+  ix->map_end_bip = (TPixelelIndex)(ix->map_start_bip + ix->map_match_bpp);
+  ix->total_bpp = ix->map_end_bip;
+  ix->isAlphaTarget = alphaTarget;
+  ix->isAlphaSource = alphaSource;
+}
+
+int prepareImageFormatIndicesFromFormatType(TFormatIndices *ix, TImageFormat format)
+{
+  switch (format) {
+  case T_RGB: prepareImageFormatIndices(ix, 3, 0, 0, 0, 0); return 0;
+  case T_RGBA: prepareImageFormatIndices(ix, 3, 0, 1, 1, 0); return 0;
+  case T_Gray: prepareImageFormatIndices(ix, 1, 0, 0, 0, 0); return 0;
+  case T_GrayA: prepareImageFormatIndices(ix, 1, 0, 1, 1, 0); return 0;
+  default: return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+  }
+}
+
+void prepareDefaultFormatIndices(TFormatIndices *ix)
+{
+  // lib/imageFormat.c:209-240: RGBA, no maps
+  ix->img_match_bpp = 3;
+  ix->colorEndBip = 4;
+  ix->alpha_bip = 4;
+  ix->map_match_bpp = 0;
+  ix->map_start_bip = 5;
+  ix->map_end_bip = 5;
+  ix->total_bpp = 5;
+  ix->isAlphaTarget = 1;
+  ix->isAlphaSource = 1;
+}
+
+// ---- maps (lib/mapOps.h:45-168) ----
+void free_map(Map *map)
+{
+  if (map->data) {
+    std::free(map->data->data);
+    std::free(map->data);
+  }
+  map->data = nullptr;
+}
+
+static void new_map(Map *map, unsigned w, unsigned h, unsigned depth, unsigned eltSize)
+{
+  map->width = w;
+  map->height = h;
+  map->depth = depth;
+  map->data = (GArray *)std::calloc(1, sizeof(GArray));
+  size_t bytes = (size_t)w * h * eltSize;
+  map->data->data = (char *)std::calloc(bytes ? bytes : 1, 1);
+  map->data->len = 0;
+}
+
+void new_pixmap(Map *map, unsigned int w, unsigned int h, unsigned int depth) { new_map(map, w, h, depth, depth); }
+void new_bytemap(Map *map, unsigned int w, unsigned int h) { new_map(map, w, h, 1, 1); }
+void new_intmap(Map *map, unsigned int w, unsigned int h) { new_map(map, w, h, sizeof(unsigned), sizeof(unsigned)); }
+void new_coordmap(Map *map, unsigned int w, unsigned int h)
+{
+  new_map(map, w, h, sizeof(Coordinates), sizeof(Coordinates));
+}
+
+void set_bytemap(Map *map, unsigned char value)
+{
+  std::memset(map->data->data, value, (size_t)map->width * map->height);
+}
+
+void invert_bytemap(Map *map)
+{
+  unsigned char *d = (unsigned char *)map->data->data;
+  for (size_t i = 0; i < (size_t)map->width * map->height; i++) d[i] = (unsigned char)~d[i];
+}
+
+void interleave_mask(Map *pixmap, Map *mask)
+{
+  size_t n = (size_t)pixmap->width * pixmap->height;
+  unsigned char *p = (unsigned char *)pixmap->data->data;
+  const unsigned char *m = (const unsigned char *)mask->data->data;
+  for (size_t i = 0; i < n; i++) p[i * pixmap->depth] = m[i * mask->depth];
+}
+
+// ---- engine entries ----
+void resynth_set_default_mode(int mode) { g_defaultMode = mode; }
+
+void resynth_default_options(ResynthOptions *opt)
+{
+  std::memset(opt, 0, sizeof *opt);
+  opt->mode = default_mode();
+  opt->prng = RESYNTH_PRNG_GRAND;
+}
+
+int imageSynth(ImageBuffer *imageBuffer, ImageBuffer *mask, TImageFormat imageFormat,
+               TImageSynthParameters *parameters, void (*progressCallback)(int, void *), void *contextInfo,
+               int *cancelFlag)
+{
+  return image_synth_host(imageBuffer, mask, nullptr, imageFormat, parameters, progressCallback, contextInfo,
+                          cancelFlag, nullptr, nullptr);
+}
+
+int imageSynth2(ImageBuffer *imageBuffer, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat imageFormat,
+                TImageSynthParameters *parameters, void (*progressCallback)(int, void *), void *contextInfo,
+                int *cancelFlag)
+{
+  return image_synth_host(imageBuffer, mask, mask2, imageFormat, parameters, progressCallback, contextInfo,
+                          cancelFlag, nullptr, nullptr);
+}
+
+int imageSynthEx(ImageBuffer *imageBuffer, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat imageFormat,
+                 TImageSynthParameters *parameters, void (*progressCallback)(int, void *), void *contextInfo,
+                 int *cancelFlag, const ResynthOptions *options, ResynthStats *stats)
+{
+  return image_synth_host(imageBuffer, mask, mask2, imageFormat, parameters, progressCallback, contextInfo,
+                          cancelFlag, options, stats);
+}
+
+int engine(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+           void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag)
+{
+  return engine_host(parameters, indices, targetMap, corpusMap, progressCallback, contextInfo, cancelFlag, nullptr,
+                     nullptr);
+}
+
+int engineEx(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+             void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
+             const ResynthOptions *options, ResynthStats *stats)
+{
+  return engine_host(parameters, indices, targetMap, corpusMap, progressCallback, contextInfo, cancelFlag, options,
+                     stats);
+}
+
+int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsigned int height, size_t rowBytes,
+                               const unsigned char *d_mask, size_t maskRowBytes, const unsigned char *d_mask2,
+                               size_t mask2RowBytes, TImageFormat imageFormat,
+                               const TImageSynthParameters *parameters, void (*progressCallback)(int, void *),
+                               void *contextInfo, int *cancelFlag, const ResynthOptions *options,
+                               ResynthStats *stats)
+{
+  TFormatIndices ix;
+  if (prepareImageFormatIndicesFromFormatType(&ix, imageFormat)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  const double t0 = now_ms();
+  Job job;
+  init_job(job, parameters, options, progressCallback, contextInfo, cancelFlag);
+  job.ix = ix;
+  Source &S = job.src;
+  S.simple = true;
+  S.tw = S.cw = (int)width;
+  S.th = S.ch = (int)height;
+  S.bpp = countPixelelsPerPixelForFormat(imageFormat);
+  S.image = d_image;
+  S.rowBytes = rowBytes;
+  S.mask = d_mask;
+  S.maskRowBytes = maskRowBytes;
+  S.mask2 = d_mask2;
+  S.mask2RowBytes = mask2RowBytes;
+  S.alphaIdx = (imageFormat == T_RGBA) ? 3 : (imageFormat == T_GrayA) ? 1 : -1;
+  unsigned nT = 0;
+  int err = ((size_t)width * height) ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
+  if (!err && !(cancelFlag && *cancelFlag))
+    HIP_TRY(launch_writeback_simple(g_dev.stream, job.wide, g_dev.tword.p, (const int2 *)g_dev.targets.p, nT,
+                                    d_image, rowBytes, S.bpp, job.nColors, (int)width));
+  HIP_TRY(hipStreamSynchronize(g_dev.stream));
+  job.st.totalMs = now_ms() - t0;
+  if (stats) *stats = job.st;
+  return err;
+}
+
+int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned int ch,
+                         const TFormatIndices *indices, const uint16_t *T, const uint32_t *Tm,
+                         unsigned int nProblems, const unsigned int *nbCount, const int32_t *nbOffset,
+                         const unsigned char *nbPixel, const unsigned int *candCount, unsigned int maxCands,
+                         const int32_t *cands, int32_t *winner, uint32_t *bestSum, int32_t *stopped)
+{
+  if (!corpus || !indices || !T || !Tm || !nbCount || !nbOffset || !nbPixel || !candCount || !cands || !winner ||
+      !bestSum || !stopped)
+    return -1;
+  for (unsigned i = 0; i < nProblems; i++)
+    if (nbCount[i] > 64 || candCount[i] > maxCands) return -1;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  hipStream_t s = g_dev.stream;
+  const int nColors = indices->img_match_bpp, nMaps = indices->map_match_bpp;
+  const bool wide = nColors + nMaps > 3;
+  const size_t wb = wide ? 8 : 4;
+  const int depth = indices->total_bpp;
+  const size_t ncp = (size_t)cw * ch, nnb = (size_t)nProblems * 64;
+  // corpus pixmap and neighbour pixels -> words on the device
+  unsigned char *dPix = (unsigned char *)g_dev.cpix.get(ncp * depth + nnb * 8 + 64);
+  unsigned char *dNbPix = dPix + ncp * depth;
+  HIP_TRY(hipMemcpyAsync(dPix, corpus, ncp * depth, hipMemcpyHostToDevice, s));
+  // neighbour pixels arrive as 8-byte records; repack them to `depth` bytes each
+  std::vector<unsigned char> nb(nnb * depth);
+  for (size_t i = 0; i < nnb; i++) std::memcpy(&nb[i * depth], nbPixel + i * 8, depth);
+  HIP_TRY(hipMemcpyAsync(dNbPix, nb.data(), nb.size(), hipMemcpyHostToDevice, s));
+  void *dC = g_dev.cword.get(ncp * wb + 64);
+  void *dNbW = g_dev.tword.get(nnb * wb + 64);
+  HIP_TRY(launch_pack_words(s, wide, dPix, (int)ncp, depth, indices->colorEndBip, indices->map_start_bip,
+                            indices->map_end_bip, 1, dC));
+  HIP_TRY(launch_pack_words(s, wide, dNbPix, (int)nnb, depth, indices->colorEndBip, indices->map_start_bip,
+                            indices->map_end_bip, 0, dNbW));
+  Tables ht;
+  std::memcpy(ht.T, T, sizeof ht.T);
+  std::memcpy(ht.Tm, Tm, sizeof ht.Tm);
+  Tables *dT = (Tables *)g_dev.tables.get(sizeof(Tables));
+  HIP_TRY(hipMemcpyAsync(dT, &ht, sizeof ht, hipMemcpyHostToDevice, s));
+  size_t szCnt = nProblems * sizeof(unsigned), szOff = nnb * sizeof(int2);
+  size_t szCand = (size_t)nProblems * maxCands * sizeof(int2), szOut = nProblems * 4;
+  char *dArgs = (char *)g_dev.compactTemp.get(2 * szCnt + szOff + szCand + 3 * szOut + 512);
+  unsigned *dNbCount = (unsigned *)dArgs;
+  unsigned *dCandCount = (unsigned *)(dArgs + szCnt);
+  int2 *dOff = (int2 *)(dArgs + 2 * szCnt);
+  int2 *dCands = (int2 *)((char *)dOff + szOff);
+  int32_t *dWin = (int32_t *)((char *)dCands + szCand);
+  uint32_t *dBest = (uint32_t *)((char *)dWin + szOut);
+  int32_t *dStop = (int32_t *)((char *)dBest + szOut);
+  HIP_TRY(hipMemcpyAsync(dNbCount, nbCount, szCnt, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dCandCount, candCount, szCnt, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dOff, nbOffset, szOff, hipMemcpyHostToDevice, s));
+  if (szCand) HIP_TRY(hipMemcpyAsync(dCands, cands, szCand, hipMemcpyHostToDevice, s));
+  const uint32_t penalty = 65535u * (uint32_t)nColors + Tm[0] * (uint32_t)nMaps;
+  HIP_TRY(launch_score_frozen(s, wide, (int)cw, (int)ch, nColors, nMaps, penalty, dC, dT, nProblems, dNbCount, dOff,
+                              dNbW, dCandCount, maxCands, dCands, dWin, dBest, dStop));
+  HIP_TRY(hipMemcpyAsync(winner, dWin, szOut, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(bestSum, dBest, szOut, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(stopped, dStop, szOut, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int resynth_device_info(char *name, size_t nameLen, int *computeUnits)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, g_dev.device) != hipSuccess) return -1;
+  if (name && nameLen) {
+    std::snprintf(name, nameLen, "%s", p.gcnArchName);
+  }
+  if (computeUnits) *computeUnits = p.multiProcessorCount;
+  return std::strncmp(p.gcnArchName, "gfx950", 6) == 0 ? 0 : 1;
+}
+
+const char *resynth_version(void) { return "resynth_mi355x 0.1 (gfx950)"; }
+
+// ---- host-side helpers (no device work) ----
+void resynth_prng_sample(int prng, unsigned int seed, unsigned int n, uint32_t *out)
+{
+  Prng p(prng, seed);
+  for (unsigned i = 0; i < n; i++) out[i] = p.raw();
+}
+
+void resynth_prng_ranges(int prng, unsigned int seed, unsigned int n, const uint32_t *dists, uint32_t *out)
+{
+  Prng p(prng, seed);
+  for (unsigned i = 0; i < n; i++) out[i] = p.range(dists[i]);
+}
+
+void resynth_metric_tables(double sensitivity, double mapWeight, uint16_t *T, uint32_t *Tm)
+{
+  Tables t;
+  quantize_tables(sensitivity, mapWeight, t);
+  std::memcpy(T, t.T, sizeof t.T);
+  std::memcpy(Tm, t.Tm, sizeof t.Tm);
+}
+
+int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsigned int n)
+{
+  Prng p(prng, kEngineSeed);
+  std::vector<Pt> pts(n);
+  for (unsigned i = 0; i < n; i++) pts[i] = {points[2 * i], points[2 * i + 1]};
+  if (matchContextType == 0 || matchContextType == 1) {
+    std::vector<uint32_t> perm(n);
+    for (unsigned i = 0; i < n; i++) perm[i] = i;
+    for (unsigned i = 0; i < n; i++) std::swap(perm[i], perm[p.range(n)]);
+    for (unsigned i = 0; i < n; i++) {
+      points[2 * i] = pts[perm[i]].x;
+      points[2 * i + 1] = pts[perm[i]].y;
+    }
+    return 0;
+  }
+  if (matchContextType < 0 || matchContextType > 8) return IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE;
+  order_directional(matchContextType, pts, p);
+  for (unsigned i = 0; i < n; i++) {
+    points[2 * i] = pts[i].x;
+    points[2 * i + 1] = pts[i].y;
+  }
+  return 0;
+}
+
+}  // extern "C"

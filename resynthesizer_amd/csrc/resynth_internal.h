@@ -1,0 +1,106 @@
+// Internal declarations shared by the host driver (resynth_host.cpp) and the gfx950
+// kernels (resynth_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace rs {
+
+// Device pixel word.  Byte 0 is a flag byte; bytes 1.. hold the matched channels,
+// colours first, then map channels, each one byte (the engine's Pixelel).
+//   target word: flag bit 0 = hasValue (lib/engine.c:152-168)
+//   corpus word: flag byte 0xFF = selected in the corpus mask (lib/engine.c:247-258)
+// A word is 32 bits when colours+maps <= 3 and 64 bits otherwise, so every pixel is
+// read and written with one aligned access: a reader sees a pixel's value and its
+// hasValue flag from the same store.
+
+struct Geometry {
+  int tw, th;             // target image
+  int cw, ch;             // corpus image
+  int nColors, nMaps;     // img_match_bpp, map_match_bpp
+  int tileH, tileV;
+  unsigned patchLimit;    // neighbours gathered: max(patchSize, 2), <= 64
+  unsigned maxProbe;      // maxProbeCount
+  unsigned nT, nC, nO;    // targets, corpus points, offsets
+  uint32_t penalty;       // 65535*nColors + Tm[0]*nMaps: score of a clipped/masked neighbour
+  int tilesW, tilesH;     // 8x8-pixel tiles of the target image
+  unsigned rhoMax;        // largest radius indexed by SynthBuffers::firstIdx
+  int skipEmpty;          // pass 0 without tiling: start scans past provably empty rings
+};
+
+constexpr int kTile = 8;
+
+struct Tables {
+  uint16_t T[512];
+  uint32_t Tm[512];
+};
+
+// Counters written by the synthesis kernels; 64 slots each to spread atomics.
+enum { CNT_SLOTS = 64 };
+struct Counters {
+  unsigned long long candidates[CNT_SLOTS];
+  unsigned long long terms[CNT_SLOTS];
+  unsigned int betters[CNT_SLOTS];
+};
+
+struct SynthBuffers {
+  void *tword;            // [th*tw] Word
+  int2 *sourceOf;         // [th*tw]
+  const void *cword;      // [ch*cw] Word
+  uint32_t *prober;       // [ch*cw] recent prober (lib/engine.c:308-327)
+  const int2 *targets;    // [nT] in synthesis order
+  const int2 *corpusPts;  // [nC] row-major
+  const int2 *offsets;    // [nO] sorted offsets
+  uint32_t *lastBest;     // [th*tw]
+  const Tables *tables;
+  Counters *counters;
+  uint8_t *tileHas;       // [tilesH*tilesW] 1 when the tile holds a pixel with a value
+  const uint32_t *firstIdx; // [rhoMax+1] number of sorted offsets with x^2+y^2 < rho^2
+};
+
+// State of the exact (serial) PRNG stream consumed on the device.
+struct StreamState {
+  uint32_t pos;           // next raw output to consume
+  uint32_t kDone;         // first visit not completed by the last launch
+  uint32_t aborted;       // visit kDone needs more raw outputs than the buffer holds
+  uint32_t pad;
+};
+
+// ---- kernel launchers (resynth_kernels.hip) ----
+hipError_t launch_adapt_simple(hipStream_t s, bool wide, const unsigned char *image, size_t rowBytes, unsigned bpp,
+                               const unsigned char *mask, size_t maskRowBytes, const unsigned char *mask2,
+                               size_t mask2RowBytes, int w, int h, int nColors, int alphaIdx, int useContext,
+                               void *tword, void *cword, uint8_t *tflag, uint8_t *cflag);
+hipError_t launch_adapt_maps(hipStream_t s, bool wide, const unsigned char *pix, int w, int h, int depth,
+                             int colorEnd, int mapStart, int mapEnd, int alphaIdx, int useContext, int isTarget,
+                             void *word, uint8_t *flag);
+size_t compact_temp_bytes(size_t n);
+hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int2 *out, unsigned *count,
+                          void *temp);
+size_t offsets_temp_bytes(size_t nO);
+hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes);
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas);
+hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx);
+hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);
+hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                                unsigned k1, unsigned seed, unsigned pass);
+hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                               unsigned k1, int prngKind, const uint32_t *stream, uint32_t streamLen,
+                               StreamState *state);
+hipError_t launch_writeback_simple(hipStream_t s, bool wide, const void *tword, const int2 *targets, unsigned nT,
+                                   unsigned char *image, size_t rowBytes, unsigned bpp, int nColors, int w);
+hipError_t launch_writeback_pixmap(hipStream_t s, bool wide, const void *tword, const int2 *targets, unsigned nT,
+                                   unsigned char *pix, int depth, int nColors, int w);
+hipError_t launch_mean_error(hipStream_t s, const uint32_t *lastBest, const int2 *targets, unsigned nT, int w,
+                             unsigned long long *sumCount);
+hipError_t launch_score_frozen(hipStream_t s, bool wide, int cw, int ch, int nColors, int nMaps, uint32_t penalty,
+                               const void *cword, const Tables *tables, unsigned nProblems, const unsigned *nbCount,
+                               const int2 *nbOffset, const void *nbWord, const unsigned *candCount,
+                               unsigned maxCands, const int2 *cands, int32_t *winner, uint32_t *bestSum,
+                               int32_t *stopped);
+hipError_t launch_pack_words(hipStream_t s, bool wide, const unsigned char *pix, int n, int depth, int colorEnd,
+                             int mapStart, int mapEnd, int corpusFlags, void *out);
+
+}  // namespace rs

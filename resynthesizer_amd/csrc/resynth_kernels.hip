@@ -1,0 +1,1343 @@
+// gfx950 kernels of the synthesis engine: image adaptation, point compaction, sorted
+// offsets, the per-target visit (neighbour gather + candidate scoring) in its serial
+// and batched forms, write-back and statistics.
+//
+// Every wavefront (64 lanes) owns one target visit at a time.  The neighbour patch
+// (<= 64 entries) is gathered by a ballot/prefix-count scan over the sorted offsets and
+// staged in LDS; candidates are scored one per lane against it, each lane gathering
+// one corpus pixel word per neighbour, with the metric tables in LDS.  The sequential
+// strict-less update of lib/synthesize.h:266-400 is recovered exactly from the per-lane
+// (final sum, max prefix sum) pairs by a ballot loop over the lanes in candidate order.
+#include "resynth_internal.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+
+namespace rs {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename Word>
+__device__ __forceinline__ uint32_t chan(Word w, int c)
+{
+  return (uint32_t)(w >> (8 * (1 + c))) & 0xFFu;
+}
+
+// Score contribution of one in-corpus neighbour (lib/synthesize.h:328-355): colours
+// except for neighbour 0 (the target itself), maps for every neighbour.
+template <typename Word>
+__device__ __forceinline__ uint32_t neighbour_term(Word t, Word q, bool first, int nColors, int nMaps,
+                                                   const uint16_t *T, const uint32_t *Tm)
+{
+  uint32_t s = 0;
+  if (!first) {
+    for (int c = 0; c < nColors; c++) s += T[256u + chan(t, c) - chan(q, c)];
+  }
+  for (int m = 0; m < nMaps; m++) s += Tm[256u + chan(t, nColors + m) - chan(q, nColors + m)];
+  return s;
+}
+
+// Accumulates neighbours [base, base+G) of candidate (cx,cy): the G corpus words are
+// loaded first, then summed in neighbour order with the early out below.
+template <typename Word, int G>
+__device__ __forceinline__ void score_group(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
+                                            unsigned base, unsigned count, const int2 *nbOff, const Word *nbPix,
+                                            const uint16_t *T, const uint32_t *Tm, uint32_t B0, uint32_t &acc,
+                                            uint32_t &mx, bool &dead)
+{
+  Word q[G];
+  bool in[G];
+#pragma unroll
+  for (int u = 0; u < G; u++) {
+    unsigned i = base + u;
+    in[u] = false;
+    q[u] = 0;
+    if (i < count) {
+      int qx = cx + nbOff[i].x, qy = cy + nbOff[i].y;
+      if ((unsigned)qx < (unsigned)g.cw && (unsigned)qy < (unsigned)g.ch) {
+        q[u] = cword[(size_t)qy * (unsigned)g.cw + (unsigned)qx];
+        in[u] = true;
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < G; u++) {
+    unsigned i = base + u;
+    if (i < count && !dead) {
+      uint32_t term = (in[u] && (q[u] & 0xFFu) == 0xFFu)
+                          ? neighbour_term<Word>(nbPix[i], q[u], i == 0, g.nColors, g.nMaps, T, Tm)
+                          : g.penalty;
+      acc += term;
+      mx = acc > mx ? acc : mx;
+      if (acc >= B0) dead = true;
+    }
+  }
+}
+
+// Scores candidate (cx,cy) against the staged patch.  The lane stops (dead) at the first
+// prefix sum >= B0, where B0 is the best sum before this batch of candidates: such a
+// candidate cannot be accepted by the sequential update, whose best only decreases.
+// For survivors F is the full sum and M the largest prefix sum (equal to F unless the
+// u32 sum wrapped); the sequential rule accepts a survivor iff M < best-at-its-turn.
+// skip0: the candidate is known to be a selected in-bounds corpus pixel and there are no
+// map channels, so neighbour 0 (the candidate itself, colours not compared) adds 0; with
+// B0 > 0 its early-out test cannot fire either.  G0 is the size of the first group of
+// loads, which is where most random candidates die.
+template <typename Word, int G0 = 8>
+__device__ __forceinline__ void score_candidate(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
+                                                unsigned count, const int2 *nbOff, const Word *nbPix,
+                                                const uint16_t *T, const uint32_t *Tm, uint32_t B0, uint32_t &F,
+                                                uint32_t &M, bool &alive, bool skip0 = false)
+{
+  uint32_t acc = 0, mx = 0;
+  bool dead = false;
+  unsigned base = (skip0 && B0 > 0) ? 1u : 0u;
+  score_group<Word, G0>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+  for (base += G0; base < count && !dead; base += 8)
+    score_group<Word, 8>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+  F = acc;
+  M = mx;
+  alive = !dead;
+}
+
+// Sequential strict-less resolution over the lanes of one wave, in lane order.
+// On return best/win are updated; stop is the lane of the first perfect match or 64.
+__device__ __forceinline__ void resolve(bool cand, uint32_t M, uint32_t F, uint32_t &best, int &win, int &stop)
+{
+  uint64_t rem = __ballot(cand);
+  stop = 64;
+  while (rem) {
+    uint64_t m = __ballot(cand && M < best) & rem;
+    if (!m) break;
+    int i = __ffsll((unsigned long long)m) - 1;
+    best = __shfl(F, i);
+    win = i;
+    rem &= (i == 63) ? 0ull : ~((2ull << i) - 1ull);
+    if (best == 0) {
+      stop = i;
+      break;
+    }
+  }
+}
+
+// First sorted-offset index worth scanning for target (tx,ty): with R empty Chebyshev
+// rings of tiles around the target's tile, every pixel holding a value is at least 8R+1
+// pixels away, so the offsets with x^2+y^2 < (8R+1)^2 cannot hold one.  `lanes` threads
+// (a wave or a workgroup) share the ring walk; `anyTile` reduces over them.
+template <typename AnyFn>
+__device__ __forceinline__ unsigned scan_start(const Geometry &g, const uint8_t *tileHas, const uint32_t *firstIdx,
+                                               int tx, int ty, unsigned tid, unsigned lanes, AnyFn anyTile)
+{
+  const int TX = tx / kTile, TY = ty / kTile;
+  const int maxR = max(max(TX, g.tilesW - 1 - TX), max(TY, g.tilesH - 1 - TY));
+  int R = -1;   // number of empty rings found
+  for (int r = 0; r <= maxR; r++) {
+    const unsigned n = r == 0 ? 1u : 8u * (unsigned)r;
+    bool hit = false;
+    for (unsigned i0 = 0; i0 < n; i0 += lanes) {
+      bool mine = false;
+      unsigned i = i0 + tid;
+      if (i < n) {
+        int X, Y;
+        if (r == 0) { X = TX; Y = TY; }
+        else {
+          // walk the ring's perimeter: top row, right column, bottom row, left column
+          unsigned side = i / (2u * r), o = i % (2u * r);
+          if (side == 0) { X = TX - r + (int)o; Y = TY - r; }
+          else if (side == 1) { X = TX + r; Y = TY - r + (int)o; }
+          else if (side == 2) { X = TX + r - (int)o; Y = TY + r; }
+          else { X = TX - r; Y = TY + r - (int)o; }
+        }
+        if (X >= 0 && Y >= 0 && X < g.tilesW && Y < g.tilesH) mine = tileHas[(size_t)Y * g.tilesW + X] != 0;
+      }
+      if (anyTile(mine)) { hit = true; break; }
+    }
+    if (hit) break;
+    R = r;
+  }
+  if (R < 0) return 1u;
+  unsigned rho = (unsigned)(kTile * R + 1);
+  if (rho > g.rhoMax) return g.nO;
+  unsigned j = firstIdx[rho];
+  return j < 1u ? 1u : j;
+}
+
+// Neighbour patch of target (tx,ty) (lib/synthesize.h:189-241): neighbour 0 is the
+// target itself; then the sorted offsets are scanned 64 at a time, keeping points in
+// the target image (or wrapped into it when tiling) that have a value.
+template <typename Word>
+__device__ unsigned gather_neighbours(const Geometry &g, const Word *tword, const int2 *sourceOf, const int2 *offsets,
+                                      int tx, int ty, int2 *nbOff, Word *nbPix, int2 *nbSrc,
+                                      const uint8_t *tileHas = nullptr, const uint32_t *firstIdx = nullptr)
+{
+  const unsigned lane = lane_id();
+  const size_t tp = (size_t)ty * (unsigned)g.tw + (unsigned)tx;
+  if (lane == 0) {
+    nbOff[0] = make_int2(0, 0);
+    nbPix[0] = tword[tp];
+    nbSrc[0] = sourceOf[tp];
+  }
+  unsigned count = 1;
+  const unsigned L = g.patchLimit;
+  unsigned start = 1;
+  if (g.skipEmpty && tileHas)
+    start = scan_start(g, tileHas, firstIdx, tx, ty, lane, 64u, [](bool v) { return __ballot(v) != 0ull; });
+  for (unsigned base = start; base < g.nO && count < L; base += 64) {
+    unsigned j = base + lane;
+    bool v = false;
+    int2 off = make_int2(0, 0);
+    size_t p = 0;
+    Word w = 0;
+    if (j < g.nO) {
+      off = offsets[j];
+      int px = tx + off.x, py = ty + off.y;
+      bool in = true;
+      if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
+      else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
+      if (py < 0) { if (g.tileV) py += g.th; else in = false; }
+      else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
+      if (in) {
+        p = (size_t)py * (unsigned)g.tw + (unsigned)px;
+        w = tword[p];
+        v = (w & 1u) != 0;
+      }
+    }
+    uint64_t m = __ballot(v);
+    unsigned rank = __popcll(m & lanemask_lt());
+    if (v && count + rank < L) {
+      unsigned idx = count + rank;
+      nbOff[idx] = off;
+      nbPix[idx] = w;
+      nbSrc[idx] = sourceOf[p];
+    }
+    unsigned c2 = count + (unsigned)__popcll(m);
+    count = c2 < L ? c2 : L;
+  }
+  wave_sync();
+  return count;
+}
+
+// Heuristic 1 (lib/synthesize.h:537-580): lane i proposes source(neighbour i) - offset i.
+// Candidates clipped/masked in the corpus, already probed by this target index
+// (heuristic 2) or repeating an earlier lane's candidate are skipped; the rest are scored
+// in lane order.  Returns the lanes whose candidate must be recorded in the prober map.
+template <typename Word>
+__device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *__restrict__ cword,
+                                                const uint32_t *prober, unsigned k, unsigned count,
+                                                const int2 *nbOff, const Word *nbPix, const int2 *nbSrc,
+                                                const uint16_t *T, const uint32_t *Tm, uint32_t &best, int &winX,
+                                                int &winY, bool &perfect, bool &scored, bool &record, int &cx,
+                                                int &cy, unsigned &evals)
+{
+  const unsigned lane = lane_id();
+  bool valid = false;
+  cx = 0;
+  cy = 0;
+  if (lane < count) {
+    int2 src = nbSrc[lane];
+    if (src.x != -1) {
+      cx = src.x - nbOff[lane].x;
+      cy = src.y - nbOff[lane].y;
+      if ((unsigned)cx < (unsigned)g.cw && (unsigned)cy < (unsigned)g.ch) {
+        Word q = cword[(size_t)cy * (unsigned)g.cw + (unsigned)cx];
+        valid = (q & 0xFFu) == 0xFFu;
+      }
+    }
+  }
+  // earlier lane with the same candidate
+  bool dup = false;
+  uint64_t vmask = __ballot(valid);
+  if (__popcll(vmask) > 1) {
+    for (unsigned j = 0; j < count; j++) {
+      int ox = __shfl(cx, (int)j), oy = __shfl(cy, (int)j);
+      if (j < lane && ((vmask >> j) & 1ull) && ox == cx && oy == cy) dup = true;
+    }
+  }
+  bool active = valid && !dup;
+  if (active) active = prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] != k;
+  uint32_t F = 0, M = 0;
+  bool alive = false;
+  if (active)
+    score_candidate<Word>(g, cword, cx, cy, count, nbOff, nbPix, T, Tm, 0xFFFFFFFFu, F, M, alive, g.nMaps == 0);
+  uint64_t amask = __ballot(active);
+  evals += (unsigned)__popcll(amask);
+  int win = -1, stop = 64;
+  uint32_t b = best;
+  resolve(active && alive, M, F, b, win, stop);
+  if (win >= 0) {
+    best = b;
+    winX = __shfl(cx, win);
+    winY = __shfl(cy, win);
+    scored = true;
+  }
+  perfect = stop < 64;
+  record = active && (int)lane < stop;
+  // The evaluation count above includes lanes after a perfect match, which the
+  // reference never reaches; correct it.
+  if (perfect) evals -= (unsigned)__popcll(amask & ~((stop == 63) ? ~0ull : ((2ull << stop) - 1ull)));
+}
+
+// ------------------------------------------------------------------ workgroup visits
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Maps one raw PRNG output to a corpus index as the engine's int_range(0, nC) does:
+// GLib g_rand_int_range rejects outputs above maxvalue; the stand-alone build's
+// lo + rand()/(RAND_MAX/n + 1) (lib/glibProxy.c:37-49) accepts every output.
+__device__ __forceinline__ bool draw_index(int prngKind, uint32_t raw, uint32_t nC, uint32_t maxvalue, uint32_t &idx)
+{
+  if (prngKind == 1) {
+    idx = raw / (2147483647u / nC + 1u);
+    return true;
+  }
+  if (raw > maxvalue) return false;
+  idx = raw % nC;
+  return true;
+}
+
+// Shared state of one target visit by a workgroup of WAVES wavefronts.
+template <typename Word, int WAVES>
+struct VisitLDS {
+  uint16_t T[512];
+  uint32_t Tm[512];
+  int2 off[64];
+  Word pix[64];
+  int2 src[64];
+  uint32_t waveCnt[WAVES];
+  uint32_t candM[64 * WAVES], candF[64 * WAVES];
+  int2 cand[64 * WAVES];
+  uint8_t candOk[64 * WAVES];
+  uint8_t candUsed[64 * WAVES];
+  // visit result
+  uint32_t best;
+  int winX, winY;
+  int perfect, scored, aborted;
+  unsigned evals;
+  uint32_t streamPos;
+  uint32_t lastUsed;
+  const uint8_t *tileHas;
+  const uint32_t *firstIdx;
+};
+
+// Neighbour patch of target (tx,ty) (lib/synthesize.h:189-241) gathered by the whole
+// workgroup: neighbour 0 is the target itself; then the sorted offsets are scanned
+// 64*WAVES at a time, keeping in-image (or wrapped, when tiling) points that have a value.
+template <typename Word, int WAVES>
+__device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, const int2 *sourceOf,
+                                         const int2 *offsets, int tx, int ty, VisitLDS<Word, WAVES> &S)
+{
+  const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const size_t tp = (size_t)ty * (unsigned)g.tw + (unsigned)tx;
+  if (tid == 0) {
+    S.off[0] = make_int2(0, 0);
+    S.pix[0] = tword[tp];
+    S.src[0] = sourceOf[tp];
+  }
+  unsigned count = 1;
+  const unsigned L = g.patchLimit;
+  unsigned start = 1;
+  if (g.skipEmpty && S.tileHas)
+    start = scan_start(g, S.tileHas, S.firstIdx, tx, ty, tid, 64u * WAVES,
+                       [](bool v) { return __syncthreads_or(v) != 0; });
+  for (unsigned base = start; base < g.nO && count < L; base += 64u * WAVES) {
+    unsigned j = base + tid;
+    bool v = false;
+    int2 off = make_int2(0, 0);
+    size_t p = 0;
+    Word w = 0;
+    if (j < g.nO) {
+      off = offsets[j];
+      int px = tx + off.x, py = ty + off.y;
+      bool in = true;
+      if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
+      else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
+      if (py < 0) { if (g.tileV) py += g.th; else in = false; }
+      else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
+      if (in) {
+        p = (size_t)py * (unsigned)g.tw + (unsigned)px;
+        w = tword[p];
+        v = (w & 1u) != 0;
+      }
+    }
+    uint64_t m = __ballot(v);
+    if (lane == 0) S.waveCnt[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    unsigned before = 0, total = 0;
+#pragma unroll
+    for (int wv = 0; wv < WAVES; wv++) {
+      unsigned c = S.waveCnt[wv];
+      before += (wv < (int)wave) ? c : 0;
+      total += c;
+    }
+    unsigned idx = count + before + (unsigned)__popcll(m & lanemask_lt());
+    if (v && idx < L) {
+      S.off[idx] = off;
+      S.pix[idx] = w;
+      S.src[idx] = sourceOf[p];
+    }
+    count = (count + total < L) ? count + total : L;
+    __syncthreads();
+  }
+  return count;
+}
+
+// Sequential strict-less resolution, by wave 0, of the candidate slots S.cand*[0..nSlots)
+// in slot order.  Slots marked in candUsed were drawn; candOk marks survivors of the early
+// out.  On return candUsed keeps only the slots the reference would have scored (every
+// drawn slot up to and including a perfect match), S.best/winX/winY/scored/perfect are
+// updated, and lastUsed is the last such slot (-1 if none).
+template <typename Word, int WAVES>
+__device__ void resolve_slots_wave0(VisitLDS<Word, WAVES> &S, unsigned nSlots, int &lastUsed)
+{
+  const unsigned lane = lane_id();
+  uint32_t best = S.best;
+  bool perfect = false;
+  lastUsed = -1;
+  for (unsigned w = 0; w * 64 < nSlots; w++) {
+    unsigned slot = w * 64 + lane;
+    bool act = slot < nSlots && S.candUsed[slot] && !perfect;
+    bool ok = act && S.candOk[slot];
+    uint32_t M = ok ? S.candM[slot] : 0u, F = ok ? S.candF[slot] : 0u;
+    int win = -1, stop = 64;
+    resolve(ok, M, F, best, win, stop);
+    if (win >= 0 && lane == 0) {
+      int2 c = S.cand[w * 64 + win];
+      S.winX = c.x;
+      S.winY = c.y;
+      S.scored = 1;
+    }
+    uint64_t am = __ballot(act);
+    if (stop < 64) {
+      perfect = true;
+      am &= (stop == 63) ? ~0ull : ((2ull << stop) - 1ull);
+    }
+    if (slot < nSlots && !((am >> lane) & 1ull)) S.candUsed[slot] = 0;
+    if (am) lastUsed = (int)(w * 64 + 63 - __clzll((long long)am));
+  }
+  if (lane == 0) {
+    S.best = best;
+    S.perfect = perfect ? 1 : 0;
+  }
+}
+
+// One target visit (lib/synthesize.h:480-640) by a workgroup.  SERIAL draws the random
+// probes from the exact PRNG stream (prngKind, stream, streamLen, S.streamPos) and aborts
+// without side effects when the stream runs dry; otherwise probes come from a per-target
+// counter-based stream.  Global writes happen only in the commit at the end.
+template <typename Word, int WAVES, bool SERIAL>
+__device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word, WAVES> &S, unsigned k,
+                         unsigned seed, unsigned pass, int prngKind, const uint32_t *stream, uint32_t streamLen,
+                         uint32_t maxvalue, unsigned long long &evalTot, unsigned long long &termTot,
+                         unsigned &betTot)
+{
+  const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  Word *tword = (Word *)b.tword;
+  const Word *cword = (const Word *)b.cword;
+  const int2 t = b.targets[k];
+  const unsigned count = gather_neighbours_wg<Word, WAVES>(g, tword, b.sourceOf, b.offsets, t.x, t.y, S);
+
+  // heuristic 1 on wave 0
+  bool record = false;
+  int cx = 0, cy = 0;
+  if (wave == 0) {
+    uint32_t best = 0xFFFFFFFFu;
+    int winX = 0, winY = 0;
+    bool perfect = false, scored = false;
+    unsigned evals = 0;
+    coherence_phase<Word>(g, cword, b.prober, k, count, S.off, S.pix, S.src, S.T, S.Tm, best, winX, winY, perfect,
+                          scored, record, cx, cy, evals);
+    if (lane == 0) {
+      S.best = best;
+      S.winX = winX;
+      S.winY = winY;
+      S.perfect = perfect;
+      S.scored = scored;
+      S.evals = evals;
+      S.aborted = 0;
+    }
+  }
+  __syncthreads();
+
+  // random probes, 64*WAVES per round
+  unsigned drawn = 0;
+  const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
+  while (!S.perfect && drawn < g.maxProbe) {
+    bool active = false;
+    uint32_t idx = 0;
+    const uint32_t pos = SERIAL ? S.streamPos : 0u;
+    if (SERIAL) {
+      if (pos + 64u * WAVES > streamLen) {
+        __syncthreads();
+        if (tid == 0) S.aborted = 1;
+        __syncthreads();
+        break;
+      }
+      uint32_t raw = stream[pos + tid];
+      bool acc = draw_index(prngKind, raw, g.nC, maxvalue, idx);
+      uint64_t am = __ballot(acc);
+      if (lane == 0) S.waveCnt[wave] = (uint32_t)__popcll(am);
+      __syncthreads();
+      unsigned before = 0;
+      for (int wv = 0; wv < (int)wave; wv++) before += S.waveCnt[wv];
+      active = acc && drawn + before + (unsigned)__popcll(am & lanemask_lt()) < g.maxProbe;
+    } else {
+      unsigned j = drawn + tid;
+      active = j < g.maxProbe;
+      if (active) {
+        uint64_t h = mix64(key * 0x100000001B3ull + j);
+        idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
+      }
+    }
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    int2 c = make_int2(0, 0);
+    if (active) {
+      c = b.corpusPts[idx];
+      score_candidate<Word>(g, cword, c.x, c.y, count, S.off, S.pix, S.T, S.Tm, S.best, F, M, alive, g.nMaps == 0);
+    }
+    S.cand[tid] = c;
+    S.candM[tid] = M;
+    S.candF[tid] = F;
+    S.candOk[tid] = alive ? 1 : 0;
+    S.candUsed[tid] = active ? 1 : 0;
+    __syncthreads();
+    if (wave == 0) {
+      int lastUsed;
+      resolve_slots_wave0<Word, WAVES>(S, 64u * WAVES, lastUsed);
+      if (lane == 0) S.lastUsed = (uint32_t)(lastUsed + 1);
+    }
+    __syncthreads();
+    const unsigned nUsed = (unsigned)__syncthreads_count(S.candUsed[tid] != 0);
+    if (SERIAL && tid == 0) {
+      // raw outputs consumed: through the last one used when the probes end in this
+      // window, else the whole window
+      bool done = S.perfect || drawn + nUsed >= g.maxProbe;
+      S.streamPos = done ? pos + S.lastUsed : pos + 64u * WAVES;
+    }
+    drawn += nUsed;
+    if (tid == 0) S.evals += nUsed;
+    __syncthreads();
+  }
+  if (SERIAL && S.aborted) return;
+
+  // commit (lib/synthesize.h:620-639)
+  if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
+  if (tid == 0) {
+    size_t tp = (size_t)t.y * (unsigned)g.tw + (unsigned)t.x;
+    Word old = tword[tp];
+    Word nw = old | (Word)1;
+    if (S.scored) {
+      int2 cur = b.sourceOf[tp];
+      if (cur.x != S.winX || cur.y != S.winY) {
+        Word cmask = 0;
+        for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
+        Word src = cword[(size_t)S.winY * (unsigned)g.cw + (unsigned)S.winX];
+        nw = (old & ~cmask) | (src & cmask) | (Word)1;
+        b.sourceOf[tp] = make_int2(S.winX, S.winY);
+        betTot++;
+      }
+    }
+    tword[tp] = nw;
+    b.lastBest[tp] = S.best;
+    if (g.skipEmpty) b.tileHas[(size_t)(t.y / kTile) * g.tilesW + t.x / kTile] = 1;
+    evalTot += S.evals;
+    termTot += (unsigned long long)S.evals * count;
+  }
+}
+
+template <typename Word, int WAVES>
+__device__ __forceinline__ void load_tables(const SynthBuffers &b, VisitLDS<Word, WAVES> &S)
+{
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    S.T[i] = b.tables->T[i];
+    S.Tm[i] = b.tables->Tm[i];
+  }
+  if (threadIdx.x == 0) {
+    S.tileHas = b.tileHas;
+    S.firstIdx = b.firstIdx;
+  }
+  __syncthreads();
+}
+
+// Batched visits: one workgroup per target of [k0, k1).
+template <typename Word, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_synth_batched(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
+                                                              unsigned seed, unsigned pass)
+{
+  __shared__ VisitLDS<Word, WAVES> S;
+  load_tables<Word, WAVES>(b, S);
+  unsigned long long ev = 0, tm = 0;
+  unsigned bt = 0;
+  for (unsigned k = k0 + blockIdx.x; k < k1; k += gridDim.x) {
+    visit_wg<Word, WAVES, false>(g, b, S, k, seed, pass, 0, nullptr, 0, 0, ev, tm, bt);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    unsigned slot = blockIdx.x & (CNT_SLOTS - 1);
+    if (ev) atomicAdd(&b.counters->candidates[slot], ev);
+    if (tm) atomicAdd(&b.counters->terms[slot], tm);
+    if (bt) atomicAdd(&b.counters->betters[slot], bt);
+  }
+}
+
+// Batched visits, one wavefront per target (large chunks: every wave stays busy on its
+// own target instead of idling through another wave's gather and heuristic-1 phases).
+template <typename Word, int WAVES, int G0>
+__global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
+                                                           unsigned seed, unsigned pass)
+{
+  __shared__ uint16_t sT[512];
+  __shared__ uint32_t sTm[512];
+  __shared__ int2 sOff[WAVES][64];
+  __shared__ Word sPix[WAVES][64];
+  __shared__ int2 sSrc[WAVES][64];
+  __shared__ unsigned long long sEval[WAVES], sTerm[WAVES];
+  __shared__ unsigned sBet[WAVES];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    sT[i] = b.tables->T[i];
+    sTm[i] = b.tables->Tm[i];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const unsigned lane = lane_id();
+  Word *tword = (Word *)b.tword;
+  const Word *cword = (const Word *)b.cword;
+  const bool skip0 = g.nMaps == 0;
+  unsigned long long myEval = 0, myTerm = 0;
+  unsigned myBet = 0;
+  for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES) {
+    const int2 t = b.targets[k];
+    const unsigned count = gather_neighbours<Word>(g, tword, b.sourceOf, b.offsets, t.x, t.y, sOff[wave],
+                                                   sPix[wave], sSrc[wave], b.tileHas, b.firstIdx);
+    uint32_t best = 0xFFFFFFFFu;
+    int winX = 0, winY = 0;
+    bool perfect = false, scored = false, record = false;
+    int cx, cy;
+    unsigned evals = 0;
+    coherence_phase<Word>(g, cword, b.prober, k, count, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, best, winX,
+                          winY, perfect, scored, record, cx, cy, evals);
+    if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
+    // random probes: per-target counter-based stream (seed, pass, target index, draw)
+    const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
+    for (unsigned base = 0; base < g.maxProbe && !perfect; base += 64) {
+      const unsigned j = base + lane;
+      const bool active = j < g.maxProbe;
+      uint32_t F = 0, M = 0;
+      bool alive = false;
+      int rx = 0, ry = 0;
+      if (active) {
+        uint64_t h = mix64(key * 0x100000001B3ull + j);
+        unsigned idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
+        int2 c = b.corpusPts[idx];
+        rx = c.x;
+        ry = c.y;
+        score_candidate<Word, G0>(g, cword, rx, ry, count, sOff[wave], sPix[wave], sT, sTm, best, F, M, alive,
+                                  skip0);
+      }
+      int win = -1, stop = 64;
+      uint32_t bb = best;
+      resolve(active && alive, M, F, bb, win, stop);
+      if (win >= 0) {
+        best = bb;
+        winX = __shfl(rx, win);
+        winY = __shfl(ry, win);
+        scored = true;
+      }
+      perfect = stop < 64;
+      evals += perfect ? (unsigned)stop + 1 : (g.maxProbe - base < 64 ? g.maxProbe - base : 64);
+    }
+    // store (lib/synthesize.h:620-639)
+    if (lane == 0) {
+      const size_t tp = (size_t)t.y * (unsigned)g.tw + (unsigned)t.x;
+      const Word old = tword[tp];
+      Word nw = old | (Word)1;
+      if (scored) {
+        const int2 cur = b.sourceOf[tp];
+        if (cur.x != winX || cur.y != winY) {
+          Word cmask = 0;
+          for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
+          const Word src = cword[(size_t)winY * (unsigned)g.cw + (unsigned)winX];
+          nw = (old & ~cmask) | (src & cmask) | (Word)1;
+          b.sourceOf[tp] = make_int2(winX, winY);
+          myBet++;
+        }
+      }
+      tword[tp] = nw;
+      b.lastBest[tp] = best;
+      if (g.skipEmpty) b.tileHas[(size_t)(t.y / kTile) * g.tilesW + t.x / kTile] = 1;
+    }
+    myEval += evals;
+    myTerm += (unsigned long long)evals * count;
+  }
+  if (lane == 0) {
+    sEval[wave] = myEval;
+    sTerm[wave] = myTerm;
+    sBet[wave] = myBet;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long e = 0, tm = 0;
+    unsigned bt = 0;
+    for (int w = 0; w < WAVES; w++) {
+      e += sEval[w];
+      tm += sTerm[w];
+      bt += sBet[w];
+    }
+    const unsigned slot = blockIdx.x & (CNT_SLOTS - 1);
+    if (e) atomicAdd(&b.counters->candidates[slot], e);
+    if (tm) atomicAdd(&b.counters->terms[slot], tm);
+    if (bt) atomicAdd(&b.counters->betters[slot], bt);
+  }
+}
+
+// Chunk sizes at or above this use one wavefront per target (env RESYNTH_WAVE_MIN), and
+// the size of the first group of random-probe loads (env RESYNTH_FIRST_GROUP: 2, 4 or 8).
+static unsigned wave_min()
+{
+  static unsigned v = [] {
+    const char *e = std::getenv("RESYNTH_WAVE_MIN");
+    return e ? (unsigned)std::atoi(e) : 2048u;
+  }();
+  return v;
+}
+static int first_group()
+{
+  static int v = [] {
+    const char *e = std::getenv("RESYNTH_FIRST_GROUP");
+    int x = e ? std::atoi(e) : 4;
+    return (x == 2 || x == 4 || x == 8) ? x : 4;
+  }();
+  return v;
+}
+
+template <typename Word>
+static void launch_wave(hipStream_t s, const Geometry &g, const SynthBuffers &b, unsigned k0, unsigned k1,
+                        unsigned seed, unsigned pass)
+{
+  constexpr int WAVES = 4;
+  unsigned blocks = (k1 - k0 + WAVES - 1) / WAVES;
+  if (blocks > 16384) blocks = 16384;
+  switch (first_group()) {
+  case 2: hipLaunchKernelGGL((k_synth_wave<Word, WAVES, 2>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass); break;
+  case 8: hipLaunchKernelGGL((k_synth_wave<Word, WAVES, 8>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass); break;
+  default: hipLaunchKernelGGL((k_synth_wave<Word, WAVES, 4>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass); break;
+  }
+}
+
+hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                                unsigned k1, unsigned seed, unsigned pass)
+{
+  constexpr int WAVES = 4;
+  if (k1 <= k0) return hipSuccess;
+  if (k1 - k0 >= wave_min()) {
+    if (wide) launch_wave<uint64_t>(s, g, b, k0, k1, seed, pass);
+    else launch_wave<uint32_t>(s, g, b, k0, k1, seed, pass);
+    return hipGetLastError();
+  }
+  unsigned blocks = k1 - k0;
+  if (wide)
+    hipLaunchKernelGGL((k_synth_batched<uint64_t, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
+                       pass);
+  else
+    hipLaunchKernelGGL((k_synth_batched<uint32_t, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
+                       pass);
+  return hipGetLastError();
+}
+
+// Serial visits: one workgroup visits targets k0..k1-1 in order (lib/refiner.h:75-100),
+// drawing probes from the exact PRNG stream.
+template <typename Word, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_synth_serial(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
+                                                             int prngKind, const uint32_t *__restrict__ stream,
+                                                             uint32_t streamLen, StreamState *state)
+{
+  __shared__ VisitLDS<Word, WAVES> S;
+  load_tables<Word, WAVES>(b, S);
+  uint32_t maxvalue;   // of g_rand_int_range(0, nC), GLib >= 2.2
+  if (g.nC <= 0x80000000u) {
+    uint32_t leftover = (0x80000000u % g.nC) * 2u;
+    if (leftover >= g.nC) leftover -= g.nC;
+    maxvalue = 0xFFFFFFFFu - leftover;
+  } else {
+    maxvalue = g.nC - 1u;
+  }
+  if (threadIdx.x == 0) S.streamPos = state->pos;
+  __syncthreads();
+  unsigned long long ev = 0, tm = 0;
+  unsigned bt = 0;
+  unsigned k = k0;
+  bool aborted = false;
+  for (; k < k1; k++) {
+    const uint32_t posBefore = S.streamPos;
+    __syncthreads();
+    visit_wg<Word, WAVES, true>(g, b, S, k, 0, 0, prngKind, stream, streamLen, maxvalue, ev, tm, bt);
+    __syncthreads();
+    if (S.aborted) {
+      __syncthreads();
+      if (threadIdx.x == 0) S.streamPos = posBefore;
+      aborted = true;
+      __syncthreads();
+      break;
+    }
+  }
+  if (threadIdx.x == 0) {
+    state->pos = S.streamPos;
+    state->kDone = k;
+    state->aborted = aborted ? 1u : 0u;
+    b.counters->candidates[0] += ev;
+    b.counters->terms[0] += tm;
+    b.counters->betters[0] += bt;
+  }
+}
+
+hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                               unsigned k1, int prngKind, const uint32_t *stream, uint32_t streamLen,
+                               StreamState *state)
+{
+  constexpr int WAVES = 4;
+  if (wide)
+    hipLaunchKernelGGL((k_synth_serial<uint64_t, WAVES>), dim3(1), dim3(64 * WAVES), 0, s, g, b, k0, k1, prngKind,
+                       stream, streamLen, state);
+  else
+    hipLaunchKernelGGL((k_synth_serial<uint32_t, WAVES>), dim3(1), dim3(64 * WAVES), 0, s, g, b, k0, k1, prngKind,
+                       stream, streamLen, state);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ adaptation
+
+// SimpleAPI adaptation (lib/adaptSimple.h:226-307 + lib/engine.c:338-431): one thread per
+// pixel builds the target word (hasValue = use_context && unselected && opaque), the
+// corpus word (mask byte 0xFF <=> selected in the corpus mask) and the point flags.
+template <typename Word>
+__global__ void k_adapt_simple(const unsigned char *__restrict__ image, size_t rowBytes, unsigned bpp,
+                               const unsigned char *__restrict__ mask, size_t maskRowBytes,
+                               const unsigned char *__restrict__ mask2, size_t mask2RowBytes, int w, int h,
+                               int nColors, int alphaIdx, int useContext, Word *tword, Word *cword, uint8_t *tflag,
+                               uint8_t *cflag)
+{
+  size_t n = (size_t)w * h;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    unsigned x = (unsigned)(p % (unsigned)w), y = (unsigned)(p / (unsigned)w);
+    const unsigned char *px = image + y * rowBytes + (size_t)x * bpp;
+    Word chans = 0;
+    for (int c = 0; c < nColors; c++) chans |= (Word)px[c] << (8 * (1 + c));
+    bool opaque = alphaIdx < 0 ? true : px[alphaIdx] != 0;
+    unsigned char m = mask[y * maskRowBytes + x];
+    unsigned char cm = mask2 ? mask2[y * mask2RowBytes + x] : (unsigned char)~m;
+    bool hasValue = useContext && m == 0 && opaque;
+    tword[p] = chans | (Word)(hasValue ? 1u : 0u);
+    cword[p] = chans | (Word)(cm == 0xFF ? 0xFFu : 0u);
+    tflag[p] = m != 0;
+    cflag[p] = (cm == 0xFF) && opaque;
+  }
+}
+
+hipError_t launch_adapt_simple(hipStream_t s, bool wide, const unsigned char *image, size_t rowBytes, unsigned bpp,
+                               const unsigned char *mask, size_t maskRowBytes, const unsigned char *mask2,
+                               size_t mask2RowBytes, int w, int h, int nColors, int alphaIdx, int useContext,
+                               void *tword, void *cword, uint8_t *tflag, uint8_t *cflag)
+{
+  size_t n = (size_t)w * h;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) blocks = 1;
+  if (wide)
+    hipLaunchKernelGGL((k_adapt_simple<uint64_t>), dim3(blocks), dim3(256), 0, s, image, rowBytes, bpp, mask,
+                       maskRowBytes, mask2, mask2RowBytes, w, h, nColors, alphaIdx, useContext, (uint64_t *)tword,
+                       (uint64_t *)cword, tflag, cflag);
+  else
+    hipLaunchKernelGGL((k_adapt_simple<uint32_t>), dim3(blocks), dim3(256), 0, s, image, rowBytes, bpp, mask,
+                       maskRowBytes, mask2, mask2RowBytes, w, h, nColors, alphaIdx, useContext, (uint32_t *)tword,
+                       (uint32_t *)cword, tflag, cflag);
+  return hipGetLastError();
+}
+
+// Engine-layout adaptation: pixmap [mask][colours][alpha?][maps] -> words and flags.
+// Target (isTarget): flag = mask != 0, word bit 0 = hasValue.  Corpus: flag = selected
+// and opaque, word byte 0 = 0xFF when selected.
+template <typename Word>
+__global__ void k_adapt_maps(const unsigned char *__restrict__ pix, int w, int h, int depth, int colorEnd,
+                             int mapStart, int mapEnd, int alphaIdx, int useContext, int isTarget, Word *word,
+                             uint8_t *flag)
+{
+  size_t n = (size_t)w * h;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+    const unsigned char *px = pix + p * depth;
+    Word chans = 0;
+    int c = 0;
+    for (int j = 1; j < colorEnd; j++, c++) chans |= (Word)px[j] << (8 * (1 + c));
+    for (int j = mapStart; j < mapEnd; j++, c++) chans |= (Word)px[j] << (8 * (1 + c));
+    bool opaque = alphaIdx < 0 ? true : px[alphaIdx] != 0;
+    unsigned char m = px[0];
+    if (isTarget) {
+      bool hasValue = useContext && m == 0 && opaque;
+      word[p] = chans | (Word)(hasValue ? 1u : 0u);
+      flag[p] = m != 0;
+    } else {
+      word[p] = chans | (Word)(m == 0xFF ? 0xFFu : 0u);
+      flag[p] = (m == 0xFF) && opaque;
+    }
+  }
+}
+
+hipError_t launch_adapt_maps(hipStream_t s, bool wide, const unsigned char *pix, int w, int h, int depth,
+                             int colorEnd, int mapStart, int mapEnd, int alphaIdx, int useContext, int isTarget,
+                             void *word, uint8_t *flag)
+{
+  size_t n = (size_t)w * h;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (blocks > 65536) blocks = 65536;
+  if (blocks == 0) blocks = 1;
+  if (wide)
+    hipLaunchKernelGGL((k_adapt_maps<uint64_t>), dim3(blocks), dim3(256), 0, s, pix, w, h, depth, colorEnd, mapStart,
+                       mapEnd, alphaIdx, useContext, isTarget, (uint64_t *)word, flag);
+  else
+    hipLaunchKernelGGL((k_adapt_maps<uint32_t>), dim3(blocks), dim3(256), 0, s, pix, w, h, depth, colorEnd, mapStart,
+                       mapEnd, alphaIdx, useContext, isTarget, (uint32_t *)word, flag);
+  return hipGetLastError();
+}
+
+// Packs engine-layout pixels (frozen-state scorer input) into words.
+template <typename Word>
+__global__ void k_pack_words(const unsigned char *pix, int n, int depth, int colorEnd, int mapStart, int mapEnd,
+                             int corpusFlags, Word *out)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned char *px = pix + (size_t)p * depth;
+  Word chans = 0;
+  int c = 0;
+  for (int j = 1; j < colorEnd; j++, c++) chans |= (Word)px[j] << (8 * (1 + c));
+  for (int j = mapStart; j < mapEnd; j++, c++) chans |= (Word)px[j] << (8 * (1 + c));
+  out[p] = chans | (Word)(corpusFlags ? (px[0] == 0xFF ? 0xFFu : 0u) : (px[0] & 1u));
+}
+
+hipError_t launch_pack_words(hipStream_t s, bool wide, const unsigned char *pix, int n, int depth, int colorEnd,
+                             int mapStart, int mapEnd, int corpusFlags, void *out)
+{
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (blocks == 0) return hipSuccess;
+  if (wide)
+    hipLaunchKernelGGL((k_pack_words<uint64_t>), dim3(blocks), dim3(256), 0, s, pix, n, depth, colorEnd, mapStart,
+                       mapEnd, corpusFlags, (uint64_t *)out);
+  else
+    hipLaunchKernelGGL((k_pack_words<uint32_t>), dim3(blocks), dim3(256), 0, s, pix, n, depth, colorEnd, mapStart,
+                       mapEnd, corpusFlags, (uint32_t *)out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ compaction
+
+// Row-major list of the flagged pixels (lib/engine.c:338-431 build these lists by a
+// row-major scan): per-tile counts, one exclusive scan of the tile counts, scatter.
+constexpr int CT_THREADS = 256;
+constexpr int CT_ITEMS = 16;
+constexpr int CT_TILE = CT_THREADS * CT_ITEMS;
+
+__global__ __launch_bounds__(CT_THREADS) void k_tile_counts(const uint8_t *flags, size_t n, unsigned *tileCount)
+{
+  __shared__ unsigned red[CT_THREADS / 64];
+  size_t base = (size_t)blockIdx.x * CT_TILE + (size_t)threadIdx.x * CT_ITEMS;
+  unsigned c = 0;
+  for (int i = 0; i < CT_ITEMS; i++)
+    if (base + i < n) c += flags[base + i] != 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane_id() == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned s = 0;
+    for (int w = 0; w < CT_THREADS / 64; w++) s += red[w];
+    tileCount[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(unsigned *tileCount, unsigned nTiles, unsigned *total)
+{
+  __shared__ unsigned carry;
+  __shared__ unsigned waveSum[16];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (unsigned base = 0; base < nTiles; base += 1024) {
+    unsigned i = base + threadIdx.x;
+    unsigned v = i < nTiles ? tileCount[i] : 0;
+    // inclusive wave scan
+    unsigned x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      unsigned y = __shfl_up(x, o);
+      if ((int)lane_id() >= o) x += y;
+    }
+    if (lane_id() == 63) waveSum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    unsigned waveOff = 0;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); w++) waveOff += waveSum[w];
+    unsigned excl = carry + waveOff + x - v;
+    if (i < nTiles) tileCount[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(CT_THREADS) void k_tile_scatter(const uint8_t *flags, size_t n, int w,
+                                                             const unsigned *tileOffset, int2 *out)
+{
+  __shared__ unsigned waveSum[CT_THREADS / 64];
+  size_t base = (size_t)blockIdx.x * CT_TILE + (size_t)threadIdx.x * CT_ITEMS;
+  uint8_t f[CT_ITEMS];
+  unsigned c = 0;
+  for (int i = 0; i < CT_ITEMS; i++) {
+    f[i] = (base + i < n) ? (flags[base + i] != 0) : 0;
+    c += f[i];
+  }
+  unsigned x = c;
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned y = __shfl_up(x, o);
+    if ((int)lane_id() >= o) x += y;
+  }
+  if (lane_id() == 63) waveSum[threadIdx.x >> 6] = x;
+  __syncthreads();
+  unsigned off = tileOffset[blockIdx.x];
+  for (int wv = 0; wv < (int)(threadIdx.x >> 6); wv++) off += waveSum[wv];
+  off += x - c;
+  for (int i = 0; i < CT_ITEMS; i++)
+    if (f[i]) {
+      size_t p = base + i;
+      out[off++] = make_int2((int)(p % (unsigned)w), (int)(p / (unsigned)w));
+    }
+}
+
+size_t compact_temp_bytes(size_t n) { return (((n + CT_TILE - 1) / CT_TILE) + 64) * sizeof(unsigned); }
+
+hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int2 *out, unsigned *count, void *temp)
+{
+  size_t n = (size_t)w * h;
+  unsigned tiles = (unsigned)((n + CT_TILE - 1) / CT_TILE);
+  if (tiles == 0) tiles = 1;
+  unsigned *tc = (unsigned *)temp;
+  hipLaunchKernelGGL(k_tile_counts, dim3(tiles), dim3(CT_THREADS), 0, s, flags, n, tc);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, tc, tiles, count);
+  hipLaunchKernelGGL(k_tile_scatter, dim3(tiles), dim3(CT_THREADS), 0, s, flags, n, w, tc, out);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ sorted offsets
+
+// lib/engine.c:465-497 sorts the row-major offsets of [-w+1,w-1]x[-h+1,h-1] by x^2+y^2
+// with a comparator that never returns 0 under GLib's merge sort, which orders equal
+// distances by descending row-major index.  The same order is a stable LSD radix sort
+// by x^2+y^2 of the offsets listed in descending row-major order.
+__global__ void k_offset_keys(int w, int h, uint32_t *keys, uint32_t *vals, size_t nO)
+{
+  const unsigned W2 = 2u * (unsigned)w - 1u;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nO; e += (size_t)gridDim.x * blockDim.x) {
+    uint32_t r = (uint32_t)(nO - 1 - e);
+    int x = (int)(r % W2) - (w - 1), y = (int)(r / W2) - (h - 1);
+    keys[e] = (uint32_t)(x * x + y * y);
+    vals[e] = r;
+  }
+}
+
+__global__ void k_offset_decode(int w, int h, const uint32_t *vals, int2 *out, size_t nO)
+{
+  const unsigned W2 = 2u * (unsigned)w - 1u;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nO; i += (size_t)gridDim.x * blockDim.x) {
+    uint32_t r = vals[i];
+    out[i] = make_int2((int)(r % W2) - (w - 1), (int)(r / W2) - (h - 1));
+  }
+}
+
+static unsigned key_bits(int w, int h)
+{
+  uint64_t mx = (uint64_t)(w - 1) * (w - 1) + (uint64_t)(h - 1) * (h - 1);
+  unsigned b = 1;
+  while (b < 32 && (mx >> b)) b++;
+  return b;
+}
+
+size_t offsets_temp_bytes(size_t nO)
+{
+  size_t sortBytes = 0;
+  (void)rocprim::radix_sort_pairs<rocprim::default_config>(nullptr, sortBytes, (uint32_t *)nullptr,
+                                                           (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                           (uint32_t *)nullptr, nO, 0, 32);
+  return 4 * nO * sizeof(uint32_t) + sortBytes + 1024;
+}
+
+hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes)
+{
+  size_t nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
+  uint32_t *k0 = (uint32_t *)temp, *v0 = k0 + nO, *k1 = v0 + nO, *v1 = k1 + nO;
+  char *sortTemp = (char *)(v1 + nO);
+  size_t sortBytes = tempBytes - 4 * nO * sizeof(uint32_t);
+  unsigned blocks = (unsigned)((nO + 255) / 256);
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_offset_keys, dim3(blocks), dim3(256), 0, s, w, h, k0, v0, nO);
+  hipError_t e = rocprim::radix_sort_pairs<rocprim::default_config>(sortTemp, sortBytes, k0, k1, v0, v1, nO, 0,
+                                                                    key_bits(w, h), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_offset_decode, dim3(blocks), dim3(256), 0, s, w, h, v1, out, nO);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ small kernels
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
+{
+  if (!n) return hipSuccess;
+  unsigned blocks = (unsigned)((n + 255) / 256);
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+// Tiles holding a pixel that has a value (target word bit 0).
+template <typename Word>
+__global__ void k_tile_init(const Word *tword, int tw, int th, uint8_t *tileHas)
+{
+  const int tilesW = (tw + kTile - 1) / kTile;
+  size_t n = (size_t)tw * th;
+  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
+    if (tword[p] & 1u) tileHas[(p / (unsigned)tw) / kTile * (size_t)tilesW + (p % (unsigned)tw) / kTile] = 1;
+}
+
+hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas)
+{
+  size_t n = (size_t)tw * th;
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  if (!blocks) return hipSuccess;
+  if (wide) hipLaunchKernelGGL((k_tile_init<uint64_t>), dim3(blocks), dim3(256), 0, s, (const uint64_t *)tword, tw, th, tileHas);
+  else hipLaunchKernelGGL((k_tile_init<uint32_t>), dim3(blocks), dim3(256), 0, s, (const uint32_t *)tword, tw, th, tileHas);
+  return hipGetLastError();
+}
+
+// firstIdx[rho] = #{(x,y) : |x| <= w-1, |y| <= h-1, x^2 + y^2 < rho^2}: the position in the
+// sorted offsets where distance rho begins (the sort key is x^2 + y^2).
+__global__ void k_first_idx(int w, int h, unsigned rhoMax, uint32_t *firstIdx)
+{
+  unsigned rho = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rho > rhoMax) return;
+  const long long r2 = (long long)rho * rho;
+  unsigned long long cnt = 0;
+  for (int x = -(w - 1); x <= w - 1; x++) {
+    long long rem = r2 - (long long)x * x - 1;   // y^2 <= rem
+    if (rem < 0) continue;
+    long long ys = (long long)sqrt((double)rem);
+    while (ys * ys > rem) ys--;
+    while ((ys + 1) * (ys + 1) <= rem) ys++;
+    if (ys > h - 1) ys = h - 1;
+    cnt += 2 * ys + 1;
+  }
+  firstIdx[rho] = (uint32_t)cnt;
+}
+
+hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx)
+{
+  hipLaunchKernelGGL(k_first_idx, dim3((rhoMax + 1 + 255) / 256), dim3(256), 0, s, w, h, rhoMax, firstIdx);
+  return hipGetLastError();
+}
+
+__global__ void k_gather_targets(const int2 *rm, const uint32_t *perm, int2 *out, unsigned n)
+{
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = rm[perm[i]];
+}
+
+hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n)
+{
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_targets, dim3((n + 255) / 256), dim3(256), 0, s, rm, perm, out, n);
+  return hipGetLastError();
+}
+
+// lib/adaptSimple.h:111-168 writes every pixel back; only target pixels can differ, so
+// only their colour channels are written (alpha and context are left as they are).
+template <typename Word>
+__global__ void k_writeback_simple(const Word *tword, const int2 *targets, unsigned nT, unsigned char *image,
+                                   size_t rowBytes, unsigned bpp, int nColors, int w)
+{
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nT) return;
+  int2 t = targets[i];
+  Word v = tword[(size_t)t.y * (unsigned)w + (unsigned)t.x];
+  unsigned char *px = image + (size_t)t.y * rowBytes + (size_t)t.x * bpp;
+  for (int c = 0; c < nColors; c++) px[c] = (unsigned char)chan(v, c);
+}
+
+hipError_t launch_writeback_simple(hipStream_t s, bool wide, const void *tword, const int2 *targets, unsigned nT,
+                                   unsigned char *image, size_t rowBytes, unsigned bpp, int nColors, int w)
+{
+  if (!nT) return hipSuccess;
+  if (wide)
+    hipLaunchKernelGGL((k_writeback_simple<uint64_t>), dim3((nT + 255) / 256), dim3(256), 0, s,
+                       (const uint64_t *)tword, targets, nT, image, rowBytes, bpp, nColors, w);
+  else
+    hipLaunchKernelGGL((k_writeback_simple<uint32_t>), dim3((nT + 255) / 256), dim3(256), 0, s,
+                       (const uint32_t *)tword, targets, nT, image, rowBytes, bpp, nColors, w);
+  return hipGetLastError();
+}
+
+// lib/synthesize.h:403-419 setColor: colour pixelels [1, colorEndBip) of target pixels.
+template <typename Word>
+__global__ void k_writeback_pixmap(const Word *tword, const int2 *targets, unsigned nT, unsigned char *pix, int depth,
+                                   int nColors, int w)
+{
+  unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nT) return;
+  int2 t = targets[i];
+  size_t p = (size_t)t.y * (unsigned)w + (unsigned)t.x;
+  Word v = tword[p];
+  for (int c = 0; c < nColors; c++) pix[p * depth + 1 + c] = (unsigned char)chan(v, c);
+}
+
+hipError_t launch_writeback_pixmap(hipStream_t s, bool wide, const void *tword, const int2 *targets, unsigned nT,
+                                   unsigned char *pix, int depth, int nColors, int w)
+{
+  if (!nT) return hipSuccess;
+  if (wide)
+    hipLaunchKernelGGL((k_writeback_pixmap<uint64_t>), dim3((nT + 255) / 256), dim3(256), 0, s,
+                       (const uint64_t *)tword, targets, nT, pix, depth, nColors, w);
+  else
+    hipLaunchKernelGGL((k_writeback_pixmap<uint32_t>), dim3((nT + 255) / 256), dim3(256), 0, s,
+                       (const uint32_t *)tword, targets, nT, pix, depth, nColors, w);
+  return hipGetLastError();
+}
+
+__global__ void k_mean_error(const uint32_t *lastBest, const int2 *targets, unsigned nT, int w,
+                             unsigned long long *sumCount)
+{
+  unsigned long long s = 0, c = 0;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nT; i += gridDim.x * blockDim.x) {
+    int2 t = targets[i];
+    uint32_t v = lastBest[(size_t)t.y * (unsigned)w + (unsigned)t.x];
+    if (v != 0xFFFFFFFFu) {
+      s += v;
+      c++;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o);
+    c += __shfl_xor(c, o);
+  }
+  if (lane_id() == 0 && c) {
+    atomicAdd(&sumCount[0], s);
+    atomicAdd(&sumCount[1], c);
+  }
+}
+
+hipError_t launch_mean_error(hipStream_t s, const uint32_t *lastBest, const int2 *targets, unsigned nT, int w,
+                             unsigned long long *sumCount)
+{
+  unsigned blocks = (nT + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_mean_error, dim3(blocks), dim3(256), 0, s, lastBest, targets, nT, w, sumCount);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ frozen-state scorer
+
+template <typename Word>
+__global__ __launch_bounds__(64) void k_score_frozen(Geometry g, const Word *cword, const Tables *tables,
+                                                     const unsigned *nbCount, const int2 *nbOffset,
+                                                     const Word *nbWord, const unsigned *candCount, unsigned maxCands,
+                                                     const int2 *cands, int32_t *winner, uint32_t *bestSum,
+                                                     int32_t *stopped)
+{
+  __shared__ uint16_t sT[512];
+  __shared__ uint32_t sTm[512];
+  __shared__ int2 sOff[64];
+  __shared__ Word sPix[64];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    sT[i] = tables->T[i];
+    sTm[i] = tables->Tm[i];
+  }
+  const unsigned prob = blockIdx.x;
+  const unsigned lane = lane_id();
+  unsigned count = nbCount[prob];
+  if (count > 64) count = 64;
+  if (lane < count) {
+    sOff[lane] = nbOffset[(size_t)prob * 64 + lane];
+    sPix[lane] = nbWord[(size_t)prob * 64 + lane];
+  }
+  __syncthreads();
+  unsigned nc = candCount[prob];
+  uint32_t best = 0xFFFFFFFFu;
+  int winIdx = -1;
+  unsigned consumed = nc;
+  for (unsigned base = 0; base < nc; base += 64) {
+    unsigned j = base + lane;
+    bool active = j < nc;
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    if (active) {
+      int2 c = cands[(size_t)prob * maxCands + j];
+      score_candidate<Word>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive);
+    }
+    int win = -1, stop = 64;
+    uint32_t bb = best;
+    resolve(active && alive, M, F, bb, win, stop);
+    if (win >= 0) {
+      best = bb;
+      winIdx = (int)(base + win);
+    }
+    if (stop < 64) {
+      consumed = base + stop + 1;
+      break;
+    }
+  }
+  if (lane == 0) {
+    winner[prob] = winIdx;
+    bestSum[prob] = best;
+    stopped[prob] = (int32_t)consumed;
+  }
+}
+
+hipError_t launch_score_frozen(hipStream_t s, bool wide, int cw, int ch, int nColors, int nMaps, uint32_t penalty,
+                               const void *cword, const Tables *tables, unsigned nProblems, const unsigned *nbCount,
+                               const int2 *nbOffset, const void *nbWord, const unsigned *candCount,
+                               unsigned maxCands, const int2 *cands, int32_t *winner, uint32_t *bestSum,
+                               int32_t *stopped)
+{
+  Geometry g = {};
+  g.cw = cw;
+  g.ch = ch;
+  g.nColors = nColors;
+  g.nMaps = nMaps;
+  g.penalty = penalty;
+  if (!nProblems) return hipSuccess;
+  if (wide)
+    hipLaunchKernelGGL((k_score_frozen<uint64_t>), dim3(nProblems), dim3(64), 0, s, g, (const uint64_t *)cword,
+                       tables, nbCount, nbOffset, (const uint64_t *)nbWord, candCount, maxCands, cands, winner,
+                       bestSum, stopped);
+  else
+    hipLaunchKernelGGL((k_score_frozen<uint32_t>), dim3(nProblems), dim3(64), 0, s, g, (const uint32_t *)cword,
+                       tables, nbCount, nbOffset, (const uint32_t *)nbWord, candCount, maxCands, cands, winner,
+                       bestSum, stopped);
+  return hipGetLastError();
+}
+
+}  // namespace rs
