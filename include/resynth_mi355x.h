@@ -150,7 +150,7 @@ typedef struct {
                           probe streams.  Serial mode: a nonzero seed replaces the engine's
                           seed (for statistics over seeds; not a reference behaviour) */
   unsigned int batch;  /* batched mode: largest number of concurrent target visits
-                          (0 = automatic: nT/32 clamped to [64, 16384]) */
+                          (0 = automatic: nT/32 clamped to [64, 65536]) */
   int timing;          /* nonzero: time every synthesis kernel with HIP events */
 } ResynthOptions;
 

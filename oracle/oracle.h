@@ -66,9 +66,10 @@ typedef struct {
                     pass 0 ramps its chunks up from 256 */
   int conflictRadius; /* batch model: >0 defers a target to a later chunk while a target of
                          the current chunk lies within this Chebyshev distance */
-  int rampPermille;   /* batch model: >0 sizes pass-0 chunks as max(64, k*rampPermille/1000)
+  int rampPermille;   /* batch model: >0 sizes pass-0 chunks as max(floor, k*rampPermille/1000)
                          where k is the number of targets already visited in the pass */
   unsigned seed;      /* 0: the engine's seed 1198472 (lib/engine.c:643); else a study seed */
+  int minChunk;       /* batch model: floor of the pass-0 ramp (0: 64) */
 } OOptions;
 
 typedef struct {

@@ -42,7 +42,7 @@ class OPixmap(ctypes.Structure):
 class OOptions(ctypes.Structure):
     _fields_ = [("threads", ctypes.c_int), ("prng", ctypes.c_int), ("batch", ctypes.c_int),
                 ("conflictRadius", ctypes.c_int), ("rampPermille", ctypes.c_int),
-                ("seed", ctypes.c_uint)]
+                ("seed", ctypes.c_uint), ("minChunk", ctypes.c_int)]
 
 
 class OStats(ctypes.Structure):
@@ -238,7 +238,8 @@ def _buffer(arr2d_bytes, width, height, row_bytes):
 
 
 def image_synth(image, mask, fmt, params=None, threads=1, prng=PRNG_GRAND, cancel=0, want_maps=False,
-                mask2=None, width=None, mask_width=None, mask2_width=None, batch=0, radius=0, ramp=0, seed=0):
+                mask2=None, width=None, mask_width=None, mask2_width=None, batch=0, radius=0, ramp=0, seed=0,
+                min_chunk=0):
     """image: uint8 (H, W*bpp + pad) row buffer or (H, W, C); mask: uint8 (H, W[+pad]).
     Returns (result, image_out) with image_out the same shape as image."""
     img = np.array(image, dtype=np.uint8, copy=True, order="C")
@@ -258,7 +259,7 @@ def image_synth(image, mask, fmt, params=None, threads=1, prng=PRNG_GRAND, cance
     log = []
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
-    opt = OOptions(threads, prng, batch, radius, ramp, seed)
+    opt = OOptions(threads, prng, batch, radius, ramp, seed, min_chunk)
     pp = ctypes.byref(_params(params)) if params is not None else None
     if mask2 is None:
         err = lib().oracle_imageSynth(ctypes.byref(ib), ctypes.byref(mb), fmt, pp, cb, None, ctypes.byref(cf),
@@ -273,7 +274,7 @@ def image_synth(image, mask, fmt, params=None, threads=1, prng=PRNG_GRAND, cance
 
 
 def engine(params, ix, target_pixmap, corpus_pixmap, threads=1, prng=PRNG_GRAND, cancel=0, want_maps=False,
-           batch=0, radius=0, ramp=0, seed=0):
+           batch=0, radius=0, ramp=0, seed=0, min_chunk=0):
     """target/corpus pixmaps: uint8 (H, W, depth) interleaved [mask, colours, alpha?, maps].
     Returns (result, target_out)."""
     t = np.array(target_pixmap, dtype=np.uint8, copy=True, order="C")
@@ -284,7 +285,7 @@ def engine(params, ix, target_pixmap, corpus_pixmap, threads=1, prng=PRNG_GRAND,
     log = []
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
-    opt = OOptions(threads, prng, batch, radius, ramp, seed)
+    opt = OOptions(threads, prng, batch, radius, ramp, seed, min_chunk)
     err = lib().oracle_engine(_params(params), ctypes.byref(ix), ctypes.byref(tp), ctypes.byref(cp), cb, None,
                               ctypes.byref(cf), ctypes.byref(opt), ctypes.byref(st))
     return _result(err, st, keep, log), t

@@ -496,7 +496,8 @@ static void refiner(Engine *E, Prng *prng, const OOptions *opt, OStats *st)
     if (opt && opt->batch > 0) {
       size_t np = (size_t)E->tgt->width * E->tgt->height;
       unsigned len = pass == 0 ? (256u < (unsigned)opt->batch ? 256u : (unsigned)opt->batch) : (unsigned)opt->batch;
-      if (pass == 0 && opt->rampPermille > 0) len = 64;
+      const unsigned floor0 = opt->minChunk > 0 ? (unsigned)opt->minChunk : 64u;
+      if (pass == 0 && opt->rampPermille > 0) len = floor0 < (unsigned)opt->batch ? floor0 : (unsigned)opt->batch;
       int R = opt->conflictRadius;
       int TW = (int)E->tgt->width, TH = (int)E->tgt->height;
       /* pending targets in synthesis order; each chunk takes up to len of them, skipping
@@ -549,7 +550,8 @@ static void refiner(Engine *E, Prng *prng, const OOptions *opt, OStats *st)
         if (pass == 0 && opt->rampPermille > 0) {
           unsigned long long done = end - nq;
           unsigned long long l = done * (unsigned long long)opt->rampPermille / 1000ull;
-          len = (unsigned)(l < 64 ? 64 : (l > (unsigned long long)opt->batch ? (unsigned long long)opt->batch : l));
+          if (l < floor0) l = floor0;
+          len = (unsigned)(l > (unsigned long long)opt->batch ? (unsigned long long)opt->batch : l);
         } else if (len < (unsigned)opt->batch) len *= 2;
         if (*E->cancel) break;
       }

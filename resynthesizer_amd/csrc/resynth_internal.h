@@ -10,7 +10,9 @@ namespace rs {
 // Device pixel word.  Byte 0 is a flag byte; bytes 1.. hold the matched channels,
 // colours first, then map channels, each one byte (the engine's Pixelel).
 //   target word: flag bit 0 = hasValue (lib/engine.c:152-168)
-//   corpus word: flag byte 0xFF = selected in the corpus mask (lib/engine.c:247-258)
+//   corpus word: flag byte 0xFF = selected in the corpus mask and opaque (a corpus point,
+//                lib/engine.c:400-431), 0xFE = selected but fully transparent (still
+//                matched, lib/engine.c:247-258), 0 = not selected
 // A word is 32 bits when colours+maps <= 3 and 64 bits otherwise, so every pixel is
 // read and written with one aligned access: a reader sees a pixel's value and its
 // hasValue flag from the same store.
@@ -27,6 +29,7 @@ struct Geometry {
   int tilesW, tilesH;     // 8x8-pixel tiles of the target image
   unsigned rhoMax;        // largest radius indexed by SynthBuffers::firstIdx
   int skipEmpty;          // pass 0 without tiling: start scans past provably empty rings
+  int probeByPixel;       // batched probes by rejection over corpus pixels (dense corpora)
 };
 
 constexpr int kTile = 8;

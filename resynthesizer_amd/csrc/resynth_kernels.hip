@@ -75,7 +75,7 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
   for (int u = 0; u < G; u++) {
     unsigned i = base + u;
     if (i < count && !dead) {
-      uint32_t term = (in[u] && (q[u] & 0xFFu) == 0xFFu)
+      uint32_t term = (in[u] && (q[u] & 0xFEu) == 0xFEu)
                           ? neighbour_term<Word>(nbPix[i], q[u], i == 0, g.nColors, g.nMaps, T, Tm)
                           : g.penalty;
       acc += term;
@@ -94,7 +94,7 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
 // map channels, so neighbour 0 (the candidate itself, colours not compared) adds 0; with
 // B0 > 0 its early-out test cannot fire either.  G0 is the size of the first group of
 // loads, which is where most random candidates die.
-template <typename Word, int G0 = 8>
+template <typename Word, int G0 = 8, int GL = 8>
 __device__ __forceinline__ void score_candidate(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
                                                 unsigned count, const int2 *nbOff, const Word *nbPix,
                                                 const uint16_t *T, const uint32_t *Tm, uint32_t B0, uint32_t &F,
@@ -104,8 +104,8 @@ __device__ __forceinline__ void score_candidate(const Geometry &g, const Word *_
   bool dead = false;
   unsigned base = (skip0 && B0 > 0) ? 1u : 0u;
   score_group<Word, G0>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
-  for (base += G0; base < count && !dead; base += 8)
-    score_group<Word, 8>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+  for (base += G0; base < count && !dead; base += GL)
+    score_group<Word, GL>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
   F = acc;
   M = mx;
   alive = !dead;
@@ -251,7 +251,7 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
       cy = src.y - nbOff[lane].y;
       if ((unsigned)cx < (unsigned)g.cw && (unsigned)cy < (unsigned)g.ch) {
         Word q = cword[(size_t)cy * (unsigned)g.cw + (unsigned)cx];
-        valid = (q & 0xFFu) == 0xFFu;
+        valid = (q & 0xFEu) == 0xFEu;
       }
     }
   }
@@ -356,14 +356,17 @@ __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, c
   if (g.skipEmpty && S.tileHas)
     start = scan_start(g, S.tileHas, S.firstIdx, tx, ty, tid, 64u * WAVES,
                        [](bool v) { return __syncthreads_or(v) != 0; });
+  int2 nextOff = make_int2(0, 0);
+  if (start + tid < g.nO) nextOff = offsets[start + tid];
   for (unsigned base = start; base < g.nO && count < L; base += 64u * WAVES) {
     unsigned j = base + tid;
     bool v = false;
-    int2 off = make_int2(0, 0);
+    int2 off = nextOff;
     size_t p = 0;
     Word w = 0;
+    // prefetch the next block of offsets while this one is gathered
+    if (j + 64u * WAVES < g.nO) nextOff = offsets[j + 64u * WAVES];
     if (j < g.nO) {
-      off = offsets[j];
       int px = tx + off.x, py = ty + off.y;
       bool in = true;
       if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
@@ -599,7 +602,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_batched(Geometry g, SynthB
 
 // Batched visits, one wavefront per target (large chunks: every wave stays busy on its
 // own target instead of idling through another wave's gather and heuristic-1 phases).
-template <typename Word, int WAVES, int G0>
+template <typename Word, int WAVES, int G0, int GL>
 __global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
                                                            unsigned seed, unsigned pass)
 {
@@ -634,34 +637,55 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuff
     coherence_phase<Word>(g, cword, b.prober, k, count, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, best, winX,
                           winY, perfect, scored, record, cx, cy, evals);
     if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
-    // random probes: per-target counter-based stream (seed, pass, target index, draw)
+    // random probes from a per-target counter-based stream (seed, pass, target index,
+    // attempt).  Dense corpora draw a uniform pixel of the corpus image and keep it when
+    // its word marks a corpus point (0xFF: selected and opaque), which is uniform over
+    // corpus points and lets the validity test ride in the first group of loads; sparse
+    // corpora index the corpus point list.  Draws are resolved in attempt order.
     const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
-    for (unsigned base = 0; base < g.maxProbe && !perfect; base += 64) {
-      const unsigned j = base + lane;
-      const bool active = j < g.maxProbe;
+    const bool byPixel = g.probeByPixel != 0;
+    const uint64_t nPix = (uint64_t)g.cw * (uint64_t)g.ch;
+    unsigned drawn = 0;
+    for (unsigned attempt = 0; drawn < g.maxProbe && !perfect; attempt += 64) {
+      const uint64_t h = mix64(key * 0x100000001B3ull + attempt + lane);
+      int2 c;
+      bool valid;
       uint32_t F = 0, M = 0;
       bool alive = false;
-      int rx = 0, ry = 0;
-      if (active) {
-        uint64_t h = mix64(key * 0x100000001B3ull + j);
-        unsigned idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
-        int2 c = b.corpusPts[idx];
-        rx = c.x;
-        ry = c.y;
-        score_candidate<Word, G0>(g, cword, rx, ry, count, sOff[wave], sPix[wave], sT, sTm, best, F, M, alive,
-                                  skip0);
+      if (byPixel) {
+        const uint64_t p = ((h >> 32) * nPix) >> 32;
+        c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
+        // the candidate's own word (neighbour 0, loaded with the first group) carries the
+        // corpus-point flag
+        const Word w0 = cword[p];
+        score_candidate<Word, G0 + 1, GL>(g, cword, c.x, c.y, count, sOff[wave], sPix[wave], sT, sTm, best, F, M,
+                                          alive, false);
+        valid = (w0 & 0xFFu) == 0xFFu;
+      } else {
+        const unsigned idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
+        c = b.corpusPts[idx];
+        valid = true;
+        score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff[wave], sPix[wave], sT, sTm, best, F, M, alive,
+                                      skip0);
       }
+      const uint64_t vm = __ballot(valid);
+      const bool active = valid && drawn + (unsigned)__popcll(vm & lanemask_lt()) < g.maxProbe;
       int win = -1, stop = 64;
       uint32_t bb = best;
       resolve(active && alive, M, F, bb, win, stop);
       if (win >= 0) {
         best = bb;
-        winX = __shfl(rx, win);
-        winY = __shfl(ry, win);
+        winX = __shfl(c.x, win);
+        winY = __shfl(c.y, win);
         scored = true;
       }
-      perfect = stop < 64;
-      evals += perfect ? (unsigned)stop + 1 : (g.maxProbe - base < 64 ? g.maxProbe - base : 64);
+      uint64_t used = __ballot(active);
+      if (stop < 64) {
+        perfect = true;
+        used &= (stop == 63) ? ~0ull : ((2ull << stop) - 1ull);
+      }
+      evals += (unsigned)__popcll(used);
+      drawn += (unsigned)__popcll(__ballot(active));
     }
     // store (lib/synthesize.h:620-639)
     if (lane == 0) {
@@ -721,10 +745,26 @@ static int first_group()
 {
   static int v = [] {
     const char *e = std::getenv("RESYNTH_FIRST_GROUP");
-    int x = e ? std::atoi(e) : 4;
-    return (x == 2 || x == 4 || x == 8) ? x : 4;
+    int x = e ? std::atoi(e) : 2;
+    return (x == 2 || x == 4 || x == 8) ? x : 2;
   }();
   return v;
+}
+
+template <typename Word, int G0>
+static void launch_wave_g(hipStream_t s, dim3 grid, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                          unsigned k1, unsigned seed, unsigned pass)
+{
+  constexpr int WAVES = 4;
+  // size of the later groups of random-probe loads (env RESYNTH_LATER_GROUP: 4 or 8)
+  static const int later = [] {
+    const char *e = std::getenv("RESYNTH_LATER_GROUP");
+    return (e && std::atoi(e) == 8) ? 8 : 4;
+  }();
+  if (later == 8)
+    hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 8>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
+  else
+    hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 4>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
 }
 
 template <typename Word>
@@ -735,9 +775,9 @@ static void launch_wave(hipStream_t s, const Geometry &g, const SynthBuffers &b,
   unsigned blocks = (k1 - k0 + WAVES - 1) / WAVES;
   if (blocks > 16384) blocks = 16384;
   switch (first_group()) {
-  case 2: hipLaunchKernelGGL((k_synth_wave<Word, WAVES, 2>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass); break;
-  case 8: hipLaunchKernelGGL((k_synth_wave<Word, WAVES, 8>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass); break;
-  default: hipLaunchKernelGGL((k_synth_wave<Word, WAVES, 4>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass); break;
+  case 4: launch_wave_g<Word, 4>(s, dim3(blocks), g, b, k0, k1, seed, pass); break;
+  case 8: launch_wave_g<Word, 8>(s, dim3(blocks), g, b, k0, k1, seed, pass); break;
+  default: launch_wave_g<Word, 2>(s, dim3(blocks), g, b, k0, k1, seed, pass); break;
   }
 }
 
@@ -752,6 +792,14 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
     return hipGetLastError();
   }
   unsigned blocks = k1 - k0;
+  if (blocks <= 256) {
+    // few targets: a 1024-lane workgroup per target shortens each visit's latency
+    if (wide)
+      hipLaunchKernelGGL((k_synth_batched<uint64_t, 16>), dim3(blocks), dim3(1024), 0, s, g, b, k0, k1, seed, pass);
+    else
+      hipLaunchKernelGGL((k_synth_batched<uint32_t, 16>), dim3(blocks), dim3(1024), 0, s, g, b, k0, k1, seed, pass);
+    return hipGetLastError();
+  }
   if (wide)
     hipLaunchKernelGGL((k_synth_batched<uint64_t, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
                        pass);
@@ -844,7 +892,7 @@ __global__ void k_adapt_simple(const unsigned char *__restrict__ image, size_t r
     unsigned char cm = mask2 ? mask2[y * mask2RowBytes + x] : (unsigned char)~m;
     bool hasValue = useContext && m == 0 && opaque;
     tword[p] = chans | (Word)(hasValue ? 1u : 0u);
-    cword[p] = chans | (Word)(cm == 0xFF ? 0xFFu : 0u);
+    cword[p] = chans | (Word)(cm == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
     tflag[p] = m != 0;
     cflag[p] = (cm == 0xFF) && opaque;
   }
@@ -892,7 +940,7 @@ __global__ void k_adapt_maps(const unsigned char *__restrict__ pix, int w, int h
       word[p] = chans | (Word)(hasValue ? 1u : 0u);
       flag[p] = m != 0;
     } else {
-      word[p] = chans | (Word)(m == 0xFF ? 0xFFu : 0u);
+      word[p] = chans | (Word)(m == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
       flag[p] = (m == 0xFF) && opaque;
     }
   }
@@ -1136,13 +1184,13 @@ hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw,
 
 // firstIdx[rho] = #{(x,y) : |x| <= w-1, |y| <= h-1, x^2 + y^2 < rho^2}: the position in the
 // sorted offsets where distance rho begins (the sort key is x^2 + y^2).
-__global__ void k_first_idx(int w, int h, unsigned rhoMax, uint32_t *firstIdx)
+__global__ __launch_bounds__(256) void k_first_idx(int w, int h, unsigned rhoMax, uint32_t *firstIdx)
 {
-  unsigned rho = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rho > rhoMax) return;
+  __shared__ unsigned long long part[4];
+  const unsigned rho = blockIdx.x;
   const long long r2 = (long long)rho * rho;
   unsigned long long cnt = 0;
-  for (int x = -(w - 1); x <= w - 1; x++) {
+  for (int x = -(w - 1) + (int)threadIdx.x; x <= w - 1; x += blockDim.x) {
     long long rem = r2 - (long long)x * x - 1;   // y^2 <= rem
     if (rem < 0) continue;
     long long ys = (long long)sqrt((double)rem);
@@ -1151,12 +1199,15 @@ __global__ void k_first_idx(int w, int h, unsigned rhoMax, uint32_t *firstIdx)
     if (ys > h - 1) ys = h - 1;
     cnt += 2 * ys + 1;
   }
-  firstIdx[rho] = (uint32_t)cnt;
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  if (lane_id() == 0) part[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) firstIdx[rho] = (uint32_t)(part[0] + part[1] + part[2] + part[3]);
 }
 
 hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx)
 {
-  hipLaunchKernelGGL(k_first_idx, dim3((rhoMax + 1 + 255) / 256), dim3(256), 0, s, w, h, rhoMax, firstIdx);
+  hipLaunchKernelGGL(k_first_idx, dim3(rhoMax + 1), dim3(256), 0, s, w, h, rhoMax, firstIdx);
   return hipGetLastError();
 }
 
