@@ -167,6 +167,9 @@ typedef struct {
   double synthKernelMs;               /* sum of synthesis kernel durations (timing != 0) */
   double prepMs, synthMs, totalMs;    /* host wall clock of the phases */
   unsigned int pixelWordBytes;        /* 4 or 8: bytes gathered per corpus pixel */
+  unsigned int windowLaunches;        /* batched: persistent sliding-window launches */
+  unsigned int windowWaitsExpired;    /* batched: window waits cut short by their spin limit
+                                         (0 unless the device misbehaves) */
 } ResynthStats;
 
 void resynth_set_default_mode(int mode);

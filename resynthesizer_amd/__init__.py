@@ -68,7 +68,8 @@ class ResynthStats(ctypes.Structure):
                 ("visits", ctypes.c_ulonglong), ("candidates", ctypes.c_ulonglong),
                 ("patchTerms", ctypes.c_ulonglong), ("meanError", ctypes.c_double), ("launches", ctypes.c_uint),
                 ("synthKernelMs", ctypes.c_double), ("prepMs", ctypes.c_double), ("synthMs", ctypes.c_double),
-                ("totalMs", ctypes.c_double), ("pixelWordBytes", ctypes.c_uint)]
+                ("totalMs", ctypes.c_double), ("pixelWordBytes", ctypes.c_uint),
+                ("windowLaunches", ctypes.c_uint), ("windowWaitsExpired", ctypes.c_uint)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

@@ -279,7 +279,7 @@ struct DeviceState {
   // workspaces
   Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
       tables, counters, compactTemp, perm, streamBuf, streamState, meanSum, image, mask, mask2, tpix, cpix, tileHas,
-      firstIdx;
+      firstIdx, wctl;
   PinnedBuf pinCounts, pinTables, pinPerm, pinStream;
   std::vector<hipEvent_t> events;
 };
@@ -525,6 +525,7 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   b.counters = cnt;
   b.tileHas = tileHas;
   b.firstIdx = firstIdx;
+  b.ctl = (uint32_t *)g_dev.wctl.get(WCTL_WORDS * sizeof(uint32_t));
 
   // pass schedule (lib/passes.h:68-93) and progress estimate (lib/progress.c:31-37)
   unsigned sizes[6];
@@ -589,6 +590,20 @@ int run_engine(Job &job, unsigned &nTargetsOut)
     long v = e ? std::atol(e) : 60;
     return (unsigned)std::min(1000L, std::max(1L, v));
   }();
+  // Sliding-window launches (RESYNTH_WINDOW=0 turns them off): once a chunk would reach
+  // RESYNTH_WINDOW_MIN targets, the rest of the pass runs as one persistent launch whose
+  // window is half the chunk the schedule above would use (a visit in a chunk misses half
+  // the chunk on average, a visit in a window up to the whole window).
+  static const int windowOn = [] {
+    const char *e = std::getenv("RESYNTH_WINDOW");
+    return e ? std::atoi(e) : 1;
+  }();
+  static const unsigned windowMin = [] {
+    const char *e = std::getenv("RESYNTH_WINDOW_MIN");
+    long v = e ? std::atol(e) : 2048;
+    return (unsigned)std::max(64L, v);
+  }();
+  unsigned windowLaunches = 0, windowExpired = 0;
   unsigned pass;
   unsigned passesRun = 0;
   unsigned bettersBefore = 0;
@@ -602,12 +617,17 @@ int run_engine(Job &job, unsigned &nTargetsOut)
     bool stop = false;
     while (k < end && !stop) {
       unsigned k1;
+      unsigned windowLen = 0;
       if (serial) {
         k1 = std::min(end, (k & ~kCallbackMask) + 4096u);
       } else {
         unsigned len = (pass == 0) ? std::max(minChunk, (unsigned)((unsigned long long)k * rampPermille / 1000u)) : cap;
         len = std::min(len, cap);
         k1 = std::min(end, k + len);
+        if (windowOn && len >= windowMin) {
+          k1 = end;
+          windowLen = len;
+        }
       }
       // progress points inside [k, k1)
       unsigned firstPoint = (k + kCallbackMask) & ~kCallbackMask;
@@ -647,6 +667,13 @@ int run_engine(Job &job, unsigned &nTargetsOut)
             kk = hState->kDone;
             if (kk < k1) refill();   // stopped for want of PRNG outputs
           }
+        } else if (windowLen) {
+          record_event(job, ev, true);
+          HIP_TRY(launch_synth_window(s, job.wide, g, b, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass,
+                                      windowLen / 2, pass == 0 ? rampPermille / 2 : 0, std::max(1u, cap / 2)));
+          record_event(job, ev, false);
+          launches++;
+          windowLaunches++;
         } else {
           record_event(job, ev, true);
           HIP_TRY(launch_synth_batched(s, job.wide, g, b, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
@@ -657,8 +684,11 @@ int run_engine(Job &job, unsigned &nTargetsOut)
       k = k1;
     }
     Counters hc;
+    uint32_t expired = 0;
     HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    if (windowLaunches) HIP_TRY(hipMemcpyAsync(&expired, b.ctl + WCTL_EXPIRED, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (expired) windowExpired++;
     unsigned bettersNow = 0;
     for (int i = 0; i < CNT_SLOTS; i++) bettersNow += hc.betters[i];
     const unsigned betters = bettersNow - bettersBefore;
@@ -672,6 +702,11 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   }
   job.st.passes = passesRun;
   job.st.launches = launches;
+  job.st.windowLaunches = windowLaunches;
+  job.st.windowWaitsExpired = windowExpired;
+  if (windowExpired)
+    std::fprintf(stderr, "resynth_mi355x: %u sliding-window pass(es) cut a wait short at its spin limit\n",
+                 windowExpired);
 
   // statistics
   {

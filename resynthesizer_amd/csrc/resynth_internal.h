@@ -60,7 +60,11 @@ struct SynthBuffers {
   Counters *counters;
   uint8_t *tileHas;       // [tilesH*tilesW] 1 when the tile holds a pixel with a value
   const uint32_t *firstIdx; // [rhoMax+1] number of sorted offsets with x^2+y^2 < rho^2
+  uint32_t *ctl;          // [WCTL_WORDS] sliding-window launch control
 };
+
+// Control words of a sliding-window launch, each on its own 128-byte line.
+enum { WCTL_TICKET = 0, WCTL_DONE = 32, WCTL_EXPIRED = 64, WCTL_WORDS = 96 };
 
 // State of the exact (serial) PRNG stream consumed on the device.
 struct StreamState {
@@ -89,6 +93,9 @@ hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32
 hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);
 hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                 unsigned k1, unsigned seed, unsigned pass);
+hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                               unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,
+                               unsigned cap);
 hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, int prngKind, const uint32_t *stream, uint32_t streamLen,
                                StreamState *state);

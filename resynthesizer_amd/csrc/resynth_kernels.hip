@@ -600,6 +600,141 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_batched(Geometry g, SynthB
   }
 }
 
+// Stores of a visit's result that other visits of the same launch read.  With PUBLISH
+// they are device-coherent stores (written through to memory), so that visits running on
+// other XCDs of a long persistent launch see them as soon as their own cached copy of the
+// line ages out, instead of when this XCD's L2 happens to evict the line.
+template <bool PUBLISH, typename T>
+__device__ __forceinline__ void put(T *p, T v)
+{
+  if constexpr (PUBLISH)
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    *p = v;
+}
+
+// One batched visit of target k by one wavefront (lib/synthesize.h:426-642).
+template <typename Word, int G0, int GL, bool PUBLISH>
+__device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers &b, unsigned k, unsigned seed,
+                                           unsigned pass, int2 *sOff, Word *sPix, int2 *sSrc, const uint16_t *sT,
+                                           const uint32_t *sTm, unsigned long long &myEval,
+                                           unsigned long long &myTerm, unsigned &myBet)
+{
+  const unsigned lane = lane_id();
+  Word *tword = (Word *)b.tword;
+  const Word *cword = (const Word *)b.cword;
+  const bool skip0 = g.nMaps == 0;
+  const int2 t = b.targets[k];
+  const unsigned count =
+      gather_neighbours<Word>(g, tword, b.sourceOf, b.offsets, t.x, t.y, sOff, sPix, sSrc, b.tileHas, b.firstIdx);
+  uint32_t best = 0xFFFFFFFFu;
+  int winX = 0, winY = 0;
+  bool perfect = false, scored = false, record = false;
+  int cx, cy;
+  unsigned evals = 0;
+  coherence_phase<Word>(g, cword, b.prober, k, count, sOff, sPix, sSrc, sT, sTm, best, winX, winY, perfect, scored,
+                        record, cx, cy, evals);
+  if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
+  // random probes from a per-target counter-based stream (seed, pass, target index,
+  // attempt).  Dense corpora draw a uniform pixel of the corpus image and keep it when
+  // its word marks a corpus point (0xFF: selected and opaque), which is uniform over
+  // corpus points and lets the validity test ride in the first group of loads; sparse
+  // corpora index the corpus point list.  Draws are resolved in attempt order.
+  const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
+  const bool byPixel = g.probeByPixel != 0;
+  const uint64_t nPix = (uint64_t)g.cw * (uint64_t)g.ch;
+  unsigned drawn = 0;
+  for (unsigned attempt = 0; drawn < g.maxProbe && !perfect; attempt += 64) {
+    const uint64_t h = mix64(key * 0x100000001B3ull + attempt + lane);
+    int2 c;
+    bool valid;
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    if (byPixel) {
+      const uint64_t p = ((h >> 32) * nPix) >> 32;
+      c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
+      // the candidate's own word (neighbour 0, loaded with the first group) carries the
+      // corpus-point flag
+      const Word w0 = cword[p];
+      score_candidate<Word, G0 + 1, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, false);
+      valid = (w0 & 0xFFu) == 0xFFu;
+    } else {
+      const unsigned idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
+      c = b.corpusPts[idx];
+      valid = true;
+      score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
+    }
+    const uint64_t vm = __ballot(valid);
+    const bool active = valid && drawn + (unsigned)__popcll(vm & lanemask_lt()) < g.maxProbe;
+    int win = -1, stop = 64;
+    uint32_t bb = best;
+    resolve(active && alive, M, F, bb, win, stop);
+    if (win >= 0) {
+      best = bb;
+      winX = __shfl(c.x, win);
+      winY = __shfl(c.y, win);
+      scored = true;
+    }
+    uint64_t used = __ballot(active);
+    if (stop < 64) {
+      perfect = true;
+      used &= (stop == 63) ? ~0ull : ((2ull << stop) - 1ull);
+    }
+    evals += (unsigned)__popcll(used);
+    drawn += (unsigned)__popcll(__ballot(active));
+  }
+  // store (lib/synthesize.h:620-639)
+  if (lane == 0) {
+    const size_t tp = (size_t)t.y * (unsigned)g.tw + (unsigned)t.x;
+    const Word old = tword[tp];
+    Word nw = old | (Word)1;
+    if (scored) {
+      const int2 cur = b.sourceOf[tp];
+      if (cur.x != winX || cur.y != winY) {
+        Word cmask = 0;
+        for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
+        const Word src = cword[(size_t)winY * (unsigned)g.cw + (unsigned)winX];
+        nw = (old & ~cmask) | (src & cmask) | (Word)1;
+        const uint64_t packed = (uint64_t)(uint32_t)winX | ((uint64_t)(uint32_t)winY << 32);
+        put<PUBLISH>((uint64_t *)&b.sourceOf[tp], packed);
+        myBet++;
+      }
+    }
+    put<PUBLISH>(&tword[tp], nw);
+    b.lastBest[tp] = best;
+    if (g.skipEmpty) b.tileHas[(size_t)(t.y / kTile) * g.tilesW + t.x / kTile] = 1;
+  }
+  myEval += evals;
+  myTerm += (unsigned long long)evals * count;
+}
+
+template <typename Word, int WAVES>
+__device__ __forceinline__ void flush_counters(const SynthBuffers &b, unsigned long long myEval,
+                                               unsigned long long myTerm, unsigned myBet,
+                                               unsigned long long *sEval, unsigned long long *sTerm, unsigned *sBet)
+{
+  const int wave = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sEval[wave] = myEval;
+    sTerm[wave] = myTerm;
+    sBet[wave] = myBet;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long e = 0, tm = 0;
+    unsigned bt = 0;
+    for (int w = 0; w < WAVES; w++) {
+      e += sEval[w];
+      tm += sTerm[w];
+      bt += sBet[w];
+    }
+    const unsigned slot = blockIdx.x & (CNT_SLOTS - 1);
+    if (e) atomicAdd(&b.counters->candidates[slot], e);
+    if (tm) atomicAdd(&b.counters->terms[slot], tm);
+    if (bt) atomicAdd(&b.counters->betters[slot], bt);
+  }
+}
+
 // Batched visits, one wavefront per target (large chunks: every wave stays busy on its
 // own target instead of idling through another wave's gather and heuristic-1 phases).
 template <typename Word, int WAVES, int G0, int GL>
@@ -619,116 +754,86 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuff
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6;
-  const unsigned lane = lane_id();
-  Word *tword = (Word *)b.tword;
-  const Word *cword = (const Word *)b.cword;
-  const bool skip0 = g.nMaps == 0;
   unsigned long long myEval = 0, myTerm = 0;
   unsigned myBet = 0;
-  for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES) {
-    const int2 t = b.targets[k];
-    const unsigned count = gather_neighbours<Word>(g, tword, b.sourceOf, b.offsets, t.x, t.y, sOff[wave],
-                                                   sPix[wave], sSrc[wave], b.tileHas, b.firstIdx);
-    uint32_t best = 0xFFFFFFFFu;
-    int winX = 0, winY = 0;
-    bool perfect = false, scored = false, record = false;
-    int cx, cy;
-    unsigned evals = 0;
-    coherence_phase<Word>(g, cword, b.prober, k, count, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, best, winX,
-                          winY, perfect, scored, record, cx, cy, evals);
-    if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
-    // random probes from a per-target counter-based stream (seed, pass, target index,
-    // attempt).  Dense corpora draw a uniform pixel of the corpus image and keep it when
-    // its word marks a corpus point (0xFF: selected and opaque), which is uniform over
-    // corpus points and lets the validity test ride in the first group of loads; sparse
-    // corpora index the corpus point list.  Draws are resolved in attempt order.
-    const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
-    const bool byPixel = g.probeByPixel != 0;
-    const uint64_t nPix = (uint64_t)g.cw * (uint64_t)g.ch;
-    unsigned drawn = 0;
-    for (unsigned attempt = 0; drawn < g.maxProbe && !perfect; attempt += 64) {
-      const uint64_t h = mix64(key * 0x100000001B3ull + attempt + lane);
-      int2 c;
-      bool valid;
-      uint32_t F = 0, M = 0;
-      bool alive = false;
-      if (byPixel) {
-        const uint64_t p = ((h >> 32) * nPix) >> 32;
-        c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
-        // the candidate's own word (neighbour 0, loaded with the first group) carries the
-        // corpus-point flag
-        const Word w0 = cword[p];
-        score_candidate<Word, G0 + 1, GL>(g, cword, c.x, c.y, count, sOff[wave], sPix[wave], sT, sTm, best, F, M,
-                                          alive, false);
-        valid = (w0 & 0xFFu) == 0xFFu;
-      } else {
-        const unsigned idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
-        c = b.corpusPts[idx];
-        valid = true;
-        score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff[wave], sPix[wave], sT, sTm, best, F, M, alive,
-                                      skip0);
-      }
-      const uint64_t vm = __ballot(valid);
-      const bool active = valid && drawn + (unsigned)__popcll(vm & lanemask_lt()) < g.maxProbe;
-      int win = -1, stop = 64;
-      uint32_t bb = best;
-      resolve(active && alive, M, F, bb, win, stop);
-      if (win >= 0) {
-        best = bb;
-        winX = __shfl(c.x, win);
-        winY = __shfl(c.y, win);
-        scored = true;
-      }
-      uint64_t used = __ballot(active);
-      if (stop < 64) {
-        perfect = true;
-        used &= (stop == 63) ? ~0ull : ((2ull << stop) - 1ull);
-      }
-      evals += (unsigned)__popcll(used);
-      drawn += (unsigned)__popcll(__ballot(active));
-    }
-    // store (lib/synthesize.h:620-639)
-    if (lane == 0) {
-      const size_t tp = (size_t)t.y * (unsigned)g.tw + (unsigned)t.x;
-      const Word old = tword[tp];
-      Word nw = old | (Word)1;
-      if (scored) {
-        const int2 cur = b.sourceOf[tp];
-        if (cur.x != winX || cur.y != winY) {
-          Word cmask = 0;
-          for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
-          const Word src = cword[(size_t)winY * (unsigned)g.cw + (unsigned)winX];
-          nw = (old & ~cmask) | (src & cmask) | (Word)1;
-          b.sourceOf[tp] = make_int2(winX, winY);
-          myBet++;
-        }
-      }
-      tword[tp] = nw;
-      b.lastBest[tp] = best;
-      if (g.skipEmpty) b.tileHas[(size_t)(t.y / kTile) * g.tilesW + t.x / kTile] = 1;
-    }
-    myEval += evals;
-    myTerm += (unsigned long long)evals * count;
-  }
-  if (lane == 0) {
-    sEval[wave] = myEval;
-    sTerm[wave] = myTerm;
-    sBet[wave] = myBet;
+  for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES)
+    visit_wave<Word, G0, GL, false>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval, myTerm,
+                                    myBet);
+  flush_counters<Word, WAVES>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
+}
+
+// Sliding-window batched visits in one persistent launch.  Waves take targets in
+// synthesis order from a ticket counter; target k starts once all but W of the targets
+// before it in the launch have finished, where W = max(minW, k * rampPermille / 1000)
+// (capped at cap) when rampPermille > 0, else cap.  A visit therefore misses the results
+// of at most W earlier visits, as a chunk of about 2W would, but no wave idles at chunk
+// ends.  Every wait is on lower tickets, held by waves that are already running, so the
+// lowest unfinished ticket never waits and the launch always drains; a spin limit bounds
+// each wait all the same, and after one expiry (flagged in ctl) waits are skipped.
+constexpr unsigned kSpinLimit = 1u << 18;
+
+__device__ __forceinline__ unsigned load_agent(const unsigned *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename Word, int WAVES, int G0, int GL>
+__global__ __launch_bounds__(64 * WAVES) void k_synth_window(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
+                                                             unsigned seed, unsigned pass, unsigned minW,
+                                                             unsigned rampPermille, unsigned cap)
+{
+  __shared__ uint16_t sT[512];
+  __shared__ uint32_t sTm[512];
+  __shared__ int2 sOff[WAVES][64];
+  __shared__ Word sPix[WAVES][64];
+  __shared__ int2 sSrc[WAVES][64];
+  __shared__ unsigned long long sEval[WAVES], sTerm[WAVES];
+  __shared__ unsigned sBet[WAVES];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    sT[i] = b.tables->T[i];
+    sTm[i] = b.tables->Tm[i];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long e = 0, tm = 0;
-    unsigned bt = 0;
-    for (int w = 0; w < WAVES; w++) {
-      e += sEval[w];
-      tm += sTerm[w];
-      bt += sBet[w];
+  const int wave = threadIdx.x >> 6;
+  const unsigned lane = lane_id();
+  unsigned *ticket = b.ctl + WCTL_TICKET, *done = b.ctl + WCTL_DONE, *expired = b.ctl + WCTL_EXPIRED;
+  const unsigned n = k1 - k0;
+  unsigned long long myEval = 0, myTerm = 0;
+  unsigned myBet = 0;
+  unsigned seen = 0;  // a lower bound of *done, which only grows
+  for (;;) {
+    unsigned tk = 0;
+    if (lane == 0) tk = atomicAdd(ticket, 1u);
+    tk = __builtin_amdgcn_readfirstlane(tk);
+    if (tk >= n) break;
+    const unsigned k = k0 + tk;
+    unsigned W = cap;
+    if (rampPermille) {
+      const unsigned r = (unsigned)((uint64_t)k * rampPermille / 1000u);
+      W = r > minW ? r : minW;
+      if (W > cap) W = cap;
     }
-    const unsigned slot = blockIdx.x & (CNT_SLOTS - 1);
-    if (e) atomicAdd(&b.counters->candidates[slot], e);
-    if (tm) atomicAdd(&b.counters->terms[slot], tm);
-    if (bt) atomicAdd(&b.counters->betters[slot], bt);
+    if (tk > W && seen < tk - W) {
+      const unsigned need = tk - W;
+      if (lane == 0) {
+        seen = load_agent(done);
+        unsigned spins = 0;
+        while (seen < need) {
+          if (++spins > kSpinLimit || load_agent(expired)) {
+            atomicOr(expired, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          seen = load_agent(done);
+        }
+      }
+      seen = __builtin_amdgcn_readfirstlane(seen);
+    }
+    visit_wave<Word, G0, GL, true>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval, myTerm,
+                                   myBet);
+    if (lane == 0) atomicAdd(done, 1u);
   }
+  flush_counters<Word, WAVES>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
 }
 
 // Chunk sizes at or above this use one wavefront per target (env RESYNTH_WAVE_MIN), and
@@ -807,6 +912,49 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
     hipLaunchKernelGGL((k_synth_batched<uint32_t, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
                        pass);
   return hipGetLastError();
+}
+
+template <typename Word, int GL>
+static hipError_t launch_window_t(hipStream_t s, const Geometry &g, const SynthBuffers &b, unsigned k0, unsigned k1,
+                                  unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille, unsigned cap)
+{
+  constexpr int WAVES = 4;
+  constexpr int G0 = 2;
+  // one resident grid: blocks beyond what the device holds at once would only start
+  // after the tickets have run out
+  static int resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_synth_window<Word, WAVES, G0, GL>, 64 * WAVES, 0) !=
+            hipSuccess || per < 1)
+      per = 1;
+    return cus * per;
+  }();
+  unsigned blocks = (k1 - k0 + WAVES - 1) / WAVES;
+  if (blocks > (unsigned)resident) blocks = (unsigned)resident;
+  hipLaunchKernelGGL((k_synth_window<Word, WAVES, G0, GL>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
+                     pass, minW, rampPermille, cap);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
+                               unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,
+                               unsigned cap)
+{
+  if (k1 <= k0) return hipSuccess;
+  if (!b.ctl) return hipErrorInvalidValue;
+  static const int later = [] {
+    const char *e = std::getenv("RESYNTH_LATER_GROUP");
+    return (e && std::atoi(e) == 8) ? 8 : 4;
+  }();
+  hipError_t e = launch_fill_u32(s, b.ctl, 0u, WCTL_WORDS);
+  if (e != hipSuccess) return e;
+  if (wide)
+    return later == 8 ? launch_window_t<uint64_t, 8>(s, g, b, k0, k1, seed, pass, minW, rampPermille, cap)
+                      : launch_window_t<uint64_t, 4>(s, g, b, k0, k1, seed, pass, minW, rampPermille, cap);
+  return later == 8 ? launch_window_t<uint32_t, 8>(s, g, b, k0, k1, seed, pass, minW, rampPermille, cap)
+                    : launch_window_t<uint32_t, 4>(s, g, b, k0, k1, seed, pass, minW, rampPermille, cap);
 }
 
 // Serial visits: one workgroup visits targets k0..k1-1 in order (lib/refiner.h:75-100),

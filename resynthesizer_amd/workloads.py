@@ -97,3 +97,34 @@ def config2(seed=2, size=2048, mask_seed=CONFIG2_MASK_SEED):
     s = size / 2048.0
     m = ellipse_mask(size, size, 12, mask_seed, axes=(60 * s, 260 * s))
     return img, (m * 255).astype(np.uint8)
+
+
+def luminance(rgb):
+    """8-bit luminance (299 R + 587 G + 114 B + 500) / 1000."""
+    r, g, b = (rgb[..., i].astype(np.int64) for i in range(3))
+    return ((299 * r + 587 * g + 114 * b + 500) // 1000).astype(np.uint8)
+
+
+# Format indices of the map-guided transfer: pixels [mask, R, G, B, map] (colours 1..3,
+# one map channel at 4; lib/imageFormatIndicies.h:46-58).
+CONFIG4_INDICES = dict(colorEndBip=4, alpha_bip=4, map_start_bip=4, map_end_bip=5, img_match_bpp=3,
+                       map_match_bpp=1, total_bpp=5, isAlphaTarget=0, isAlphaSource=0)
+
+
+def config4(seed=4, corpus_size=768, target_size=1536):
+    """Map-guided texture transfer in the engine's pixmap layout [mask, R, G, B, map]:
+    corpus = value-noise RGB texture with its luminance as the map, every pixel selected;
+    target = a whole canvas (every pixel a target, no context) whose map is a radial
+    gradient blended with 2-octave value noise.  Returns (target, corpus) uint8 pixmaps."""
+    rgb = value_noise(corpus_size, corpus_size, 3, seed)
+    corpus = np.concatenate([np.full((corpus_size, corpus_size, 1), 255, np.uint8), rgb,
+                             luminance(rgb)[..., None]], axis=2)
+    n = target_size
+    y, x = np.mgrid[0:n, 0:n].astype(np.float64)
+    r = np.sqrt((x - n / 2) ** 2 + (y - n / 2) ** 2) / (n / np.sqrt(2))
+    noise = value_noise(n, n, 1, seed + 1, octaves=2, base_cell=max(n // 12, 2))[..., 0] / 255.0
+    tmap = np.clip(np.rint(255.0 * (0.7 * (1.0 - r) + 0.3 * noise)), 0, 255).astype(np.uint8)
+    target = np.zeros((n, n, 5), np.uint8)
+    target[..., 0] = 255
+    target[..., 4] = tmap
+    return target, np.ascontiguousarray(corpus)

@@ -275,3 +275,73 @@ def test_batched_invariants_and_quality_config1():
     ratio = np.mean(bat) / np.mean(ser)
     print("config 1 mean error: batched %.0f serial %.0f ratio %.4f" % (np.mean(bat), np.mean(ser), ratio))
     assert ratio <= 1.05, (bat, ser)
+
+
+def test_batched_window_launches_config2_768():
+    """Config 2's generators at 768x768 (about 75k targets): the later passes and the rest
+    of pass 0 run as persistent sliding-window launches.  Context pixels stay untouched,
+    every target takes a corpus colour, no window wait hits its spin limit, and the mean
+    best-match error stays close to the serial engine's (2 PRNG seeds each side)."""
+    img, mask = W.config2(size=768)
+    sel = mask != 0
+    p = R.default_params()
+    corpus_colours = {tuple(c) for c in img[~sel][:, :3]}
+    ser = []
+    for s in (0, 11):
+        r, _ = R.image_synth(img, mask, R.T_RGBA, p, mode=R.MODE_SERIAL, seed=s)
+        assert r.err == 0
+        ser.append(r.meanError)
+    bat = []
+    for s in (1, 2):
+        r, out = R.image_synth(img, mask, R.T_RGBA, p, mode=R.MODE_BATCHED, seed=s)
+        assert r.err == 0
+        assert r.stats["windowLaunches"] >= 2, r.stats
+        assert r.stats["windowWaitsExpired"] == 0
+        assert np.array_equal(out[~sel], img[~sel])
+        assert np.array_equal(out[sel][:, 3], img[sel][:, 3])
+        assert all(tuple(c) in corpus_colours for c in out[sel][::7, :3])
+        bat.append(r.meanError)
+    ratio = np.mean(bat) / np.mean(ser)
+    print("768 mean error: batched %.0f serial %.0f ratio %.4f" % (np.mean(bat), np.mean(ser), ratio))
+    assert ratio <= 1.05, (bat, ser)
+
+
+# ---------------------------------------------------------------- map-guided transfer (engine)
+
+def test_serial_map_transfer_matches_oracle():
+    """Config 4's map-guided transfer through engine() at reduced size (corpus 48x48,
+    whole 72x72 target canvas with no context): the serial mode equals the oracle byte for
+    byte, the map term included."""
+    tp, cp = W.config4(seed=4, corpus_size=48, target_size=72)
+    p = R.default_params(mapWeight=0.5)
+    ix = R.prepare_indices(3, 1, False, False, True)
+    assert (ix.colorEndBip, ix.map_start_bip, ix.map_end_bip, ix.total_bpp) == (4, 4, 5, 5)
+    rg, og = R.engine(p, ix, tp, cp, mode=R.MODE_SERIAL)
+    rc, oc = O.engine(_oparams(p), O.prepare_indices(3, 1, False, False, True), tp, cp)
+    assert rg.err == rc.err == 0
+    assert rg.passes == rc.passes and rg.betters == rc.betters
+    assert rg.nTargets == 72 * 72
+    assert np.array_equal(og, oc)
+    assert np.array_equal(og[..., 4], tp[..., 4])     # maps are matched, never written
+
+
+def test_batched_map_transfer_config4():
+    """Config 4 at full size (768x768 corpus, 1536x1536 target canvas): every target pixel
+    gets a corpus colour, the target map is untouched, and the batched error stays close to
+    the serial engine's on a 96x96 crop of the same problem."""
+    tp, cp = W.config4()
+    p = R.default_params(mapWeight=0.5)
+    ix = R.prepare_indices(3, 1, False, False, True)
+    r, out = R.engine(p, ix, tp, cp, mode=R.MODE_BATCHED, seed=1)
+    assert r.err == 0 and r.nTargets == 1536 * 1536
+    assert r.stats["windowWaitsExpired"] == 0
+    assert np.array_equal(out[..., 4], tp[..., 4])
+    corpus_colours = {tuple(c) for c in cp[..., 1:4].reshape(-1, 3)}
+    assert all(tuple(c) in corpus_colours for c in out[..., 1:4].reshape(-1, 3)[::997])
+    assert np.isfinite(r.meanError) and r.meanError > 0
+    ts, cs = W.config4(corpus_size=96, target_size=96)
+    ser = [R.engine(p, ix, ts, cs, mode=R.MODE_SERIAL, seed=s)[0].meanError for s in (0, 11)]
+    bat = [R.engine(p, ix, ts, cs, mode=R.MODE_BATCHED, seed=s)[0].meanError for s in (1, 2)]
+    ratio = np.mean(bat) / np.mean(ser)
+    print("config 4 crop mean error: batched %.0f serial %.0f ratio %.4f" % (np.mean(bat), np.mean(ser), ratio))
+    assert ratio <= 1.08, (bat, ser)
