@@ -196,6 +196,58 @@ int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsig
                                void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
                                const ResynthOptions *options, ResynthStats *stats);
 
+/* ---- Many requests in one launch.  No counterpart in the reference (each imageSynth call
+ * there is one request); for servers healing many images of uneven sizes at once.  Every
+ * request is an imageSynth call (same contract, mask semantics and error codes), all run
+ * in batched mode by one persistent multi-request launch, captured in a HIP graph. ---- */
+typedef struct {
+  ImageBuffer image;              /* pixels synthesized in place (host or device memory) */
+  ImageBuffer mask;               /* image-sized selection; nonzero = synthesize */
+  TImageFormat format;
+  TImageSynthParameters params;
+  int error;                      /* out: what imageSynth would return for this request */
+  unsigned int nTargets;          /* out */
+  unsigned int passes;            /* out */
+  double meanError;               /* out: mean best-match sum of the targets' last visits */
+  unsigned long long candidates;  /* out: scorer invocations */
+} ResynthRequest;
+
+typedef struct {
+  int wavesPerBlock;              /* targets per workgroup, one wavefront each: 2, 4 or 8 */
+  int firstGroup;                 /* corpus loads issued together for a candidate's first
+                                     neighbours: 2 or 4 */
+  int laterGroup;                 /* loads issued together for the rest: 4 or 8 */
+} ResynthKernelConfig;
+
+typedef struct {
+  unsigned int requests;          /* requests synthesized (error 0) */
+  unsigned long long targets;
+  ResynthKernelConfig config;     /* kernel shape used */
+  unsigned long long candidates, patchTerms;
+  double prepMs;                  /* host wall clock of every request's preparation */
+  double synthKernelMs;           /* the multi-request launch(es), HIP events */
+  double totalMs;
+  unsigned int graphLaunches;     /* one per pixel-word width present */
+  unsigned int waitsExpired;      /* waits cut short by their spin limit (0 normally) */
+} ResynthBatchStats;
+
+/* Runs n requests.  deviceBuffers != 0: image.data and mask.data are device pointers.
+ * options may be NULL (batched defaults; the mode field is ignored: always batched);
+ * config may be NULL (the autotuned shape if resynth_autotune_batch ran, else 4/2/4).
+ * Returns 0, or -1 on invalid arguments or no device; per-request results are in
+ * requests[i].error and the other out fields. */
+int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int deviceBuffers,
+                              const ResynthOptions *options, const ResynthKernelConfig *config,
+                              ResynthBatchStats *stats);
+
+/* Autotuner of the multi-request launch: times every kernel shape (wavesPerBlock x
+ * firstGroup x laterGroup, 12 shapes) on copies of the sample requests, keeps the fastest
+ * for later resynth_image_synth_batch calls and returns it in *best.  msPerConfig (may be
+ * NULL) receives the 12 timings in the order waves {2,4,8} x first {2,4} x later {4,8}.
+ * The sample's images are left unchanged.  Returns 0, or -1 on invalid arguments. */
+int resynth_autotune_batch(const ResynthRequest *sample, unsigned int n, int deviceBuffers,
+                           ResynthKernelConfig *best, double *msPerConfig);
+
 /* Frozen-state scorer (the computeBestFit loop of lib/synthesize.h:266-400 over an
  * ordered candidate list, strict-less update, stop at the first perfect match) for
  * nProblems independent problems, evaluated on the GPU.  Host pointers:
@@ -222,6 +274,12 @@ void resynth_prng_sample(int prng, unsigned int seed, unsigned int n, uint32_t *
 void resynth_prng_ranges(int prng, unsigned int seed, unsigned int n, const uint32_t *dists, uint32_t *out);
 void resynth_metric_tables(double sensitivity, double mapWeight, uint16_t *T, uint32_t *Tm);
 int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsigned int n);
+
+/* sizeof of ResynthOptions, ResynthStats, ResynthRequest, ResynthKernelConfig and
+ * ResynthBatchStats, then the offsets of ResynthRequest.params, ResynthRequest.error and
+ * ResynthBatchStats.prepMs, into out[0..min(n, 8)); returns how many were written.  Lets
+ * a binding check its mirror of these structs. */
+unsigned int resynth_struct_sizes(size_t *out, unsigned int n);
 
 /* Library/device information: returns 0 when a gfx950 device is usable. */
 int resynth_device_info(char *name, size_t nameLen, int *computeUnits);

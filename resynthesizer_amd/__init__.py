@@ -78,6 +78,29 @@ class ResynthStats(ctypes.Structure):
         return d
 
 
+class ResynthRequest(ctypes.Structure):
+    _fields_ = [("image", ImageBuffer), ("mask", ImageBuffer), ("format", ctypes.c_int),
+                ("params", TImageSynthParameters), ("error", ctypes.c_int), ("nTargets", ctypes.c_uint),
+                ("passes", ctypes.c_uint), ("meanError", ctypes.c_double), ("candidates", ctypes.c_ulonglong)]
+
+
+class ResynthKernelConfig(ctypes.Structure):
+    _fields_ = [("wavesPerBlock", ctypes.c_int), ("firstGroup", ctypes.c_int), ("laterGroup", ctypes.c_int)]
+
+
+class ResynthBatchStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint), ("targets", ctypes.c_ulonglong), ("config", ResynthKernelConfig),
+                ("candidates", ctypes.c_ulonglong), ("patchTerms", ctypes.c_ulonglong), ("prepMs", ctypes.c_double),
+                ("synthKernelMs", ctypes.c_double), ("totalMs", ctypes.c_double),
+                ("graphLaunches", ctypes.c_uint), ("waitsExpired", ctypes.c_uint)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        c = self.config
+        d["config"] = (c.wavesPerBlock, c.firstGroup, c.laterGroup)
+        return d
+
+
 PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p)
 
 # Every function declared in include/resynth_mi355x.h, with its ctypes signature.
@@ -115,6 +138,12 @@ SIGNATURES = {
                                                   ctypes.c_int, _P(TImageSynthParameters), PROGRESS,
                                                   ctypes.c_void_p, _P(ctypes.c_int), _P(ResynthOptions),
                                                   _P(ResynthStats)]),
+    "resynth_image_synth_batch": (ctypes.c_int, [_P(ResynthRequest), ctypes.c_uint, ctypes.c_int,
+                                                 _P(ResynthOptions), _P(ResynthKernelConfig),
+                                                 _P(ResynthBatchStats)]),
+    "resynth_autotune_batch": (ctypes.c_int, [_P(ResynthRequest), ctypes.c_uint, ctypes.c_int,
+                                              _P(ResynthKernelConfig), ctypes.c_void_p]),
+    "resynth_struct_sizes": (ctypes.c_uint, [ctypes.c_void_p, ctypes.c_uint]),
     "resynth_score_frozen": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, _P(TFormatIndices),
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
@@ -268,6 +297,65 @@ def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes
                                            mask2_row_bytes, fmt, pp, cb, None, ctypes.byref(cf), ctypes.byref(opt),
                                            ctypes.byref(st))
     return Result(err, st, log)
+
+
+def _requests(images, masks, fmts, params, device):
+    n = len(images)
+    reqs = (ResynthRequest * n)()
+    keep = []
+    for i in range(n):
+        if device:
+            img, m = images[i], masks[i]     # (data_ptr, width, height, rowBytes) tuples
+            reqs[i].image = ImageBuffer(*img)
+            reqs[i].mask = ImageBuffer(*m)
+        else:
+            img = np.ascontiguousarray(images[i], dtype=np.uint8)
+            m = np.ascontiguousarray(masks[i], dtype=np.uint8)
+            keep += [img, m]
+            reqs[i].image = ImageBuffer(img.ctypes.data, img.shape[1], img.shape[0], img.strides[0])
+            reqs[i].mask = ImageBuffer(m.ctypes.data, m.shape[1], m.shape[0], m.strides[0])
+        reqs[i].format = fmts[i]
+        reqs[i].params = params[i] if params[i] is not None else default_params()
+    return reqs, keep
+
+
+def image_synth_batch(images, masks, fmts, params, device=False, config=None, seed=0, batch=0):
+    """Many imageSynth requests in one multi-request launch.  Host mode: images are uint8
+    (H, W, C) arrays (copied, the copies synthesized); device mode: images and masks are
+    (data_ptr, width, height, rowBytes) tuples synthesized in place.  config: None or
+    (wavesPerBlock, firstGroup, laterGroup).  Returns (rc, per-request dicts, stats dict,
+    output arrays in host mode)."""
+    if not device:
+        images = [np.array(a, dtype=np.uint8, copy=True, order="C") for a in images]
+    reqs, keep = _requests(images, masks, fmts, params, device)
+    cfg = ResynthKernelConfig(*config) if config else None
+    st = ResynthBatchStats()
+    opt = options(MODE_BATCHED, PRNG_GRAND, seed, batch, False)
+    rc = lib().resynth_image_synth_batch(reqs, len(images), int(device), ctypes.byref(opt),
+                                         ctypes.byref(cfg) if cfg else None, ctypes.byref(st))
+    out = [dict(error=r.error, nTargets=r.nTargets, passes=r.passes, meanError=r.meanError,
+                candidates=r.candidates) for r in reqs]
+    if not device:
+        out_imgs = keep[0::2]
+        return rc, out, st.as_dict(), out_imgs
+    return rc, out, st.as_dict(), None
+
+
+def autotune_batch(images, masks, fmts, params, device=False):
+    """Runs the autotuner on a sample of requests.  Returns (rc, best (waves, first, later),
+    the 12 timings in ms)."""
+    reqs, keep = _requests(images, masks, fmts, params, device)
+    best = ResynthKernelConfig()
+    ms = np.zeros(12, np.float64)
+    rc = lib().resynth_autotune_batch(reqs, len(images), int(device), ctypes.byref(best), ms.ctypes.data)
+    return rc, (best.wavesPerBlock, best.firstGroup, best.laterGroup), ms
+
+
+def struct_sizes():
+    """sizeof of the extension structs as the library was compiled, in header order."""
+    out = np.zeros(16, np.uint64)
+    n = lib().resynth_struct_sizes(out.ctypes.data, 16)
+    return [int(v) for v in out[:n]]
 
 
 def score_frozen(corpus_pixmap, ix, T, Tm, nb_counts, nb_offsets, nb_pixels, cand_counts, cands):

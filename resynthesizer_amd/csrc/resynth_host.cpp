@@ -11,8 +11,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -271,17 +273,22 @@ struct PinnedBuf {
   }
 };
 
+// Device and pinned buffers of one engine run (grow-only, reused across calls).
+struct Workspace {
+  Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
+      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, firstIdx, wctl;
+  PinnedBuf pinCounts, pinTables, pinPerm;
+};
+
 struct DeviceState {
   bool ready = false;
   bool ok = false;
   int device = 0;
   hipStream_t stream = nullptr;
-  // workspaces
-  Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
-      tables, counters, compactTemp, perm, streamBuf, streamState, meanSum, image, mask, mask2, tpix, cpix, tileHas,
-      firstIdx, wctl;
-  PinnedBuf pinCounts, pinTables, pinPerm, pinStream;
-  std::vector<hipEvent_t> events;
+  Workspace ws;                                   // single-request entries
+  std::vector<std::unique_ptr<Workspace>> pool;   // one per request of a multi-request call
+  Buf streamBuf, streamState, manyDesc, manyCtl, manyGlobal;
+  PinnedBuf pinStream, pinDesc, pinCtl;
 };
 
 std::mutex g_mutex;
@@ -358,6 +365,7 @@ struct Job {
   Progress prog;
   bool wide = false;
   int nColors = 0, nMaps = 0;
+  Workspace *W = nullptr;
 };
 
 void record_event(Job &job, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev, bool start)
@@ -370,11 +378,23 @@ void record_event(Job &job, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev, 
   else ev.back().second = e;
 }
 
-// The engine (lib/engine.c:539-690) on device buffers.  Returns a TImageSynthError;
-// the synthesized colours are in the target words (g_dev.tword) on success.
-int run_engine(Job &job, unsigned &nTargetsOut)
+// An engine run prepared on the device: point lists, sorted offsets, tables, target
+// order and the state maps, described by g and b.
+struct Prepared {
+  Geometry g = {};
+  SynthBuffers b = {};
+  unsigned sizes[6] = {};
+  unsigned nT = 0;
+  size_t wb = 4;
+  std::unique_ptr<Prng> prng;   // the engine's PRNG after the ordering draws
+};
+
+// Preparation of an engine run (lib/engine.c:539-660) on the job's workspace.  Returns a
+// TImageSynthError.
+int prepare_engine(Job &job, Prepared &pr)
 {
   hipStream_t s = g_dev.stream;
+  Workspace &W = *job.W;
   const TImageSynthParameters &P = job.P;
   const TFormatIndices &ix = job.ix;
   const double t0 = now_ms();
@@ -390,10 +410,10 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   const int useContext = P.matchContextType != 0;
   const bool serialMode = job.opt.mode == RESYNTH_MODE_SERIAL;
 
-  void *tword = g_dev.tword.get(nTpix * wb);
-  void *cword = g_dev.cword.get(std::max(nCpix, nTpix) * wb);
-  uint8_t *tflag = (uint8_t *)g_dev.tflag.get(nTpix + 64);
-  uint8_t *cflag = (uint8_t *)g_dev.cflag.get(std::max(nCpix, nTpix) + 64);
+  void *tword = W.tword.get(nTpix * wb);
+  void *cword = W.cword.get(std::max(nCpix, nTpix) * wb);
+  uint8_t *tflag = (uint8_t *)W.tflag.get(nTpix + 64);
+  uint8_t *cflag = (uint8_t *)W.cflag.get(std::max(nCpix, nTpix) + 64);
   if (S.simple) {
     HIP_TRY(launch_adapt_simple(s, job.wide, S.image, S.rowBytes, S.bpp, S.mask, S.maskRowBytes, S.mask2,
                                 S.mask2RowBytes, tw, th, job.nColors, S.alphaIdx, useContext, tword, cword, tflag,
@@ -405,14 +425,14 @@ int run_engine(Job &job, unsigned &nTargetsOut)
                               ix.map_end_bip, ix.isAlphaSource ? ix.alpha_bip : -1, useContext, 0, cword, cflag));
   }
   // row-major target and corpus point lists (lib/engine.c:338-431)
-  int2 *targetsRM = (int2 *)g_dev.targetsRM.get(nTpix * sizeof(int2) + 64);
-  int2 *corpusPts = (int2 *)g_dev.corpusPts.get(nCpix * sizeof(int2) + 64);
+  int2 *targetsRM = (int2 *)W.targetsRM.get(nTpix * sizeof(int2) + 64);
+  int2 *corpusPts = (int2 *)W.corpusPts.get(nCpix * sizeof(int2) + 64);
   size_t ctb = std::max(compact_temp_bytes(nTpix), compact_temp_bytes(nCpix));
-  char *ctemp = (char *)g_dev.compactTemp.get(2 * ctb + 256);
+  char *ctemp = (char *)W.compactTemp.get(2 * ctb + 256);
   unsigned *dCounts = (unsigned *)(ctemp + 2 * ctb);
   HIP_TRY(launch_compact(s, tflag, tw, th, targetsRM, dCounts + 0, ctemp));
   HIP_TRY(launch_compact(s, cflag, cw, ch, corpusPts, dCounts + 1, ctemp + ctb));
-  unsigned *hCounts = (unsigned *)g_dev.pinCounts.get(64);
+  unsigned *hCounts = (unsigned *)W.pinCounts.get(64);
   HIP_TRY(hipMemcpyAsync(hCounts, dCounts, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
   hipEvent_t countsReady;
   HIP_TRY(hipEventCreateWithFlags(&countsReady, hipEventDisableTiming));
@@ -424,17 +444,17 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   const int ow = std::min(cw, tw), oh = std::min(ch, th);
   const size_t nO = (size_t)(2 * ow - 1) * (size_t)(2 * oh - 1);
   job.st.nOffsets = (unsigned)nO;
-  int2 *offsets = (int2 *)g_dev.offsets.get(nO * sizeof(int2));
+  int2 *offsets = (int2 *)W.offsets.get(nO * sizeof(int2));
   size_t otb = offsets_temp_bytes(nO);
-  HIP_TRY(launch_sorted_offsets(s, ow, oh, offsets, g_dev.offTemp.get(otb), otb));
-  Tables *hT = (Tables *)g_dev.pinTables.get(sizeof(Tables));
+  HIP_TRY(launch_sorted_offsets(s, ow, oh, offsets, W.offTemp.get(otb), otb));
+  Tables *hT = (Tables *)W.pinTables.get(sizeof(Tables));
   quantize_tables(P.sensitivityToOutliers, P.mapWeight, *hT);
-  Tables *dT = (Tables *)g_dev.tables.get(sizeof(Tables));
+  Tables *dT = (Tables *)W.tables.get(sizeof(Tables));
   HIP_TRY(hipMemcpyAsync(dT, hT, sizeof(Tables), hipMemcpyHostToDevice, s));
-  int2 *sourceOf = (int2 *)g_dev.sourceOf.get(nTpix * sizeof(int2));
-  uint32_t *prober = (uint32_t *)g_dev.prober.get(nCpix * sizeof(uint32_t));
-  uint32_t *lastBest = (uint32_t *)g_dev.lastBest.get(nTpix * sizeof(uint32_t));
-  Counters *cnt = (Counters *)g_dev.counters.get(sizeof(Counters));
+  int2 *sourceOf = (int2 *)W.sourceOf.get(nTpix * sizeof(int2));
+  uint32_t *prober = (uint32_t *)W.prober.get(nCpix * sizeof(uint32_t));
+  uint32_t *lastBest = (uint32_t *)W.lastBest.get(nTpix * sizeof(uint32_t));
+  Counters *cnt = (Counters *)W.counters.get(sizeof(Counters));
   HIP_TRY(launch_fill_u32(s, (uint32_t *)sourceOf, 0xFFFFFFFFu, nTpix * 2));
   HIP_TRY(launch_fill_u32(s, prober, 0xFFFFFFFFu, nCpix));
   HIP_TRY(launch_fill_u32(s, lastBest, 0xFFFFFFFFu, nTpix));
@@ -443,11 +463,11 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   // skip provably empty offsets in pass-0 scans
   const int tilesW = (tw + kTile - 1) / kTile, tilesH = (th + kTile - 1) / kTile;
   const size_t nTiles = (size_t)tilesW * tilesH;
-  uint8_t *tileHas = (uint8_t *)g_dev.tileHas.get((nTiles + 3) / 4 * 4);
+  uint8_t *tileHas = (uint8_t *)W.tileHas.get((nTiles + 3) / 4 * 4);
   HIP_TRY(launch_fill_u32(s, (uint32_t *)tileHas, 0u, (nTiles + 3) / 4));
   HIP_TRY(launch_tile_init(s, job.wide, tword, tw, th, tileHas));
   const unsigned rhoMax = (unsigned)std::ceil(std::sqrt((double)(ow - 1) * (ow - 1) + (double)(oh - 1) * (oh - 1))) + 1;
-  uint32_t *firstIdx = (uint32_t *)g_dev.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
+  uint32_t *firstIdx = (uint32_t *)W.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
   HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
 
   HIP_TRY(hipEventSynchronize(countsReady));
@@ -455,7 +475,7 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   const unsigned nT = hCounts[0], nC = hCounts[1];
   job.st.nTargets = nT;
   job.st.nCorpus = nC;
-  nTargetsOut = nT;
+  pr.nT = nT;
   if (!nT || !nC) {
     HIP_TRY(hipStreamSynchronize(s));
     return !nT ? IMAGE_SYNTH_ERROR_EMPTY_TARGET : IMAGE_SYNTH_ERROR_EMPTY_CORPUS;
@@ -464,8 +484,9 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   // target order (lib/orderTarget.h:269-343) with the engine's PRNG
   // the engine's seed (lib/engine.c:643); a nonzero option seed gives the serial mode
   // another stream, for statistics over seeds
-  Prng prng(job.opt.prng, (serialMode && job.opt.seed) ? job.opt.seed : kEngineSeed);
-  int2 *targets = (int2 *)g_dev.targets.get((size_t)nT * sizeof(int2) + 64);
+  pr.prng.reset(new Prng(job.opt.prng, (serialMode && job.opt.seed) ? job.opt.seed : kEngineSeed));
+  Prng &prng = *pr.prng;
+  int2 *targets = (int2 *)W.targets.get((size_t)nT * sizeof(int2) + 64);
   const int type = P.matchContextType;
   if (type == 0 || type == 1) {
     std::vector<uint32_t> perm(nT);
@@ -474,9 +495,9 @@ int run_engine(Job &job, unsigned &nTargetsOut)
       uint32_t j = prng.range(nT);
       std::swap(perm[i], perm[j]);
     }
-    uint32_t *hPerm = (uint32_t *)g_dev.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
+    uint32_t *hPerm = (uint32_t *)W.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
     std::memcpy(hPerm, perm.data(), (size_t)nT * sizeof(uint32_t));
-    uint32_t *dPerm = (uint32_t *)g_dev.perm.get((size_t)nT * sizeof(uint32_t) + 64);
+    uint32_t *dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
     HIP_TRY(hipMemcpyAsync(dPerm, hPerm, (size_t)nT * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIP_TRY(launch_gather_targets(s, targetsRM, dPerm, targets, nT));
   } else if (type >= 2 && type <= 8) {
@@ -484,7 +505,7 @@ int run_engine(Job &job, unsigned &nTargetsOut)
     HIP_TRY(hipMemcpyAsync(pts.data(), targetsRM, (size_t)nT * sizeof(int2), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     order_directional(type, pts, prng);
-    Pt *hp = (Pt *)g_dev.pinPerm.get((size_t)nT * sizeof(Pt) + 64);
+    Pt *hp = (Pt *)W.pinPerm.get((size_t)nT * sizeof(Pt) + 64);
     std::copy(pts.begin(), pts.end(), hp);
     HIP_TRY(hipMemcpyAsync(targets, hp, (size_t)nT * sizeof(int2), hipMemcpyHostToDevice, s));
   } else {
@@ -525,7 +546,7 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   b.counters = cnt;
   b.tileHas = tileHas;
   b.firstIdx = firstIdx;
-  b.ctl = (uint32_t *)g_dev.wctl.get(WCTL_WORDS * sizeof(uint32_t));
+  b.ctl = (uint32_t *)W.wctl.get(WCTL_WORDS * sizeof(uint32_t));
 
   // pass schedule (lib/passes.h:68-93) and progress estimate (lib/progress.c:31-37)
   unsigned sizes[6];
@@ -537,8 +558,32 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   job.prog.estimated = 0;
   for (int i = 0; i < 6; i++) job.prog.estimated += sizes[i];
   HIP_TRY(hipStreamSynchronize(s));
+  job.st.prepMs = now_ms() - t0;
+  job.st.pixelWordBytes = (unsigned)wb;
+  pr.g = g;
+  pr.b = b;
+  std::copy(sizes, sizes + 6, pr.sizes);
+  pr.wb = wb;
+  return 0;
+}
+
+// The refiner (lib/refiner.h:43-122) over a prepared run: serial or batched passes with
+// the reference's progress, cancel and stop rule, then the run's statistics.
+int refine_engine(Job &job, Prepared &pr)
+{
+  hipStream_t s = g_dev.stream;
+  Workspace &W = *job.W;
   const double t1 = now_ms();
-  job.st.prepMs = t1 - t0;
+  const Geometry &g0 = pr.g;
+  Geometry g = g0;
+  SynthBuffers &b = pr.b;
+  const unsigned nT = pr.nT;
+  const unsigned *sizes = pr.sizes;
+  Prng &prng = *pr.prng;
+  Counters *cnt = b.counters;
+  const uint32_t *lastBest = b.lastBest;
+  const int2 *targets = b.targets;
+  const int tw = g.tw;
 
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
   unsigned launches = 0;
@@ -711,7 +756,7 @@ int run_engine(Job &job, unsigned &nTargetsOut)
   // statistics
   {
     Counters hc;
-    unsigned long long *dSum = (unsigned long long *)g_dev.meanSum.get(16);
+    unsigned long long *dSum = (unsigned long long *)W.meanSum.get(16);
     HIP_TRY(launch_fill_u32(s, (uint32_t *)dSum, 0u, 4));
     HIP_TRY(launch_mean_error(s, lastBest, targets, nT, tw, dSum));
     unsigned long long hs[2];
@@ -732,10 +777,20 @@ int run_engine(Job &job, unsigned &nTargetsOut)
       HIP_TRY(hipEventDestroy(e.second));
     }
     job.st.synthKernelMs = ms;
-    job.st.pixelWordBytes = (unsigned)wb;
   }
   job.st.synthMs = now_ms() - t1;
   return 0;
+}
+
+// The engine (lib/engine.c:539-690) on device buffers.  Returns a TImageSynthError;
+// the synthesized colours are in the target words of the job's workspace on success.
+int run_engine(Job &job, unsigned &nTargetsOut)
+{
+  Prepared pr;
+  const int err = prepare_engine(job, pr);
+  nTargetsOut = pr.nT;
+  if (err) return err;
+  return refine_engine(job, pr);
 }
 
 void init_job(Job &job, const TImageSynthParameters *params, const ResynthOptions *opt, void (*cb)(int, void *),
@@ -749,6 +804,7 @@ void init_job(Job &job, const TImageSynthParameters *params, const ResynthOption
   job.prog.cb = cb;
   job.prog.ctx = ctx;
   job.prog.cancel = cancel;
+  job.W = &g_dev.ws;
 }
 
 // Builds the corpus mask of imageSynth2 as lib/adaptSimple.h:48-96 fills it: mask2's own
@@ -794,8 +850,8 @@ int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, 
   S.mask2RowBytes = w;
   S.alphaIdx = (format == T_RGBA) ? 3 : (format == T_GrayA) ? 1 : -1;
   const size_t npx = (size_t)w * h;
-  unsigned char *dImg = (unsigned char *)g_dev.image.get(npx * bpp + 64);
-  unsigned char *dMask = (unsigned char *)g_dev.mask.get(npx + 64);
+  unsigned char *dImg = (unsigned char *)g_dev.ws.image.get(npx * bpp + 64);
+  unsigned char *dMask = (unsigned char *)g_dev.ws.mask.get(npx + 64);
   if (npx) {
     HIP_TRY(hipMemcpy2DAsync(dImg, S.rowBytes, image->data, image->rowBytes, S.rowBytes, h, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpy2DAsync(dMask, w, mask->data, mask->rowBytes, w, h, hipMemcpyHostToDevice, s));
@@ -803,14 +859,14 @@ int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, 
   S.image = dImg;
   S.mask = dMask;
   if (mask2) {
-    unsigned char *dM2 = (unsigned char *)g_dev.mask2.get(npx + 64);
+    unsigned char *dM2 = (unsigned char *)g_dev.ws.mask2.get(npx + 64);
     HIP_TRY(hipMemcpyAsync(dM2, m2.data(), npx, hipMemcpyHostToDevice, s));
     S.mask2 = dM2;
   }
   unsigned nT = 0;
   int err = npx ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
   if (!err && !(cancelFlag && *cancelFlag)) {
-    HIP_TRY(launch_writeback_simple(s, job.wide, g_dev.tword.p, (const int2 *)g_dev.targets.p, nT, dImg,
+    HIP_TRY(launch_writeback_simple(s, job.wide, g_dev.ws.tword.p, (const int2 *)g_dev.ws.targets.p, nT, dImg,
                                     S.rowBytes, bpp, job.nColors, (int)w));
     HIP_TRY(hipMemcpy2DAsync(image->data, image->rowBytes, dImg, S.rowBytes, S.rowBytes, h, hipMemcpyDeviceToHost,
                              s));
@@ -839,8 +895,8 @@ int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *
   S.ch = (int)corpusMap->height;
   const size_t depth = indices->total_bpp;
   const size_t tbytes = (size_t)S.tw * S.th * depth, cbytes = (size_t)S.cw * S.ch * depth;
-  unsigned char *dT = (unsigned char *)g_dev.tpix.get(tbytes + 64);
-  unsigned char *dC = (unsigned char *)g_dev.cpix.get(cbytes + 64);
+  unsigned char *dT = (unsigned char *)g_dev.ws.tpix.get(tbytes + 64);
+  unsigned char *dC = (unsigned char *)g_dev.ws.cpix.get(cbytes + 64);
   if (tbytes) HIP_TRY(hipMemcpyAsync(dT, targetMap->data->data, tbytes, hipMemcpyHostToDevice, s));
   if (cbytes) HIP_TRY(hipMemcpyAsync(dC, corpusMap->data->data, cbytes, hipMemcpyHostToDevice, s));
   S.tpix = dT;
@@ -850,7 +906,7 @@ int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *
                                : (tbytes ? IMAGE_SYNTH_ERROR_EMPTY_CORPUS : IMAGE_SYNTH_ERROR_EMPTY_TARGET);
   if (!err) {
     // lib/engine.c:675-676: results are left in targetMap
-    HIP_TRY(launch_writeback_pixmap(s, job.wide, g_dev.tword.p, (const int2 *)g_dev.targets.p, nT, dT, (int)depth,
+    HIP_TRY(launch_writeback_pixmap(s, job.wide, g_dev.ws.tword.p, (const int2 *)g_dev.ws.targets.p, nT, dT, (int)depth,
                                     job.nColors, S.tw));
     HIP_TRY(hipMemcpyAsync(targetMap->data->data, dT, tbytes, hipMemcpyDeviceToHost, s));
   }
@@ -858,6 +914,279 @@ int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *
   job.st.totalMs = now_ms() - t0;
   if (stats) *stats = job.st;
   return err;
+}
+
+// ------------------------------------------------------------------ many requests
+
+ManyConfig g_tuned[2] = {{0, 0, 0}, {0, 0, 0}};   // per pixel-word width, from the autotuner
+const ManyConfig kDefaultMany = {4, 2, 4};
+
+unsigned env_permille(const char *name, long dflt)
+{
+  const char *e = std::getenv(name);
+  long v = e ? std::atol(e) : dflt;
+  return (unsigned)std::min(1000L, std::max(1L, v));
+}
+
+// Sliding-window parameters of a request: the single-request schedule's chunk sizes
+// halved (see refine_engine).
+void request_window(const ResynthOptions &o, unsigned nT, unsigned &minW, unsigned &ramp, unsigned &capW)
+{
+  const unsigned cap = o.batch ? o.batch : std::min(65536u, std::max(64u, nT / 32u));
+  const unsigned minChunk = std::min(cap, std::max(1u, std::min(64u, nT / 2048u)));
+  minW = std::max(1u, minChunk / 2);
+  ramp = std::max(1u, env_permille("RESYNTH_RAMP_PERMILLE", 60) / 2);
+  capW = std::max(1u, cap / 2);
+}
+
+int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOptions *opt, const ManyConfig *force,
+              ResynthBatchStats *st)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  const double t0 = now_ms();
+  hipStream_t s = g_dev.stream;
+  while (g_dev.pool.size() < n) g_dev.pool.emplace_back(new Workspace());
+  ResynthOptions o;
+  if (opt) o = *opt;
+  else resynth_default_options(&o);
+  o.mode = RESYNTH_MODE_BATCHED;
+  o.timing = 0;
+  std::vector<Job> jobs(n);
+  std::vector<Prepared> prs(n);
+  ResynthBatchStats bs;
+  std::memset(&bs, 0, sizeof bs);
+
+  // every request's preparation (lib/imageSynth.c:62-137, lib/engine.c:539-660)
+  for (unsigned i = 0; i < n; i++) {
+    ResynthRequest &q = reqs[i];
+    q.error = 0;
+    q.nTargets = 0;
+    q.passes = 0;
+    q.meanError = 0;
+    q.candidates = 0;
+    Job &job = jobs[i];
+    init_job(job, &q.params, &o, nullptr, nullptr, nullptr);
+    job.W = g_dev.pool[i].get();
+    const ImageBuffer *image = &q.image, *mask = &q.mask;
+    if (image->width != mask->width || image->height != mask->height) {
+      q.error = IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
+      continue;
+    }
+    if (prepareImageFormatIndicesFromFormatType(&job.ix, q.format)) {
+      q.error = IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+      continue;
+    }
+    const unsigned w = image->width, h = image->height;
+    const unsigned bpp = countPixelelsPerPixelForFormat(q.format);
+    const size_t npx = (size_t)w * h;
+    if (!npx) {
+      q.error = IMAGE_SYNTH_ERROR_EMPTY_TARGET;
+      continue;
+    }
+    Source &S = job.src;
+    S.simple = true;
+    S.tw = S.cw = (int)w;
+    S.th = S.ch = (int)h;
+    S.bpp = bpp;
+    S.alphaIdx = (q.format == T_RGBA) ? 3 : (q.format == T_GrayA) ? 1 : -1;
+    if (device) {
+      S.image = image->data;
+      S.rowBytes = image->rowBytes;
+      S.mask = mask->data;
+      S.maskRowBytes = mask->rowBytes;
+    } else {
+      Workspace &W = *job.W;
+      S.rowBytes = (size_t)w * bpp;
+      S.maskRowBytes = w;
+      unsigned char *dImg = (unsigned char *)W.image.get(npx * bpp + 64);
+      unsigned char *dMask = (unsigned char *)W.mask.get(npx + 64);
+      HIP_TRY(hipMemcpy2DAsync(dImg, S.rowBytes, image->data, image->rowBytes, S.rowBytes, h, hipMemcpyHostToDevice,
+                               s));
+      HIP_TRY(hipMemcpy2DAsync(dMask, w, mask->data, mask->rowBytes, w, h, hipMemcpyHostToDevice, s));
+      S.image = dImg;
+      S.mask = dMask;
+    }
+    q.error = prepare_engine(job, prs[i]);
+    q.nTargets = prs[i].nT;
+  }
+  bs.prepMs = now_ms() - t0;
+
+  // one multi-request launch per pixel-word width, captured in a graph
+  std::vector<unsigned> passesRun(n, 0);
+  for (int wide = 0; wide < 2; wide++) {
+    std::vector<unsigned> idx;
+    for (unsigned i = 0; i < n; i++)
+      if (!reqs[i].error && jobs[i].wide == (wide != 0)) idx.push_back(i);
+    if (idx.empty()) continue;
+    const unsigned m = (unsigned)idx.size();
+    ReqDesc *hD = (ReqDesc *)g_dev.pinDesc.get((size_t)m * sizeof(ReqDesc));
+    for (unsigned j = 0; j < m; j++) {
+      const Prepared &pr = prs[idx[j]];
+      const Job &job = jobs[idx[j]];
+      ReqDesc &d = hD[j];
+      std::memset(&d, 0, sizeof d);
+      d.g = pr.g;
+      d.g.skipEmpty = (!pr.g.tileH && !pr.g.tileV) ? 1 : 0;
+      d.b = pr.b;
+      std::copy(pr.sizes, pr.sizes + 6, d.sizes);
+      d.seed = job.opt.seed ? job.opt.seed : kEngineSeed;
+      request_window(job.opt, pr.nT, d.minW, d.ramp, d.capW);
+    }
+    ReqDesc *dD = (ReqDesc *)g_dev.manyDesc.get((size_t)m * sizeof(ReqDesc));
+    ReqCtl *dC = (ReqCtl *)g_dev.manyCtl.get((size_t)m * sizeof(ReqCtl));
+    uint32_t *dG = (uint32_t *)g_dev.manyGlobal.get(MG_WORDS * sizeof(uint32_t));
+    HIP_TRY(hipMemcpyAsync(dD, hD, (size_t)m * sizeof(ReqDesc), hipMemcpyHostToDevice, s));
+    const ManyConfig cfg = force ? *force : (g_tuned[wide].waves ? g_tuned[wide] : kDefaultMany);
+    bs.config.wavesPerBlock = cfg.waves;
+    bs.config.firstGroup = cfg.firstGroup;
+    bs.config.laterGroup = cfg.laterGroup;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    HIP_TRY(launch_fill_u32(s, (uint32_t *)dC, 0u, (size_t)m * sizeof(ReqCtl) / 4));
+    HIP_TRY(launch_fill_u32(s, dG, 0u, MG_WORDS));
+    HIP_TRY(launch_fill_u32(s, dG + MG_ACTIVE, m, 1));
+    HIP_TRY(launch_synth_many(s, wide != 0, cfg, dD, dC, m, dG));
+    HIP_TRY(hipStreamEndCapture(s, &graph));
+    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    HIP_TRY(hipGraphLaunch(exec, s));
+    HIP_TRY(hipEventRecord(e1, s));
+    ReqCtl *hC = (ReqCtl *)g_dev.pinCtl.get((size_t)m * sizeof(ReqCtl) + MG_WORDS * sizeof(uint32_t));
+    uint32_t *hG = (uint32_t *)(hC + m);
+    HIP_TRY(hipMemcpyAsync(hC, dC, (size_t)m * sizeof(ReqCtl), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hG, dG, MG_WORDS * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    bs.synthKernelMs += ms;
+    bs.graphLaunches++;
+    if (hG[MG_EXPIRED]) bs.waitsExpired++;
+    for (unsigned j = 0; j < m; j++) passesRun[idx[j]] = (unsigned)hC[j].passesRun;
+    HIP_TRY(hipEventDestroy(e0));
+    HIP_TRY(hipEventDestroy(e1));
+    HIP_TRY(hipGraphExecDestroy(exec));
+    HIP_TRY(hipGraphDestroy(graph));
+  }
+  if (bs.waitsExpired)
+    std::fprintf(stderr, "resynth_mi355x: a multi-request launch cut a wait short at its spin limit\n");
+
+  // results: mean error, counters and the write-back of every request
+  std::vector<Counters> hc(n);
+  std::vector<unsigned long long> sums(2 * (size_t)n, 0);
+  for (unsigned i = 0; i < n; i++) {
+    if (reqs[i].error) continue;
+    Job &job = jobs[i];
+    Prepared &pr = prs[i];
+    Workspace &W = *job.W;
+    unsigned long long *dSum = (unsigned long long *)W.meanSum.get(16);
+    HIP_TRY(launch_fill_u32(s, (uint32_t *)dSum, 0u, 4));
+    HIP_TRY(launch_mean_error(s, pr.b.lastBest, pr.b.targets, pr.nT, pr.g.tw, dSum));
+    HIP_TRY(hipMemcpyAsync(&sums[2 * i], dSum, 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&hc[i], pr.b.counters, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    const Source &S = job.src;
+    unsigned char *dImg = const_cast<unsigned char *>(S.image);
+    HIP_TRY(launch_writeback_simple(s, job.wide, pr.b.tword, pr.b.targets, pr.nT, dImg, S.rowBytes, S.bpp,
+                                    job.nColors, S.tw));
+    if (!device)
+      HIP_TRY(hipMemcpy2DAsync(reqs[i].image.data, reqs[i].image.rowBytes, dImg, S.rowBytes, S.rowBytes,
+                               (size_t)S.th, hipMemcpyDeviceToHost, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (unsigned i = 0; i < n; i++) {
+    ResynthRequest &q = reqs[i];
+    if (q.error) continue;
+    q.passes = passesRun[i];
+    q.meanError = sums[2 * i + 1] ? (double)sums[2 * i] / (double)sums[2 * i + 1] : 0.0;
+    for (int k = 0; k < CNT_SLOTS; k++) {
+      q.candidates += hc[i].candidates[k];
+      bs.patchTerms += hc[i].terms[k];
+    }
+    bs.candidates += q.candidates;
+    bs.requests++;
+    bs.targets += q.nTargets;
+  }
+  bs.totalMs = now_ms() - t0;
+  if (st) *st = bs;
+  return 0;
+}
+
+// Times every kernel shape on private device copies of the sample and keeps the fastest.
+int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, ResynthKernelConfig *best, double *msOut)
+{
+  {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return -1;
+  }
+  hipStream_t s = g_dev.stream;
+  std::vector<ResynthRequest> reqs(sample, sample + n);
+  std::vector<void *> pristine(n, nullptr), work(n, nullptr), masks(n, nullptr);
+  std::vector<size_t> bytes(n, 0), mbytes(n, 0);
+  bool wideSeen[2] = {false, false};
+  for (unsigned i = 0; i < n; i++) {
+    const ResynthRequest &q = sample[i];
+    bytes[i] = q.image.rowBytes * q.image.height;
+    mbytes[i] = q.mask.rowBytes * q.mask.height;
+    if (!bytes[i] || !mbytes[i]) continue;
+    HIP_TRY(hipMalloc(&pristine[i], bytes[i]));
+    HIP_TRY(hipMalloc(&work[i], bytes[i]));
+    HIP_TRY(hipMalloc(&masks[i], mbytes[i]));
+    const hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIP_TRY(hipMemcpyAsync(pristine[i], q.image.data, bytes[i], k, s));
+    HIP_TRY(hipMemcpyAsync(masks[i], q.mask.data, mbytes[i], k, s));
+    reqs[i].image.data = (unsigned char *)work[i];
+    reqs[i].mask.data = (unsigned char *)masks[i];
+    TFormatIndices ix;
+    if (!prepareImageFormatIndicesFromFormatType(&ix, q.format))
+      wideSeen[(ix.img_match_bpp + ix.map_match_bpp) > 3 ? 1 : 0] = true;
+  }
+  auto restore = [&]() {
+    for (unsigned i = 0; i < n; i++)
+      if (work[i]) HIP_TRY(hipMemcpyAsync(work[i], pristine[i], bytes[i], hipMemcpyDeviceToDevice, s));
+  };
+  double bestMs = 1e30;
+  ManyConfig bestCfg = kDefaultMany;
+  int c = 0, rc = 0;
+  restore();
+  rc = run_batch(reqs.data(), n, true, nullptr, &kDefaultMany, nullptr);   // warm-up
+  const int wavesList[3] = {2, 4, 8}, firstList[2] = {2, 4}, laterList[2] = {4, 8};
+  for (int wv : wavesList)
+    for (int fg : firstList)
+      for (int lg : laterList) {
+        if (rc) break;
+        const ManyConfig cfg = {wv, fg, lg};
+        restore();
+        ResynthBatchStats st;
+        rc = run_batch(reqs.data(), n, true, nullptr, &cfg, &st);
+        if (msOut) msOut[c] = st.synthKernelMs;
+        if (!rc && st.synthKernelMs < bestMs) {
+          bestMs = st.synthKernelMs;
+          bestCfg = cfg;
+        }
+        c++;
+      }
+  HIP_TRY(hipStreamSynchronize(s));
+  for (unsigned i = 0; i < n; i++) {
+    if (pristine[i]) HIP_TRY(hipFree(pristine[i]));
+    if (work[i]) HIP_TRY(hipFree(work[i]));
+    if (masks[i]) HIP_TRY(hipFree(masks[i]));
+  }
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    for (int w = 0; w < 2; w++)
+      if (wideSeen[w]) g_tuned[w] = bestCfg;
+  }
+  if (best) {
+    best->wavesPerBlock = bestCfg.waves;
+    best->firstGroup = bestCfg.firstGroup;
+    best->laterGroup = bestCfg.laterGroup;
+  }
+  return 0;
 }
 
 }  // namespace
@@ -1061,12 +1390,45 @@ int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsig
   unsigned nT = 0;
   int err = ((size_t)width * height) ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
   if (!err && !(cancelFlag && *cancelFlag))
-    HIP_TRY(launch_writeback_simple(g_dev.stream, job.wide, g_dev.tword.p, (const int2 *)g_dev.targets.p, nT,
+    HIP_TRY(launch_writeback_simple(g_dev.stream, job.wide, g_dev.ws.tword.p, (const int2 *)g_dev.ws.targets.p, nT,
                                     d_image, rowBytes, S.bpp, job.nColors, (int)width));
   HIP_TRY(hipStreamSynchronize(g_dev.stream));
   job.st.totalMs = now_ms() - t0;
   if (stats) *stats = job.st;
   return err;
+}
+
+int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int deviceBuffers,
+                              const ResynthOptions *options, const ResynthKernelConfig *config,
+                              ResynthBatchStats *stats)
+{
+  if (!requests && n) return -1;
+  ManyConfig cfg;
+  const ManyConfig *force = nullptr;
+  if (config && config->wavesPerBlock) {
+    cfg = {config->wavesPerBlock, config->firstGroup, config->laterGroup};
+    if (!many_config_valid(cfg)) return -1;
+    force = &cfg;
+  }
+  return run_batch(requests, n, deviceBuffers != 0, options, force, stats);
+}
+
+unsigned int resynth_struct_sizes(size_t *out, unsigned int n)
+{
+  const size_t v[8] = {sizeof(ResynthOptions),          sizeof(ResynthStats),
+                       sizeof(ResynthRequest),          sizeof(ResynthKernelConfig),
+                       sizeof(ResynthBatchStats),       offsetof(ResynthRequest, params),
+                       offsetof(ResynthRequest, error), offsetof(ResynthBatchStats, prepMs)};
+  unsigned k = 0;
+  for (; k < n && k < 8; k++) out[k] = v[k];
+  return k;
+}
+
+int resynth_autotune_batch(const ResynthRequest *sample, unsigned int n, int deviceBuffers,
+                           ResynthKernelConfig *best, double *msPerConfig)
+{
+  if (!sample || !n) return -1;
+  return autotune_batch(sample, n, deviceBuffers != 0, best, msPerConfig);
 }
 
 int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned int ch,
@@ -1089,15 +1451,15 @@ int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned 
   const int depth = indices->total_bpp;
   const size_t ncp = (size_t)cw * ch, nnb = (size_t)nProblems * 64;
   // corpus pixmap and neighbour pixels -> words on the device
-  unsigned char *dPix = (unsigned char *)g_dev.cpix.get(ncp * depth + nnb * 8 + 64);
+  unsigned char *dPix = (unsigned char *)g_dev.ws.cpix.get(ncp * depth + nnb * 8 + 64);
   unsigned char *dNbPix = dPix + ncp * depth;
   HIP_TRY(hipMemcpyAsync(dPix, corpus, ncp * depth, hipMemcpyHostToDevice, s));
   // neighbour pixels arrive as 8-byte records; repack them to `depth` bytes each
   std::vector<unsigned char> nb(nnb * depth);
   for (size_t i = 0; i < nnb; i++) std::memcpy(&nb[i * depth], nbPixel + i * 8, depth);
   HIP_TRY(hipMemcpyAsync(dNbPix, nb.data(), nb.size(), hipMemcpyHostToDevice, s));
-  void *dC = g_dev.cword.get(ncp * wb + 64);
-  void *dNbW = g_dev.tword.get(nnb * wb + 64);
+  void *dC = g_dev.ws.cword.get(ncp * wb + 64);
+  void *dNbW = g_dev.ws.tword.get(nnb * wb + 64);
   HIP_TRY(launch_pack_words(s, wide, dPix, (int)ncp, depth, indices->colorEndBip, indices->map_start_bip,
                             indices->map_end_bip, 1, dC));
   HIP_TRY(launch_pack_words(s, wide, dNbPix, (int)nnb, depth, indices->colorEndBip, indices->map_start_bip,
@@ -1105,11 +1467,11 @@ int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned 
   Tables ht;
   std::memcpy(ht.T, T, sizeof ht.T);
   std::memcpy(ht.Tm, Tm, sizeof ht.Tm);
-  Tables *dT = (Tables *)g_dev.tables.get(sizeof(Tables));
+  Tables *dT = (Tables *)g_dev.ws.tables.get(sizeof(Tables));
   HIP_TRY(hipMemcpyAsync(dT, &ht, sizeof ht, hipMemcpyHostToDevice, s));
   size_t szCnt = nProblems * sizeof(unsigned), szOff = nnb * sizeof(int2);
   size_t szCand = (size_t)nProblems * maxCands * sizeof(int2), szOut = nProblems * 4;
-  char *dArgs = (char *)g_dev.compactTemp.get(2 * szCnt + szOff + szCand + 3 * szOut + 512);
+  char *dArgs = (char *)g_dev.ws.compactTemp.get(2 * szCnt + szOff + szCand + 3 * szOut + 512);
   unsigned *dNbCount = (unsigned *)dArgs;
   unsigned *dCandCount = (unsigned *)(dArgs + szCnt);
   int2 *dOff = (int2 *)(dArgs + 2 * szCnt);

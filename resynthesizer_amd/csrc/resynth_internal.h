@@ -74,6 +74,38 @@ struct StreamState {
   uint32_t pad;
 };
 
+// One request of a multi-request launch (device memory).
+struct ReqDesc {
+  Geometry g;             // skipEmpty marks a request whose pass-0 scans may skip empty rings
+  SynthBuffers b;
+  unsigned sizes[6];      // pass sizes (lib/passes.h:68-93)
+  unsigned seed;          // probe-stream seed
+  unsigned minW;          // pass-0 window floor (>= 1)
+  unsigned ramp;          // pass-0 window growth, permille of the targets visited in the pass
+  unsigned capW;          // window cap
+};
+
+// Scheduling state of one request, each hot word on its own 128-byte line.
+struct ReqCtl {
+  unsigned long long ctr;            // (pass << 32) | next ticket of that pass; pass 6 = finished
+  unsigned long long passesRun;      // set when the request finishes
+  unsigned long long pad0[14];
+  struct {
+    unsigned long long word;         // (betters << 32) | visits finished in the pass
+    unsigned long long pad[15];
+  } pass[6];
+};
+
+// Global words of a multi-request launch.
+enum { MG_ACTIVE = 0, MG_EXPIRED = 32, MG_WORDS = 64 };
+
+// Kernel shape of the multi-request launch, chosen by the autotuner.
+struct ManyConfig {
+  int waves;              // targets per workgroup (one wavefront each): 2, 4 or 8
+  int firstGroup;         // loads issued together for a candidate's first neighbours: 2 or 4
+  int laterGroup;         // loads issued together afterwards: 4 or 8
+};
+
 // ---- kernel launchers (resynth_kernels.hip) ----
 hipError_t launch_adapt_simple(hipStream_t s, bool wide, const unsigned char *image, size_t rowBytes, unsigned bpp,
                                const unsigned char *mask, size_t maskRowBytes, const unsigned char *mask2,
@@ -96,6 +128,9 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
 hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,
                                unsigned cap);
+hipError_t launch_synth_many(hipStream_t s, bool wide, const ManyConfig &cfg, const ReqDesc *descs, ReqCtl *ctls,
+                             unsigned nReq, uint32_t *global);
+bool many_config_valid(const ManyConfig &cfg);
 hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, int prngKind, const uint32_t *stream, uint32_t streamLen,
                                StreamState *state);

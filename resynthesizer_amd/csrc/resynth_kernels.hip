@@ -914,6 +914,187 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
   return hipGetLastError();
 }
 
+// Many requests in one persistent launch.  Each request runs its passes as sliding
+// windows (k_synth_window) with its own ticket counter; a wave keeps visiting targets of
+// its current request and moves on to the next request when that one has no ticket to
+// give.  The wave that finishes a pass applies the refiner's stop rule for its request
+// (lib/refiner.h:111-115) and opens the next pass or retires the request.  Waits are on
+// lower tickets of the same pass only, so every pass completes; waves leave when no
+// request is active.  Spin limits bound every wait and the idle polling all the same
+// (an expiry is flagged in the global words).
+constexpr unsigned kIdleLimit = 1u << 24;
+
+__device__ __forceinline__ unsigned long long load_agent64(const unsigned long long *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long first_lane64(unsigned long long v)
+{
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+template <typename Word, int WAVES, int G0, int GL>
+__global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls,
+                                                           unsigned nReq, uint32_t *global)
+{
+  __shared__ uint16_t sT[WAVES][512];
+  __shared__ uint32_t sTm[WAVES][512];
+  __shared__ int2 sOff[WAVES][64];
+  __shared__ Word sPix[WAVES][64];
+  __shared__ int2 sSrc[WAVES][64];
+  const int wave = threadIdx.x >> 6;
+  const unsigned lane = lane_id();
+  unsigned *active = global + MG_ACTIVE, *expired = global + MG_EXPIRED;
+  unsigned r = (blockIdx.x * WAVES + wave) % nReq;
+  const Tables *loaded = nullptr;
+  unsigned idleRound = 0, idleTotal = 0;
+  for (;;) {
+    unsigned act = 0;
+    if (lane == 0) act = load_agent(active);
+    if (__builtin_amdgcn_readfirstlane(act) == 0) break;
+    r = __builtin_amdgcn_readfirstlane(r);
+    const ReqDesc &d = descs[r];
+    ReqCtl &c = ctls[r];
+    unsigned long long t64 = 0;
+    bool got = false;
+    if (lane == 0) {
+      const unsigned long long cur = load_agent64(&c.ctr);
+      unsigned p = (unsigned)(cur >> 32);
+      if (p < 6 && (unsigned)cur < d.sizes[p]) {
+        t64 = atomicAdd(&c.ctr, 1ull);
+        p = (unsigned)(t64 >> 32);
+        got = p < 6 && (unsigned)t64 < d.sizes[p];
+      }
+    }
+    if ((__ballot(got) & 1ull) == 0) {
+      r = (r + 1 == nReq) ? 0u : r + 1;
+      if (++idleRound >= nReq) {
+        idleRound = 0;
+        __builtin_amdgcn_s_sleep(8);
+      }
+      if (++idleTotal > kIdleLimit) {
+        if (lane == 0) atomicOr(expired, 1u);
+        break;
+      }
+      continue;
+    }
+    idleRound = 0;
+    t64 = first_lane64(t64);
+    const unsigned pass = (unsigned)(t64 >> 32), tk = (unsigned)t64;
+    unsigned W = d.capW;
+    if (pass == 0) {
+      const unsigned rr = (unsigned)((uint64_t)tk * d.ramp / 1000u);
+      W = rr > d.minW ? rr : d.minW;
+      if (W > d.capW) W = d.capW;
+    }
+    if (tk > W) {
+      const unsigned need = tk - W;
+      if (lane == 0) {
+        unsigned seen = (unsigned)load_agent64(&c.pass[pass].word);
+        unsigned spins = 0;
+        while (seen < need) {
+          if (++spins > kSpinLimit || load_agent(expired)) {
+            atomicOr(expired, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+          seen = (unsigned)load_agent64(&c.pass[pass].word);
+        }
+      }
+    }
+    if (d.b.tables != loaded) {
+      for (int i = lane; i < 512; i += 64) {
+        sT[wave][i] = d.b.tables->T[i];
+        sTm[wave][i] = d.b.tables->Tm[i];
+      }
+      wave_sync();
+      loaded = d.b.tables;
+    }
+    Geometry g = d.g;
+    g.skipEmpty = (pass == 0 && d.g.skipEmpty) ? 1 : 0;
+    unsigned long long ev = 0, tm = 0;
+    unsigned bet = 0;
+    visit_wave<Word, G0, GL, true>(g, d.b, tk, d.seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT[wave], sTm[wave], ev,
+                                   tm, bet);
+    if (lane == 0) {
+      const unsigned slot = tk & (CNT_SLOTS - 1);
+      if (ev) atomicAdd(&d.b.counters->candidates[slot], ev);
+      if (tm) atomicAdd(&d.b.counters->terms[slot], tm);
+      const unsigned long long old = atomicAdd(&c.pass[pass].word, ((unsigned long long)bet << 32) | 1ull);
+      if ((unsigned)old + 1u == d.sizes[pass]) {
+        const unsigned betters = (unsigned)(old >> 32) + bet;
+        const bool more = pass < 5 && d.sizes[pass + 1] > 0 && !((double)((float)betters / (float)d.g.nT) < 0.1);
+        if (more) {
+          atomicExch(&c.ctr, (unsigned long long)(pass + 1) << 32);
+        } else {
+          // an empty next pass counts as run, as in the host loop
+          c.passesRun = pass + 1 + ((pass < 5 && d.sizes[pass + 1] == 0) ? 1 : 0);
+          atomicExch(&c.ctr, 6ull << 32);
+          atomicSub(active, 1u);
+        }
+      }
+    }
+  }
+}
+
+bool many_config_valid(const ManyConfig &c)
+{
+  return (c.waves == 2 || c.waves == 4 || c.waves == 8) && (c.firstGroup == 2 || c.firstGroup == 4) &&
+         (c.laterGroup == 4 || c.laterGroup == 8);
+}
+
+template <typename Word, int WAVES, int G0, int GL>
+static hipError_t launch_many_t(hipStream_t s, const ReqDesc *descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
+{
+  static int resident = [] {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_synth_many<Word, WAVES, G0, GL>, 64 * WAVES, 0) !=
+            hipSuccess || per < 1)
+      per = 1;
+    return cus * per;
+  }();
+  hipLaunchKernelGGL((k_synth_many<Word, WAVES, G0, GL>), dim3(resident), dim3(64 * WAVES), 0, s, descs, ctls, nReq,
+                     global);
+  return hipGetLastError();
+}
+
+template <typename Word, int WAVES>
+static hipError_t launch_many_w(hipStream_t s, const ManyConfig &c, const ReqDesc *d, ReqCtl *x, unsigned n,
+                                uint32_t *gl)
+{
+  if (c.firstGroup == 2)
+    return c.laterGroup == 4 ? launch_many_t<Word, WAVES, 2, 4>(s, d, x, n, gl)
+                             : launch_many_t<Word, WAVES, 2, 8>(s, d, x, n, gl);
+  return c.laterGroup == 4 ? launch_many_t<Word, WAVES, 4, 4>(s, d, x, n, gl)
+                           : launch_many_t<Word, WAVES, 4, 8>(s, d, x, n, gl);
+}
+
+template <typename Word>
+static hipError_t launch_many_word(hipStream_t s, const ManyConfig &c, const ReqDesc *d, ReqCtl *x, unsigned n,
+                                   uint32_t *gl)
+{
+  switch (c.waves) {
+  case 2: return launch_many_w<Word, 2>(s, c, d, x, n, gl);
+  case 8: return launch_many_w<Word, 8>(s, c, d, x, n, gl);
+  default: return launch_many_w<Word, 4>(s, c, d, x, n, gl);
+  }
+}
+
+// The caller zeroes ctls, sets each ctr to pass 0 and global[MG_ACTIVE] to nReq.
+hipError_t launch_synth_many(hipStream_t s, bool wide, const ManyConfig &cfg, const ReqDesc *descs, ReqCtl *ctls,
+                             unsigned nReq, uint32_t *global)
+{
+  if (!nReq) return hipSuccess;
+  if (!many_config_valid(cfg)) return hipErrorInvalidValue;
+  return wide ? launch_many_word<uint64_t>(s, cfg, descs, ctls, nReq, global)
+              : launch_many_word<uint32_t>(s, cfg, descs, ctls, nReq, global);
+}
+
 template <typename Word, int GL>
 static hipError_t launch_window_t(hipStream_t s, const Geometry &g, const SynthBuffers &b, unsigned k0, unsigned k1,
                                   unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille, unsigned cap)

@@ -128,3 +128,35 @@ def config4(seed=4, corpus_size=768, target_size=1536):
     target[..., 0] = 255
     target[..., 4] = tmap
     return target, np.ascontiguousarray(corpus)
+
+
+def config3_request(k, base_seed=3000):
+    """Request k of the uneven-request batch: an RGB image with sides U[128,1536] from the
+    grating+noise generator (seed base_seed+k), a mask of 1-6 seeded ellipses covering about
+    3-25% of the image, and patchSize drawn from {10, 30, 50}.  Returns (image, mask, patch)."""
+    rng = np.random.RandomState(base_seed + k)
+    w, h = rng.randint(128, 1537, size=2)
+    img = grating_noise(int(h), int(w), 3, base_seed + k)
+    count = rng.randint(1, 7)
+    frac = rng.uniform(0.03, 0.25)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    m = np.zeros((h, w), bool)
+    for _ in range(count):
+        area = frac * w * h / count
+        aspect = rng.uniform(0.5, 2.0)
+        a = np.sqrt(area * aspect / np.pi)
+        b = area / (np.pi * a)
+        cx, cy = rng.uniform(0.15 * w, 0.85 * w), rng.uniform(0.15 * h, 0.85 * h)
+        t = rng.uniform(0, np.pi)
+        dx, dy = x - cx, y - cy
+        u = dx * np.cos(t) + dy * np.sin(t)
+        v = -dx * np.sin(t) + dy * np.cos(t)
+        m |= (u / a) ** 2 + (v / b) ** 2 <= 1.0
+    patch = int(rng.choice([10, 30, 50]))
+    return img, (m * 255).astype(np.uint8), patch
+
+
+def config3(n=96, base_seed=3000):
+    """The batch of n uneven heal requests: lists of images, masks and patch sizes."""
+    reqs = [config3_request(k, base_seed) for k in range(n)]
+    return [r[0] for r in reqs], [r[1] for r in reqs], [r[2] for r in reqs]

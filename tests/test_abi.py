@@ -144,3 +144,12 @@ def test_target_ordering_matches_oracle(mtype, prng):
         e1, a = R.order_targets(mtype, np.array(single), prng)
         e2, b, _ = O.order_targets(mtype, np.array(single), prng)
         assert e1 == e2 and (e1 or np.array_equal(a, b))
+
+
+def test_extension_struct_layouts_match_library():
+    """The ctypes mirrors of the extension structs have the library's sizes and offsets."""
+    got = R.struct_sizes()
+    want = [ctypes.sizeof(R.ResynthOptions), ctypes.sizeof(R.ResynthStats), ctypes.sizeof(R.ResynthRequest),
+            ctypes.sizeof(R.ResynthKernelConfig), ctypes.sizeof(R.ResynthBatchStats),
+            R.ResynthRequest.params.offset, R.ResynthRequest.error.offset, R.ResynthBatchStats.prepMs.offset]
+    assert got == want

@@ -345,3 +345,85 @@ def test_batched_map_transfer_config4():
     ratio = np.mean(bat) / np.mean(ser)
     print("config 4 crop mean error: batched %.0f serial %.0f ratio %.4f" % (np.mean(bat), np.mean(ser), ratio))
     assert ratio <= 1.08, (bat, ser)
+
+
+# ---------------------------------------------------------------- many requests in one launch
+
+def _uneven_requests(n=10, seed=77):
+    rng = np.random.RandomState(seed)
+    imgs, masks, fmts, params = [], [], [], []
+    for k in range(n):
+        w, h = (int(v) for v in rng.randint(48, 321, size=2))
+        fmt = [R.T_RGB, R.T_RGBA, R.T_Gray][k % 3]
+        bpp = {R.T_RGB: 3, R.T_RGBA: 4, R.T_Gray: 1}[fmt]
+        img = W.grating_noise(h, w, bpp, seed + k)
+        if fmt == R.T_RGBA:
+            img[..., 3] = 255
+        m = W.ellipse_mask(h, w, int(rng.randint(1, 4)), seed + k, axes=(0.08 * min(w, h), 0.25 * min(w, h)))
+        imgs.append(img)
+        masks.append((m * 255).astype(np.uint8))
+        fmts.append(fmt)
+        params.append(R.default_params(patchSize=int(rng.choice([10, 30, 50]))))
+    return imgs, masks, fmts, params
+
+
+def test_batch_of_uneven_requests():
+    """Requests of uneven sizes, formats and patch sizes in one multi-request launch: each
+    request keeps its context, takes corpus colours, reports the errors imageSynth would
+    (mask mismatch, bad format, empty target), and its mean match error stays close to the
+    same request run alone in batched mode."""
+    imgs, masks, fmts, params = _uneven_requests()
+    # three bad requests at the end
+    imgs += [imgs[0], imgs[1], imgs[2]]
+    masks += [masks[0][:-1], masks[1], np.zeros_like(masks[2])]
+    fmts += [fmts[0], 9, fmts[2]]
+    params += [params[0], params[1], params[2]]
+    rc, res, st, outs = R.image_synth_batch(imgs, masks, fmts, params, seed=1)
+    assert rc == 0
+    n = len(imgs) - 3
+    assert [r["error"] for r in res[n:]] == [R.IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH,
+                                           R.IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT,
+                                           R.IMAGE_SYNTH_ERROR_EMPTY_TARGET]
+    assert st["requests"] == n and st["waitsExpired"] == 0 and st["graphLaunches"] == 1
+    ratios = []
+    for i in range(n):
+        r = res[i]
+        assert r["error"] == 0 and r["passes"] >= 1
+        sel = masks[i] != 0
+        assert r["nTargets"] == int(sel.sum())
+        out, img = outs[i], imgs[i]
+        assert np.array_equal(out[~sel], img[~sel])
+        ncol = 1 if fmts[i] == R.T_Gray else 3
+        corpus = {tuple(c) for c in img[~sel].reshape(-1, img.shape[-1])[:, :ncol]}
+        assert all(tuple(c) in corpus for c in out[sel].reshape(-1, img.shape[-1])[::3, :ncol])
+        single = np.mean([R.image_synth(img, masks[i], fmts[i], params[i], mode=R.MODE_BATCHED, seed=s)[0].meanError
+                          for s in (1, 2)])
+        ratios.append(r["meanError"] / single)
+    print("batch vs single mean error ratios", np.round(ratios, 3))
+    assert np.mean(ratios) <= 1.05, ratios
+
+
+def test_batch_device_buffers_and_autotuner():
+    """The device-buffer form of the batch entry and the autotuner: the tuner times all 12
+    kernel shapes, leaves its sample unchanged, and later batches use the shape it chose."""
+    import torch
+    imgs, masks, fmts, params = _uneven_requests(n=4, seed=91)
+    d_imgs = [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in imgs]
+    d_masks = [torch.from_numpy(m).cuda() for m in masks]
+    dev_img = [(t.data_ptr(), a.shape[1], a.shape[0], a.strides[0]) for t, a in zip(d_imgs, imgs)]
+    dev_mask = [(t.data_ptr(), m.shape[1], m.shape[0], m.strides[0]) for t, m in zip(d_masks, masks)]
+    rc, best, ms = R.autotune_batch(dev_img, dev_mask, fmts, params, device=True)
+    assert rc == 0 and best[0] in (2, 4, 8) and best[1] in (2, 4) and best[2] in (4, 8)
+    assert (ms > 0).all(), ms
+    torch.cuda.synchronize()
+    for t, a in zip(d_imgs, imgs):
+        assert np.array_equal(t.cpu().numpy(), a)
+    rc, res, st, _ = R.image_synth_batch(dev_img, dev_mask, fmts, params, device=True)
+    assert rc == 0 and st["config"] == best
+    torch.cuda.synchronize()
+    for t, a, m, r in zip(d_imgs, imgs, masks, res):
+        assert r["error"] == 0
+        out = t.cpu().numpy()
+        sel = m != 0
+        assert np.array_equal(out[~sel], a[~sel])
+        assert not np.array_equal(out[sel], a[sel])
