@@ -635,15 +635,16 @@ int refine_engine(Job &job, Prepared &pr)
     long v = e ? std::atol(e) : 60;
     return (unsigned)std::min(1000L, std::max(1L, v));
   }();
-  // Sliding-window launches (RESYNTH_WINDOW=0 turns them off): once a chunk would reach
+  // Sliding-window launches (RESYNTH_WINDOW=1 turns them on): once a chunk would reach
   // RESYNTH_WINDOW_MIN targets, the rest of the pass runs as one persistent launch whose
   // window is half the chunk the schedule above would use (a visit in a chunk misses half
-  // the chunk on average, a visit in a window up to the whole window).
-  static const int windowOn = [] {
+  // the chunk on average, a visit in a window up to the whole window).  Off by default:
+  // the two device-wide counters every visit updates cost more than the chunk tails.
+  const int windowOn = [] {
     const char *e = std::getenv("RESYNTH_WINDOW");
-    return e ? std::atoi(e) : 1;
+    return e ? std::atoi(e) : 0;
   }();
-  static const unsigned windowMin = [] {
+  const unsigned windowMin = [] {
     const char *e = std::getenv("RESYNTH_WINDOW_MIN");
     long v = e ? std::atol(e) : 2048;
     return (unsigned)std::max(64L, v);
