@@ -170,6 +170,9 @@ typedef struct {
   unsigned int windowLaunches;        /* batched: persistent sliding-window launches */
   unsigned int windowWaitsExpired;    /* batched: window waits cut short by their spin limit
                                          (0 unless the device misbehaves) */
+  unsigned int waveLaunches;          /* batched: launches of the wavefront-per-target kernel */
+  double waveKernelMs;                /* their summed durations (timing != 0) */
+  unsigned long long wavePatchTerms;  /* patch terms they scored */
 } ResynthStats;
 
 void resynth_set_default_mode(int mode);

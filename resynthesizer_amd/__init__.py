@@ -69,7 +69,9 @@ class ResynthStats(ctypes.Structure):
                 ("patchTerms", ctypes.c_ulonglong), ("meanError", ctypes.c_double), ("launches", ctypes.c_uint),
                 ("synthKernelMs", ctypes.c_double), ("prepMs", ctypes.c_double), ("synthMs", ctypes.c_double),
                 ("totalMs", ctypes.c_double), ("pixelWordBytes", ctypes.c_uint),
-                ("windowLaunches", ctypes.c_uint), ("windowWaitsExpired", ctypes.c_uint)]
+                ("windowLaunches", ctypes.c_uint), ("windowWaitsExpired", ctypes.c_uint),
+                ("waveLaunches", ctypes.c_uint), ("waveKernelMs", ctypes.c_double),
+                ("wavePatchTerms", ctypes.c_ulonglong)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}

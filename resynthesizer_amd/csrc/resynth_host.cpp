@@ -276,7 +276,8 @@ struct PinnedBuf {
 // Device and pinned buffers of one engine run (grow-only, reused across calls).
 struct Workspace {
   Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
-      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, firstIdx, wctl;
+      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, firstIdx, wctl,
+      bandStart;
   PinnedBuf pinCounts, pinTables, pinPerm;
 };
 
@@ -368,14 +369,21 @@ struct Job {
   Workspace *W = nullptr;
 };
 
-void record_event(Job &job, std::vector<std::pair<hipEvent_t, hipEvent_t>> &ev, bool start)
+// A synthesis launch bracketed by HIP events (ResynthOptions.timing); `wave` marks the
+// one-wavefront-per-target kernel, the dominant one.
+struct TimedLaunch {
+  hipEvent_t a = nullptr, b = nullptr;
+  bool wave = false;
+};
+
+void record_event(Job &job, std::vector<TimedLaunch> &ev, bool start, bool wave = false)
 {
   if (!job.opt.timing) return;
   hipEvent_t e;
   HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(e, g_dev.stream));
-  if (start) ev.push_back({e, nullptr});
-  else ev.back().second = e;
+  if (start) ev.push_back({e, nullptr, wave});
+  else ev.back().b = e;
 }
 
 // An engine run prepared on the device: point lists, sorted offsets, tables, target
@@ -432,6 +440,8 @@ int prepare_engine(Job &job, Prepared &pr)
   unsigned *dCounts = (unsigned *)(ctemp + 2 * ctb);
   HIP_TRY(launch_compact(s, tflag, tw, th, targetsRM, dCounts + 0, ctemp));
   HIP_TRY(launch_compact(s, cflag, cw, ch, corpusPts, dCounts + 1, ctemp + ctb));
+  uint32_t *bandStart = (uint32_t *)W.bandStart.get((kBands + 1) * sizeof(uint32_t));
+  HIP_TRY(launch_band_starts(s, corpusPts, dCounts + 1, ch, bandStart));
   unsigned *hCounts = (unsigned *)W.pinCounts.get(64);
   HIP_TRY(hipMemcpyAsync(hCounts, dCounts, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
   hipEvent_t countsReady;
@@ -532,6 +542,11 @@ int prepare_engine(Job &job, Prepared &pr)
     return e ? std::atoi(e) : -1;
   }();
   g.probeByPixel = probeKnob >= 0 ? (probeKnob ? 1 : 0) : ((uint64_t)nC * 2 >= (uint64_t)nCpix ? 1 : 0);
+  // Batched random probes come from the XCD's band of corpus rows when the corpus words
+  // outgrow an XCD's L2 (RESYNTH_XCD_BANDS=0 turns this off).
+  const char *bandEnv = std::getenv("RESYNTH_XCD_BANDS");
+  const bool bandsOn = bandEnv ? std::atoi(bandEnv) != 0 : true;
+  g.bands = (!serialMode && bandsOn && nCpix * wb > (4u << 20) && ch >= 8 * kBands) ? kBands : 0;
 
   SynthBuffers b = {};
   b.tword = tword;
@@ -547,6 +562,7 @@ int prepare_engine(Job &job, Prepared &pr)
   b.tileHas = tileHas;
   b.firstIdx = firstIdx;
   b.ctl = (uint32_t *)W.wctl.get(WCTL_WORDS * sizeof(uint32_t));
+  b.bandStart = bandStart;
 
   // pass schedule (lib/passes.h:68-93) and progress estimate (lib/progress.c:31-37)
   unsigned sizes[6];
@@ -585,7 +601,7 @@ int refine_engine(Job &job, Prepared &pr)
   const int2 *targets = b.targets;
   const int tw = g.tw;
 
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<TimedLaunch> ev;
   unsigned launches = 0;
   const bool serial = job.opt.mode == RESYNTH_MODE_SERIAL;
 
@@ -721,7 +737,7 @@ int refine_engine(Job &job, Prepared &pr)
           launches++;
           windowLaunches++;
         } else {
-          record_event(job, ev, true);
+          record_event(job, ev, true, k1 - k >= synth_wave_min());
           HIP_TRY(launch_synth_batched(s, job.wide, g, b, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
           record_event(job, ev, false);
           launches++;
@@ -768,16 +784,23 @@ int refine_engine(Job &job, Prepared &pr)
     for (int i = 0; i < CNT_SLOTS; i++) {
       job.st.candidates += hc.candidates[i];
       job.st.patchTerms += hc.terms[i];
+      job.st.wavePatchTerms += hc.termsWave[i];
     }
     double ms = 0;
+    double wms = 0;
     for (auto &e : ev) {
       float f = 0;
-      HIP_TRY(hipEventElapsedTime(&f, e.first, e.second));
+      HIP_TRY(hipEventElapsedTime(&f, e.a, e.b));
       ms += f;
-      HIP_TRY(hipEventDestroy(e.first));
-      HIP_TRY(hipEventDestroy(e.second));
+      if (e.wave) {
+        wms += f;
+        job.st.waveLaunches++;
+      }
+      HIP_TRY(hipEventDestroy(e.a));
+      HIP_TRY(hipEventDestroy(e.b));
     }
     job.st.synthKernelMs = ms;
+    job.st.waveKernelMs = wms;
   }
   job.st.synthMs = now_ms() - t1;
   return 0;

@@ -30,7 +30,11 @@ struct Geometry {
   unsigned rhoMax;        // largest radius indexed by SynthBuffers::firstIdx
   int skipEmpty;          // pass 0 without tiling: start scans past provably empty rings
   int probeByPixel;       // batched probes by rejection over corpus pixels (dense corpora)
+  int bands;              // batched: 0, or kBands (a workgroup on XCD x draws its random probes
+                          // from corpus rows [x*ch/kBands, (x+1)*ch/kBands))
 };
+
+constexpr int kBands = 8;   // one band of corpus rows per XCD
 
 constexpr int kTile = 8;
 
@@ -44,6 +48,7 @@ enum { CNT_SLOTS = 64 };
 struct Counters {
   unsigned long long candidates[CNT_SLOTS];
   unsigned long long terms[CNT_SLOTS];
+  unsigned long long termsWave[CNT_SLOTS];   // terms scored by k_synth_wave launches only
   unsigned int betters[CNT_SLOTS];
 };
 
@@ -61,6 +66,7 @@ struct SynthBuffers {
   uint8_t *tileHas;       // [tilesH*tilesW] 1 when the tile holds a pixel with a value
   const uint32_t *firstIdx; // [rhoMax+1] number of sorted offsets with x^2+y^2 < rho^2
   uint32_t *ctl;          // [WCTL_WORDS] sliding-window launch control
+  const uint32_t *bandStart; // [kBands+1] index in corpusPts of each band's first point
 };
 
 // Control words of a sliding-window launch, each on its own 128-byte line.
@@ -120,11 +126,13 @@ hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int
 size_t offsets_temp_bytes(size_t nO);
 hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+hipError_t launch_band_starts(hipStream_t s, const int2 *corpusPts, const unsigned *nC, int ch, uint32_t *out);
 hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas);
 hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx);
 hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);
 hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                 unsigned k1, unsigned seed, unsigned pass);
+unsigned synth_wave_min();   // chunks of at least this many targets use k_synth_wave
 hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,
                                unsigned cap);

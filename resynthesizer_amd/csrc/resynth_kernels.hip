@@ -290,6 +290,37 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
 
 // ------------------------------------------------------------------ workgroup visits
 
+// Band of corpus rows that this workgroup's random probes come from: workgroups are
+// dispatched to the XCDs in turn, so band x is probed from XCD x only and stays in that
+// XCD's L2.  Returns the row range [r0, r1).
+__device__ __forceinline__ void probe_band(const Geometry &g, int &r0, int &r1)
+{
+  if (g.bands) {
+    const int x = (int)(blockIdx.x % (unsigned)kBands);
+    r0 = x * g.ch / kBands;
+    r1 = (x + 1) * g.ch / kBands;
+  } else {
+    r0 = 0;
+    r1 = g.ch;
+  }
+}
+
+// Index into the corpus point list of a random probe drawn from hash h (batched modes),
+// restricted to the workgroup's band when banding is on and the band holds points.
+__device__ __forceinline__ unsigned probe_index(const Geometry &g, const SynthBuffers &b, uint64_t h)
+{
+  unsigned lo = 0, n = g.nC;
+  if (g.bands) {
+    const unsigned x = blockIdx.x % (unsigned)kBands;
+    const unsigned a = b.bandStart[x], e = b.bandStart[x + 1];
+    if (e > a) {
+      lo = a;
+      n = e - a;
+    }
+  }
+  return lo + (unsigned)(((h >> 32) * (uint64_t)n) >> 32);
+}
+
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
   z += 0x9E3779B97F4A7C15ull;
@@ -505,7 +536,7 @@ __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word
       active = j < g.maxProbe;
       if (active) {
         uint64_t h = mix64(key * 0x100000001B3ull + j);
-        idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
+        idx = probe_index(g, b, h);
       }
     }
     uint32_t F = 0, M = 0;
@@ -642,7 +673,10 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   // corpora index the corpus point list.  Draws are resolved in attempt order.
   const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
   const bool byPixel = g.probeByPixel != 0;
-  const uint64_t nPix = (uint64_t)g.cw * (uint64_t)g.ch;
+  int r0, r1;
+  probe_band(g, r0, r1);
+  const uint64_t nPix = (uint64_t)g.cw * (uint64_t)(r1 - r0);
+  const uint64_t pix0 = (uint64_t)g.cw * (uint64_t)r0;
   unsigned drawn = 0;
   for (unsigned attempt = 0; drawn < g.maxProbe && !perfect; attempt += 64) {
     const uint64_t h = mix64(key * 0x100000001B3ull + attempt + lane);
@@ -651,7 +685,7 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
     uint32_t F = 0, M = 0;
     bool alive = false;
     if (byPixel) {
-      const uint64_t p = ((h >> 32) * nPix) >> 32;
+      const uint64_t p = pix0 + (((h >> 32) * nPix) >> 32);
       c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
       // the candidate's own word (neighbour 0, loaded with the first group) carries the
       // corpus-point flag
@@ -659,8 +693,7 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
       score_candidate<Word, G0 + 1, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, false);
       valid = (w0 & 0xFFu) == 0xFFu;
     } else {
-      const unsigned idx = (unsigned)(((h >> 32) * (uint64_t)g.nC) >> 32);
-      c = b.corpusPts[idx];
+      c = b.corpusPts[probe_index(g, b, h)];
       valid = true;
       score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
     }
@@ -708,7 +741,7 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   myTerm += (unsigned long long)evals * count;
 }
 
-template <typename Word, int WAVES>
+template <typename Word, int WAVES, bool WAVE_KERNEL>
 __device__ __forceinline__ void flush_counters(const SynthBuffers &b, unsigned long long myEval,
                                                unsigned long long myTerm, unsigned myBet,
                                                unsigned long long *sEval, unsigned long long *sTerm, unsigned *sBet)
@@ -731,6 +764,7 @@ __device__ __forceinline__ void flush_counters(const SynthBuffers &b, unsigned l
     const unsigned slot = blockIdx.x & (CNT_SLOTS - 1);
     if (e) atomicAdd(&b.counters->candidates[slot], e);
     if (tm) atomicAdd(&b.counters->terms[slot], tm);
+    if (WAVE_KERNEL && tm) atomicAdd(&b.counters->termsWave[slot], tm);
     if (bt) atomicAdd(&b.counters->betters[slot], bt);
   }
 }
@@ -759,7 +793,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuff
   for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES)
     visit_wave<Word, G0, GL, false>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval, myTerm,
                                     myBet);
-  flush_counters<Word, WAVES>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
+  flush_counters<Word, WAVES, true>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
 }
 
 // Sliding-window batched visits in one persistent launch.  Waves take targets in
@@ -833,7 +867,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_window(Geometry g, SynthBu
                                    myBet);
     if (lane == 0) atomicAdd(done, 1u);
   }
-  flush_counters<Word, WAVES>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
+  flush_counters<Word, WAVES, false>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
 }
 
 // Chunk sizes at or above this use one wavefront per target (env RESYNTH_WAVE_MIN), and
@@ -846,6 +880,8 @@ static unsigned wave_min()
   }();
   return v;
 }
+unsigned synth_wave_min() { return wave_min(); }
+
 static int first_group()
 {
   static int v = [] {
@@ -1476,6 +1512,27 @@ hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *t
 }
 
 // ------------------------------------------------------------------ small kernels
+
+// First index in the row-major corpus point list of every band's first row.
+__global__ void k_band_starts(const int2 *corpusPts, const unsigned *nC, int ch, uint32_t *out)
+{
+  const int x = (int)threadIdx.x;
+  if (x > kBands) return;
+  const int row = x * ch / kBands;
+  unsigned lo = 0, hi = *nC;
+  while (lo < hi) {
+    const unsigned mid = (lo + hi) / 2;
+    if (corpusPts[mid].y < row) lo = mid + 1;
+    else hi = mid;
+  }
+  out[x] = lo;
+}
+
+hipError_t launch_band_starts(hipStream_t s, const int2 *corpusPts, const unsigned *nC, int ch, uint32_t *out)
+{
+  hipLaunchKernelGGL(k_band_starts, dim3(1), dim3(64), 0, s, corpusPts, nC, ch, out);
+  return hipGetLastError();
+}
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
 {
