@@ -542,10 +542,12 @@ int prepare_engine(Job &job, Prepared &pr)
     return e ? std::atoi(e) : -1;
   }();
   g.probeByPixel = probeKnob >= 0 ? (probeKnob ? 1 : 0) : ((uint64_t)nC * 2 >= (uint64_t)nCpix ? 1 : 0);
-  // Batched random probes come from the XCD's band of corpus rows when the corpus words
-  // outgrow an XCD's L2 (RESYNTH_XCD_BANDS=0 turns this off).
+  // RESYNTH_XCD_BANDS=1: batched random probes come from the XCD's band of corpus rows
+  // when the corpus words outgrow an XCD's L2.  Off by default: on a 2048x2048 heal it cuts
+  // the visit kernel's HBM fetches about 9x but not its time (the scorer is bound by
+  // issue and latency, not by fetches), and it narrows each visit's probe distribution.
   const char *bandEnv = std::getenv("RESYNTH_XCD_BANDS");
-  const bool bandsOn = bandEnv ? std::atoi(bandEnv) != 0 : true;
+  const bool bandsOn = bandEnv ? std::atoi(bandEnv) != 0 : false;
   g.bands = (!serialMode && bandsOn && nCpix * wb > (4u << 20) && ch >= 8 * kBands) ? kBands : 0;
 
   SynthBuffers b = {};
