@@ -278,6 +278,38 @@ void resynth_prng_ranges(int prng, unsigned int seed, unsigned int n, const uint
 void resynth_metric_tables(double sensitivity, double mapWeight, uint16_t *T, uint32_t *Tm);
 int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsigned int n);
 
+/* ---- Learned candidate proposals.  No counterpart in the reference.  A small BF16 MLP
+ * (FP32 accumulate) embeds a pixel's neighbourhood (the 30 nearest offsets: R, G, B / 255
+ * and a presence flag each, 120 inputs zero-padded to 128) through 256 -> 256 -> 64 with
+ * ReLU and L2 normalisation; it is trained with InfoNCE (shared weights for both towers)
+ * by hand-written forward/backward GEMM kernels on the GPU. ---- */
+typedef struct ResynthModel ResynthModel;
+
+/* Number of FP32 parameters: W1[128x256], b1[256], W2[256x256], b2[256], W3[256x64], b3[64],
+ * weights row-major [in x out], in that order. */
+unsigned int resynth_model_param_count(void);
+/* He-uniform weights (seeded), zero biases; NULL without a device. */
+ResynthModel *resynth_model_create(unsigned int seed);
+void resynth_model_free(ResynthModel *model);
+int resynth_model_get_params(const ResynthModel *model, float *out);
+int resynth_model_set_params(ResynthModel *model, const float *in);   /* also resets Adam */
+/* x: n rows of 128 bf16 bit patterns (host); out: n x 64 normalised embeddings (host). */
+int resynth_model_embed(const ResynthModel *model, const uint16_t *x, unsigned int n, float *out);
+/* InfoNCE loss over n pairs (n a multiple of 64; logits z_q z_k^T / tau, positives on the
+ * diagonal) and its gradient with respect to the parameters (host arrays; grad may be NULL). */
+int resynth_model_infonce(ResynthModel *model, const uint16_t *xq, const uint16_t *xk, unsigned int n, float tau,
+                          float *loss, float *grad);
+/* Adam (lr, betas 0.9/0.999) for `steps` steps on batches of `batch` pairs (a multiple of
+ * 64) drawn by a seeded hash from the n host pairs; losses[steps] (may be NULL) gets each
+ * step's loss. */
+int resynth_model_train(ResynthModel *model, const uint16_t *xq, const uint16_t *xk, unsigned int n,
+                        unsigned int steps, unsigned int batch, float lr, float tau, unsigned int seed,
+                        float *losses);
+/* The MFMA GEMM used by the network, on host arrays: C[M x N] (fp32) = op(A) op(B) with bf16
+ * A ([M x K], or [K x M] when ta) and B ([K x N], or [N x K] when tb); K a multiple of 32,
+ * M and N multiples of 8.  For tests. */
+int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, const uint16_t *B, float *C);
+
 /* sizeof of ResynthOptions, ResynthStats, ResynthRequest, ResynthKernelConfig and
  * ResynthBatchStats, then the offsets of ResynthRequest.params, ResynthRequest.error and
  * ResynthBatchStats.prepMs, into out[0..min(n, 8)); returns how many were written.  Lets

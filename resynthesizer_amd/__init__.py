@@ -146,6 +146,19 @@ SIGNATURES = {
     "resynth_autotune_batch": (ctypes.c_int, [_P(ResynthRequest), ctypes.c_uint, ctypes.c_int,
                                               _P(ResynthKernelConfig), ctypes.c_void_p]),
     "resynth_struct_sizes": (ctypes.c_uint, [ctypes.c_void_p, ctypes.c_uint]),
+    "resynth_model_param_count": (ctypes.c_uint, []),
+    "resynth_model_create": (ctypes.c_void_p, [ctypes.c_uint]),
+    "resynth_model_free": (None, [ctypes.c_void_p]),
+    "resynth_model_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_model_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_model_embed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]),
+    "resynth_model_infonce": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
+                                             ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_model_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
+                                           ctypes.c_uint, ctypes.c_uint, ctypes.c_float, ctypes.c_float,
+                                           ctypes.c_uint, ctypes.c_void_p]),
+    "resynth_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "resynth_score_frozen": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, _P(TFormatIndices),
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
@@ -351,6 +364,78 @@ def autotune_batch(images, masks, fmts, params, device=False):
     ms = np.zeros(12, np.float64)
     rc = lib().resynth_autotune_batch(reqs, len(images), int(device), ctypes.byref(best), ms.ctypes.data)
     return rc, (best.wavesPerBlock, best.firstGroup, best.laterGroup), ms
+
+
+class Model:
+    """The learned-proposal network on the device (resynth_model_*)."""
+
+    def __init__(self, seed=1):
+        self.h = lib().resynth_model_create(seed)
+        if not self.h:
+            raise RuntimeError("resynth_model_create failed (no device)")
+        self.n = lib().resynth_model_param_count()
+
+    def close(self):
+        if self.h:
+            lib().resynth_model_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.n, np.float32)
+        assert lib().resynth_model_get_params(self.h, out.ctypes.data) == 0
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n
+        assert lib().resynth_model_set_params(self.h, p.ctypes.data) == 0
+
+    def embed(self, x_bits):
+        x = np.ascontiguousarray(x_bits, dtype=np.uint16)
+        out = np.zeros((x.shape[0], 64), np.float32)
+        assert lib().resynth_model_embed(self.h, x.ctypes.data, x.shape[0], out.ctypes.data) == 0
+        return out
+
+    def infonce(self, xq_bits, xk_bits, tau):
+        xq = np.ascontiguousarray(xq_bits, dtype=np.uint16)
+        xk = np.ascontiguousarray(xk_bits, dtype=np.uint16)
+        loss = np.zeros(1, np.float32)
+        grad = np.zeros(self.n, np.float32)
+        rc = lib().resynth_model_infonce(self.h, xq.ctypes.data, xk.ctypes.data, xq.shape[0], tau, loss.ctypes.data,
+                                         grad.ctypes.data)
+        if rc:
+            raise ValueError("resynth_model_infonce rejected its arguments")
+        return float(loss[0]), grad
+
+    def train(self, xq_bits, xk_bits, steps, batch=256, lr=1e-3, tau=0.1, seed=1):
+        xq = np.ascontiguousarray(xq_bits, dtype=np.uint16)
+        xk = np.ascontiguousarray(xk_bits, dtype=np.uint16)
+        losses = np.zeros(steps, np.float32)
+        rc = lib().resynth_model_train(self.h, xq.ctypes.data, xk.ctypes.data, xq.shape[0], steps, batch, lr, tau,
+                                       seed, losses.ctypes.data)
+        if rc:
+            raise ValueError("resynth_model_train rejected its arguments")
+        return losses
+
+
+def gemm_bf16(a_bits, b_bits, ta=False, tb=False):
+    """MFMA GEMM on host arrays of bf16 bit patterns: op(A) op(B) in fp32."""
+    A = np.ascontiguousarray(a_bits, dtype=np.uint16)
+    B = np.ascontiguousarray(b_bits, dtype=np.uint16)
+    M = A.shape[1] if ta else A.shape[0]
+    K = A.shape[0] if ta else A.shape[1]
+    N = B.shape[0] if tb else B.shape[1]
+    C = np.zeros((M, N), np.float32)
+    rc = lib().resynth_gemm_bf16(int(ta), int(tb), M, N, K, A.ctypes.data, B.ctypes.data, C.ctypes.data)
+    if rc:
+        raise ValueError("resynth_gemm_bf16 rejected its arguments")
+    return C
 
 
 def struct_sizes():

@@ -1,0 +1,467 @@
+// Learned candidate proposals for the batched engine: a small BF16 MLP (FP32 accumulate)
+// that embeds a pixel's neighbourhood, trained with InfoNCE on neighbourhood -> best-match
+// pairs logged from the plain engine, and an inverted-file index over corpus embeddings
+// whose searches are MFMA GEMMs with fused top-k.  The exact integer scorer still judges
+// every proposal; proposals only replace most of the random probes.
+//
+// Network: x (120 inputs + 8 zero pads: for each of the 30 nearest offsets, R, G, B / 255
+// and a presence flag) -> 256 -> ReLU -> 256 -> ReLU -> 64 -> L2 normalisation.
+#include "resynth_learn.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace rs {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+  // round to nearest even (inputs here are finite)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ------------------------------------------------------------------ GEMM
+//
+// C[M x N] = op(A)[M x K] * op(B)[K x N], A and B bf16, FP32 accumulation on
+// v_mfma_f32_16x16x32_bf16.  op(A) is A ([M x K] row-major, lda) or, with TA, the
+// transpose of A stored [K x M] (lda).  op(B) is B ([K x N], ldb) or, with TB, the
+// transpose of B stored [N x K] (ldb).  K must be a multiple of 32; M and N are bounded.
+// Workgroup tile 64 x 64, four waves of 32 x 32 (2 x 2 MFMA tiles), K step 32; both
+// tiles are staged in LDS k-contiguous, so each lane's MFMA operand is one 16-byte read.
+constexpr int GT = 64, GK = 32, GPAD = 8;
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_load_tiles(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B,
+                                                int ldb, int m0, int n0, int k0, uint16_t (*As)[GK + GPAD],
+                                                uint16_t (*Bs)[GK + GPAD])
+{
+  const int t = threadIdx.x;  // 256 threads
+  // A tile: 64 rows (m) x 32 k
+  if (!TA) {
+    const int r = t >> 2, c = (t & 3) * 8;  // 64 rows x 4 groups of 8
+    const int m = m0 + r;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (m < M) v = *(const uint4 *)(A + (size_t)m * lda + k0 + c);
+    *(uint4 *)&As[r][c] = v;
+  } else {
+    // stored [K x M]: 32 rows of k, 64 m each; thread reads 8 consecutive m of one k
+    const int kr = t >> 3, c = (t & 7) * 8;
+    const int m = m0 + c;
+    uint16_t tmp[8];
+    if (m + 7 < M) {
+      uint4 v = *(const uint4 *)(A + (size_t)(k0 + kr) * lda + m);
+      __builtin_memcpy(tmp, &v, 16);
+    } else {
+      for (int i = 0; i < 8; i++) tmp[i] = (m + i < M) ? A[(size_t)(k0 + kr) * lda + m + i] : (uint16_t)0;
+    }
+    for (int i = 0; i < 8; i++) As[c + i][kr] = tmp[i];
+  }
+  // B tile into Bs[n][k]
+  if (TB) {
+    const int r = t >> 2, c = (t & 3) * 8;
+    const int n = n0 + r;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (n < N) v = *(const uint4 *)(B + (size_t)n * ldb + k0 + c);
+    *(uint4 *)&Bs[r][c] = v;
+  } else {
+    const int kr = t >> 3, c = (t & 7) * 8;
+    const int n = n0 + c;
+    uint16_t tmp[8];
+    if (n + 7 < N) {
+      uint4 v = *(const uint4 *)(B + (size_t)(k0 + kr) * ldb + n);
+      __builtin_memcpy(tmp, &v, 16);
+    } else {
+      for (int i = 0; i < 8; i++) tmp[i] = (n + i < N) ? B[(size_t)(k0 + kr) * ldb + n + i] : (uint16_t)0;
+    }
+    for (int i = 0; i < 8; i++) Bs[c + i][kr] = tmp[i];
+  }
+}
+
+// Accumulates the workgroup's 64 x 64 tile; acc[a][b] is the 16 x 16 MFMA tile (a, b) of
+// this wave's 32 x 32 quadrant.  Lane l of a 16 x 16 tile holds rows 4*(l/16)+r, column l%16.
+template <bool TA, bool TB>
+__device__ __forceinline__ void gemm_tile(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                                          int m0, int n0, f32x4 (&acc)[2][2])
+{
+  __shared__ __attribute__((aligned(16))) uint16_t As[GT][GK + GPAD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[GT][GK + GPAD];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += GK) {
+    gemm_load_tiles<TA, TB>(M, N, K, A, lda, B, ldb, m0, n0, k0, As, Bs);
+    __syncthreads();
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+    bf16x8 fa[2], fb[2];
+    for (int a = 0; a < 2; a++) fa[a] = *(const bf16x8 *)&As[wm + a * 16 + r][kq];
+    for (int b = 0; b < 2; b++) fb[b] = *(const bf16x8 *)&Bs[wn + b * 16 + r][kq];
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    __syncthreads();
+  }
+}
+
+// Epilogues
+enum { EPI_F32 = 0, EPI_BIAS_RELU_BF16 = 1, EPI_BIAS_F32 = 2, EPI_BF16 = 3 };
+
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B,
+                                              int ldb, float *C, uint16_t *Cb, int ldc, const float *bias,
+                                              float scale)
+{
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  f32x4 acc[2][2];
+  gemm_tile<TA, TB>(M, N, K, A, lda, B, ldb, m0, n0, acc);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm + a * 16 + (lane >> 4) * 4 + r;
+        const int n = n0 + wn + b * 16 + (lane & 15);
+        if (m >= M || n >= N) continue;
+        float v = acc[a][b][r] * scale;
+        if (EPI == EPI_BIAS_RELU_BF16 || EPI == EPI_BIAS_F32) v += bias[n];
+        if (EPI == EPI_BIAS_RELU_BF16) {
+          v = v > 0.f ? v : 0.f;
+          Cb[(size_t)m * ldc + n] = f2bf(v);
+        } else if (EPI == EPI_BF16) {
+          Cb[(size_t)m * ldc + n] = f2bf(v);
+        } else {
+          C[(size_t)m * ldc + n] = v;
+        }
+      }
+}
+
+template <bool TA, bool TB, int EPI>
+static hipError_t gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                       float *C, uint16_t *Cb, int ldc, const float *bias, float scale)
+{
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (K % GK) return hipErrorInvalidValue;
+  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+  hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, Cb, ldc, bias, scale);
+  return hipGetLastError();
+}
+
+hipError_t learn_gemm(hipStream_t s, int ta, int tb, int M, int N, int K, const uint16_t *A, int lda,
+                      const uint16_t *B, int ldb, float *C, int ldc)
+{
+  if (!ta && !tb) return gemm<false, false, EPI_F32>(s, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, 1.f);
+  if (!ta && tb) return gemm<false, true, EPI_F32>(s, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, 1.f);
+  if (ta && !tb) return gemm<true, false, EPI_F32>(s, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, 1.f);
+  return gemm<true, true, EPI_F32>(s, M, N, K, A, lda, B, ldb, C, nullptr, ldc, nullptr, 1.f);
+}
+
+// ------------------------------------------------------------------ elementwise helpers
+
+__global__ void k_f32_to_bf16(const float *x, uint16_t *y, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+__global__ void k_zero(float *x, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.f;
+}
+
+// y += x
+__global__ void k_axpy(const float *x, float *y, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+__global__ void k_scale1(float *x, float a) { x[0] *= a; }
+
+static hipError_t to_bf16(hipStream_t s, const float *x, uint16_t *y, size_t n)
+{
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
+  return hipGetLastError();
+}
+
+// One wavefront per row of 64: z = e / |e|, also returning |e| (guarded against 0).
+__global__ void k_normalize64(const float *e, float *z, float *norm, uint16_t *zb, int rows)
+{
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const float v = e[(size_t)row * 64 + lane];
+  float s = v * v;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  const float n = sqrtf(fmaxf(s, 1e-12f));
+  const float zz = v / n;
+  if (z) z[(size_t)row * 64 + lane] = zz;
+  if (zb) zb[(size_t)row * 64 + lane] = f2bf(zz);
+  if (norm && lane == 0) norm[row] = n;
+}
+
+// Backward of the normalisation: de = (dz - z (z . dz)) / |e|.
+__global__ void k_normalize64_bwd(const float *z, const float *norm, const float *dz, uint16_t *deb, float *de,
+                                  int rows)
+{
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= rows) return;
+  const float zz = z[(size_t)row * 64 + lane], g = dz[(size_t)row * 64 + lane];
+  float d = zz * g;
+  for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
+  const float v = (g - zz * d) / norm[row];
+  if (deb) deb[(size_t)row * 64 + lane] = f2bf(v);
+  if (de) de[(size_t)row * 64 + lane] = v;
+}
+
+// InfoNCE over a B x B logit matrix L (row i: query i against every key; the positive is
+// the diagonal).  One workgroup per row: loss_i = logsumexp(L_i) - L_ii, and
+// dL_i = (softmax(L_i) - e_i) / B written as bf16 (and transposed into dLt).
+__global__ __launch_bounds__(256) void k_infonce_rows(const float *L, int B, float *rowLoss, uint16_t *dL,
+                                                      uint16_t *dLt)
+{
+  const int i = blockIdx.x;
+  __shared__ float red[256];
+  const float *row = L + (size_t)i * B;
+  float mx = -3.4e38f;
+  for (int j = threadIdx.x; j < B; j += 256) mx = fmaxf(mx, row[j]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  mx = red[0];
+  __syncthreads();
+  float sum = 0.f;
+  for (int j = threadIdx.x; j < B; j += 256) sum += expf(row[j] - mx);
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  sum = red[0];
+  const float lse = mx + logf(sum);
+  if (threadIdx.x == 0) rowLoss[i] = lse - row[i];
+  const float inv = 1.f / (float)B;
+  for (int j = threadIdx.x; j < B; j += 256) {
+    float g = expf(row[j] - lse);
+    if (j == i) g -= 1.f;
+    g *= inv;
+    dL[(size_t)i * B + j] = f2bf(g);
+    dLt[(size_t)j * B + i] = f2bf(g);
+  }
+}
+
+// Backward of ReLU through a bf16 activation: g_pre = g * (act > 0), as bf16.
+__global__ void k_relu_bwd(const float *g, const uint16_t *act, uint16_t *out, size_t n)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (act[i] != 0 && !(act[i] & 0x8000u)) ? f2bf(g[i]) : (uint16_t)0;
+}
+
+// Column sums of a rows x cols fp32 matrix into out (bias gradients).
+__global__ void k_colsum(const float *g, int rows, int cols, float *out)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; r++) s += g[(size_t)r * cols + c];
+  out[c] = s;
+}
+
+// Column sums of a bf16 matrix (bias gradient of a layer whose output gradient is bf16).
+__global__ void k_colsum_bf16(const uint16_t *g, int rows, int cols, float *out)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; r++) s += bf2f(g[(size_t)r * cols + c]);
+  out[c] = s;
+}
+
+__global__ void k_sum(const float *x, int n, float *out)
+{
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < n; i += 256) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// Adam on FP32 master weights, refreshing the bf16 copies used by the GEMMs.
+__global__ void k_adam(float *w, const float *g, float *m, float *v, uint16_t *wb, size_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+// ------------------------------------------------------------------ model
+
+// Parameter offsets in the flat FP32 vector (weights [in x out] row-major).
+constexpr size_t OW1 = 0, OB1 = OW1 + (size_t)kIn * kH1, OW2 = OB1 + kH1, OB2 = OW2 + (size_t)kH1 * kH2,
+                 OW3 = OB2 + kH2, OB3 = OW3 + (size_t)kH2 * kEmb, NPARAM = OB3 + kEmb;
+static_assert(NPARAM == kParamCount, "parameter layout");
+
+// Forward of n rows of x (bf16 [n x kIn]) into activations (bf16) and e (fp32 [n x 64]).
+hipError_t mlp_forward(hipStream_t s, const float *w, const uint16_t *wb, const uint16_t *x, int n, uint16_t *h1,
+                       uint16_t *h2, float *e)
+{
+  hipError_t r;
+  if ((r = gemm<false, false, EPI_BIAS_RELU_BF16>(s, n, kH1, kIn, x, kIn, wb + OW1, kH1, nullptr, h1, kH1, w + OB1,
+                                                  1.f)))
+    return r;
+  if ((r = gemm<false, false, EPI_BIAS_RELU_BF16>(s, n, kH2, kH1, h1, kH1, wb + OW2, kH2, nullptr, h2, kH2, w + OB2,
+                                                  1.f)))
+    return r;
+  return gemm<false, false, EPI_BIAS_F32>(s, n, kEmb, kH2, h2, kH2, wb + OW3, kEmb, e, nullptr, kEmb, w + OB3, 1.f);
+}
+
+hipError_t mlp_embed(hipStream_t s, const float *w, const uint16_t *wb, const uint16_t *x, int n, LearnScratch &t,
+                     uint16_t *zb, float *z)
+{
+  hipError_t r;
+  if ((r = mlp_forward(s, w, wb, x, n, t.h1, t.h2, t.e))) return r;
+  hipLaunchKernelGGL(k_normalize64, dim3((n + 3) / 4), dim3(256), 0, s, t.e, z, nullptr, zb, n);
+  return hipGetLastError();
+}
+
+// Backward of one tower given dE (fp32 [n x 64]); accumulates parameter gradients into
+// grad (which the caller zeroes).  x, h1, h2 are the forward's inputs and activations.
+static hipError_t mlp_backward(hipStream_t s, const uint16_t *wb, const uint16_t *x, const uint16_t *h1,
+                               const uint16_t *h2, const uint16_t *dEb, int n, LearnScratch &t, float *grad)
+{
+  hipError_t r;
+  // dW3 = h2^T dE, db3 = colsum dE
+  if ((r = gemm<true, false, EPI_F32>(s, kH2, kEmb, n, h2, kH2, dEb, kEmb, t.gw, nullptr, kEmb, nullptr, 1.f))) return r;
+  hipLaunchKernelGGL(k_axpy, dim3((kH2 * kEmb + 255) / 256), dim3(256), 0, s, t.gw, grad + OW3, (size_t)kH2 * kEmb);
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(kEmb), 0, s, dEb, n, kEmb, t.gb);
+  hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, s, t.gb, grad + OB3, (size_t)kEmb);
+  // dH2 = dE W3^T -> through ReLU
+  if ((r = gemm<false, true, EPI_F32>(s, n, kH2, kEmb, dEb, kEmb, wb + OW3, kEmb, t.g, nullptr, kH2, nullptr, 1.f)))
+    return r;
+  hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)(((size_t)n * kH2 + 255) / 256)), dim3(256), 0, s, t.g, h2, t.gb16,
+                     (size_t)n * kH2);
+  if ((r = gemm<true, false, EPI_F32>(s, kH1, kH2, n, h1, kH1, t.gb16, kH2, t.gw, nullptr, kH2, nullptr, 1.f))) return r;
+  hipLaunchKernelGGL(k_axpy, dim3((kH1 * kH2 + 255) / 256), dim3(256), 0, s, t.gw, grad + OW2, (size_t)kH1 * kH2);
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(kH2), 0, s, t.gb16, n, kH2, t.gb);
+  hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, s, t.gb, grad + OB2, (size_t)kH2);
+  // dH1 = dH2pre W2^T -> through ReLU
+  if ((r = gemm<false, true, EPI_F32>(s, n, kH1, kH2, t.gb16, kH2, wb + OW2, kH2, t.g, nullptr, kH1, nullptr, 1.f)))
+    return r;
+  hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)(((size_t)n * kH1 + 255) / 256)), dim3(256), 0, s, t.g, h1, t.gb16,
+                     (size_t)n * kH1);
+  if ((r = gemm<true, false, EPI_F32>(s, kIn, kH1, n, x, kIn, t.gb16, kH1, t.gw, nullptr, kH1, nullptr, 1.f))) return r;
+  hipLaunchKernelGGL(k_axpy, dim3((kIn * kH1 + 255) / 256), dim3(256), 0, s, t.gw, grad + OW1, (size_t)kIn * kH1);
+  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(kH1), 0, s, t.gb16, n, kH1, t.gb);
+  hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, s, t.gb, grad + OB1, (size_t)kH1);
+  return hipGetLastError();
+}
+
+// InfoNCE loss and gradients for n pairs (xq[i], xk[i]) with temperature tau; both towers
+// share the weights.  grad (fp32, kParamCount) is overwritten; *loss (device) gets the mean.
+hipError_t mlp_infonce(hipStream_t s, const float *w, const uint16_t *wb, const uint16_t *xq, const uint16_t *xk, int n,
+                       float tau, LearnScratch &q, LearnScratch &k, float *grad, float *loss)
+{
+  hipError_t r;
+  hipLaunchKernelGGL(k_zero, dim3((kParamCount + 255) / 256), dim3(256), 0, s, grad, (size_t)kParamCount);
+  if ((r = mlp_forward(s, w, wb, xq, n, q.h1, q.h2, q.e))) return r;
+  if ((r = mlp_forward(s, w, wb, xk, n, k.h1, k.h2, k.e))) return r;
+  hipLaunchKernelGGL(k_normalize64, dim3((n + 3) / 4), dim3(256), 0, s, q.e, q.z, q.norm, q.zb, n);
+  hipLaunchKernelGGL(k_normalize64, dim3((n + 3) / 4), dim3(256), 0, s, k.e, k.z, k.norm, k.zb, n);
+  // L = zq zk^T / tau
+  if ((r = gemm<false, true, EPI_F32>(s, n, n, kEmb, q.zb, kEmb, k.zb, kEmb, q.L, nullptr, n, nullptr, 1.f / tau)))
+    return r;
+  hipLaunchKernelGGL(k_infonce_rows, dim3(n), dim3(256), 0, s, q.L, n, q.rowLoss, q.dL, q.dLt);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, s, q.rowLoss, n, loss);
+  hipLaunchKernelGGL(k_scale1, dim3(1), dim3(1), 0, s, loss, 1.f / (float)n);
+  // dzq = dL zk / tau ; dzk = dL^T zq / tau
+  if ((r = gemm<false, false, EPI_F32>(s, n, kEmb, n, q.dL, n, k.zb, kEmb, q.dz, nullptr, kEmb, nullptr, 1.f / tau)))
+    return r;
+  if ((r = gemm<false, false, EPI_F32>(s, n, kEmb, n, q.dLt, n, q.zb, kEmb, k.dz, nullptr, kEmb, nullptr, 1.f / tau)))
+    return r;
+  hipLaunchKernelGGL(k_normalize64_bwd, dim3((n + 3) / 4), dim3(256), 0, s, q.z, q.norm, q.dz, q.dEb, nullptr, n);
+  hipLaunchKernelGGL(k_normalize64_bwd, dim3((n + 3) / 4), dim3(256), 0, s, k.z, k.norm, k.dz, k.dEb, nullptr, n);
+  if ((r = mlp_backward(s, wb, xq, q.h1, q.h2, q.dEb, n, q, grad))) return r;
+  return mlp_backward(s, wb, xk, k.h1, k.h2, k.dEb, n, k, grad);
+}
+
+hipError_t adam_step(hipStream_t s, float *w, const float *g, float *m, float *v, uint16_t *wb, float lr, int step)
+{
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)step), c2 = 1.f - std::pow(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3((kParamCount + 255) / 256), dim3(256), 0, s, w, g, m, v, wb, (size_t)kParamCount, lr,
+                     b1, b2, 1e-8f, c1, c2);
+  return hipGetLastError();
+}
+
+hipError_t params_to_bf16(hipStream_t s, const float *w, uint16_t *wb) { return to_bf16(s, w, wb, kParamCount); }
+
+// dst row i = src row idx[i]; cols is a multiple of 8 (16-byte rows pieces).
+__global__ void k_gather_rows(const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst)
+{
+  const int per = cols / 8;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * per) return;
+  const int i = (int)(t / per), c = (int)(t % per);
+  ((uint4 *)(dst + (size_t)i * cols))[c] = ((const uint4 *)(src + (size_t)idx[i] * cols))[c];
+}
+
+hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst)
+{
+  if (n <= 0) return hipSuccess;
+  const size_t work = (size_t)n * (cols / 8);
+  hipLaunchKernelGGL(k_gather_rows, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, src, cols, idx, n, dst);
+  return hipGetLastError();
+}
+
+__device__ __forceinline__ uint64_t lmix64(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// idx[i] = uniform draw in [0, range) from a counter-based hash of (key, i).
+__global__ void k_hash_idx(uint32_t *idx, int n, uint32_t range, uint64_t key)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t h = lmix64(key * 0x100000001B3ull + (uint64_t)i);
+  idx[i] = (uint32_t)(((h >> 32) * (uint64_t)range) >> 32);
+}
+
+hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uint64_t key)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hash_idx, dim3((n + 255) / 256), dim3(256), 0, s, idx, n, range, key);
+  return hipGetLastError();
+}
+
+}  // namespace rs

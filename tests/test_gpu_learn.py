@@ -1,0 +1,93 @@
+"""GPU tests of the learned-proposal network: the MFMA GEMM (which also pins the
+v_mfma_f32_16x16x32_bf16 lane layout the kernels assume), the forward pass and the InfoNCE
+gradient against the float64 CPU oracle, and training."""
+import numpy as np
+import pytest
+
+import mlp_oracle as MO
+import resynthesizer_amd as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16(rng, shape, scale=1.0):
+    return MO.bf16_bits(rng.normal(0, scale, size=shape).astype(np.float32))
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_mfma_gemm_matches_float64(ta, tb):
+    rng = np.random.RandomState(10 + 2 * ta + tb)
+    M, N, K = 72, 200, 96
+    A = _bf16(rng, (K, M) if ta else (M, K))
+    B = _bf16(rng, (N, K) if tb else (K, N))
+    C = R.gemm_bf16(A, B, ta=bool(ta), tb=bool(tb))
+    a = MO.from_bits(A)
+    b = MO.from_bits(B)
+    ref = (a.T if ta else a) @ (b.T if tb else b)
+    assert np.abs(C - ref).max() <= 1e-4 * np.abs(ref).max() + 1e-5
+
+
+def _features(n, seed):
+    rng = np.random.RandomState(seed)
+    x = np.zeros((n, 128), np.float32)
+    x[:, :120] = rng.uniform(0, 1, size=(n, 120))
+    pres = rng.randint(0, 2, size=(n, 30)).astype(np.float32)
+    x[:, 3:120:4] = pres
+    for c in range(3):
+        x[:, c:120:4] *= pres
+    return x
+
+
+def _oracle_params(model):
+    """The model's parameters as the GPU uses them: weights rounded to bf16, biases fp32."""
+    p = model.params().astype(np.float64)
+    d = MO.unpack(p)
+    for k in ("W1", "W2", "W3"):
+        d[k] = MO.bf16_round(d[k])
+    return MO.pack(d)
+
+
+def test_forward_matches_oracle():
+    m = R.Model(seed=7)
+    x = _features(300, 1)
+    xb = MO.bf16_bits(x)
+    z = m.embed(xb)
+    zr, _ = MO.forward(_oracle_params(m), MO.from_bits(xb))
+    assert z.shape == (300, 64)
+    assert np.allclose((z * z).sum(1), 1.0, atol=1e-4)
+    cos = (z * zr).sum(1)
+    assert cos.min() > 0.995, cos.min()
+    assert np.abs(z - zr).max() < 0.05
+
+
+def test_infonce_gradient_matches_oracle():
+    """Loss within 1% and every parameter block's gradient within 5% (relative norm) of the
+    float64 oracle at bf16 inputs and weights; tau 0.1, 128 pairs."""
+    m = R.Model(seed=3)
+    xq = _features(128, 2)
+    xk = np.clip(xq + np.random.RandomState(3).normal(0, 0.03, size=xq.shape).astype(np.float32), 0, 1)
+    xk[:, 120:] = 0
+    qb, kb = MO.bf16_bits(xq), MO.bf16_bits(xk)
+    loss, g = m.infonce(qb, kb, 0.1)
+    lr, gr = MO.infonce(_oracle_params(m), MO.from_bits(qb), MO.from_bits(kb), 0.1)
+    assert abs(loss - lr) <= 0.01 * abs(lr), (loss, lr)
+    o = 0
+    for name, shp in MO.SHAPES:
+        n = int(np.prod(shp))
+        a, b = g[o:o + n].astype(np.float64), gr[o:o + n]
+        o += n
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        assert rel < 0.05, (name, rel)
+
+
+def test_training_reduces_loss():
+    """Adam on 4096 noisy pairs: the loss of the last 20 steps is well below the first."""
+    m = R.Model(seed=11)
+    xq = _features(4096, 5)
+    xk = np.clip(xq + np.random.RandomState(6).normal(0, 0.05, size=xq.shape).astype(np.float32), 0, 1)
+    xk[:, 120:] = 0
+    losses = m.train(MO.bf16_bits(xq), MO.bf16_bits(xk), steps=300, batch=256, lr=2e-3, tau=0.1, seed=1)
+    assert np.isfinite(losses).all()
+    first, last = losses[:20].mean(), losses[-20:].mean()
+    print("loss %.3f -> %.3f" % (first, last))
+    assert last < 0.5 * first, (first, last)
