@@ -143,6 +143,11 @@ enum { RESYNTH_MODE_SERIAL = 0, RESYNTH_MODE_BATCHED = 1 };
  * (the stand-alone build through lib/glibProxy.c). */
 enum { RESYNTH_PRNG_GRAND = 0, RESYNTH_PRNG_LIBC = 1 };
 
+/* Learned-proposal network (opaque; see resynth_model_* below).  No counterpart in the
+ * reference: an extension of the batched mode that proposes candidates from corpus
+ * embeddings so that most random probes can be cut; the exact scorer still judges. */
+typedef struct ResynthModel ResynthModel;
+
 typedef struct {
   int mode;            /* RESYNTH_MODE_* */
   int prng;            /* RESYNTH_PRNG_* (serial mode) */
@@ -152,6 +157,10 @@ typedef struct {
   unsigned int batch;  /* batched mode: largest number of concurrent target visits
                           (0 = automatic: nT/32 clamped to [64, 65536]) */
   int timing;          /* nonzero: time every synthesis kernel with HIP events */
+  const ResynthModel *model;  /* batched mode: learned candidate proposals from this network
+                                 (NULL: the plain method) */
+  unsigned int learnedProbes; /* random probes per visit that has proposals (0 = 32); visits
+                                 of the first, small pass-0 chunks keep maxProbeCount */
 } ResynthOptions;
 
 typedef struct {
@@ -173,6 +182,10 @@ typedef struct {
   unsigned int waveLaunches;          /* batched: launches of the wavefront-per-target kernel */
   double waveKernelMs;                /* their summed durations (timing != 0) */
   unsigned long long wavePatchTerms;  /* patch terms they scored */
+  unsigned int learnedLaunches;       /* launches whose visits scored learned proposals */
+  unsigned int learnedKeys;           /* corpus keys embedded (0: no proposals) */
+  unsigned int learnedClusters;       /* inverted-file clusters */
+  double learnedIndexMs;              /* host wall clock of embedding the corpus and indexing */
 } ResynthStats;
 
 void resynth_set_default_mode(int mode);
@@ -278,13 +291,11 @@ void resynth_prng_ranges(int prng, unsigned int seed, unsigned int n, const uint
 void resynth_metric_tables(double sensitivity, double mapWeight, uint16_t *T, uint32_t *Tm);
 int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsigned int n);
 
-/* ---- Learned candidate proposals.  No counterpart in the reference.  A small BF16 MLP
+/* ---- Learned candidate proposals (continued).  A small BF16 MLP
  * (FP32 accumulate) embeds a pixel's neighbourhood (the 30 nearest offsets: R, G, B / 255
  * and a presence flag each, 120 inputs zero-padded to 128) through 256 -> 256 -> 64 with
  * ReLU and L2 normalisation; it is trained with InfoNCE (shared weights for both towers)
  * by hand-written forward/backward GEMM kernels on the GPU. ---- */
-typedef struct ResynthModel ResynthModel;
-
 /* Number of FP32 parameters: W1[128x256], b1[256], W2[256x256], b2[256], W3[256x64], b3[64],
  * weights row-major [in x out], in that order. */
 unsigned int resynth_model_param_count(void);
@@ -309,6 +320,20 @@ int resynth_model_train(ResynthModel *model, const uint16_t *xq, const uint16_t 
  * A ([M x K], or [K x M] when ta) and B ([K x N], or [N x K] when tb); K a multiple of 32,
  * M and N multiples of 8.  For tests. */
 int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, const uint16_t *B, float *C);
+
+/* Runs imageSynth in batched mode (the plain method, options' seed) on the image, which is
+ * synthesized in place, and appends up to maxPairs training pairs to the model's store:
+ * for targets drawn by a seeded hash, the target's neighbourhood in the result (each
+ * neighbour kept with a per-pair probability in [0.2, 1), as a partly filled hole shows it)
+ * and the neighbourhood of the corpus point the plain method chose for it.  Returns the
+ * imageSynth error code (pairs are only logged on success). */
+int resynth_model_log_pairs(ResynthModel *model, ImageBuffer *image, ImageBuffer *mask, TImageFormat format,
+                            const TImageSynthParameters *params, unsigned int maxPairs, unsigned int seed);
+unsigned int resynth_model_pair_count(const ResynthModel *model);
+void resynth_model_clear_pairs(ResynthModel *model);
+/* resynth_model_train on the logged pairs. */
+int resynth_model_fit(ResynthModel *model, unsigned int steps, unsigned int batch, float lr, float tau,
+                      unsigned int seed, float *losses);
 
 /* sizeof of ResynthOptions, ResynthStats, ResynthRequest, ResynthKernelConfig and
  * ResynthBatchStats, then the offsets of ResynthRequest.params, ResynthRequest.error and

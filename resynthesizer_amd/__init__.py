@@ -59,7 +59,7 @@ class Map(ctypes.Structure):
 
 class ResynthOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("prng", ctypes.c_int), ("seed", ctypes.c_uint), ("batch", ctypes.c_uint),
-                ("timing", ctypes.c_int)]
+                ("timing", ctypes.c_int), ("model", ctypes.c_void_p), ("learnedProbes", ctypes.c_uint)]
 
 
 class ResynthStats(ctypes.Structure):
@@ -71,7 +71,9 @@ class ResynthStats(ctypes.Structure):
                 ("totalMs", ctypes.c_double), ("pixelWordBytes", ctypes.c_uint),
                 ("windowLaunches", ctypes.c_uint), ("windowWaitsExpired", ctypes.c_uint),
                 ("waveLaunches", ctypes.c_uint), ("waveKernelMs", ctypes.c_double),
-                ("wavePatchTerms", ctypes.c_ulonglong)]
+                ("wavePatchTerms", ctypes.c_ulonglong), ("learnedLaunches", ctypes.c_uint),
+                ("learnedKeys", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
+                ("learnedIndexMs", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -157,6 +159,12 @@ SIGNATURES = {
     "resynth_model_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
                                            ctypes.c_uint, ctypes.c_uint, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_uint, ctypes.c_void_p]),
+    "resynth_model_log_pairs": (ctypes.c_int, [ctypes.c_void_p, _P(ImageBuffer), _P(ImageBuffer), ctypes.c_int,
+                                               _P(TImageSynthParameters), ctypes.c_uint, ctypes.c_uint]),
+    "resynth_model_pair_count": (ctypes.c_uint, [ctypes.c_void_p]),
+    "resynth_model_clear_pairs": (None, [ctypes.c_void_p]),
+    "resynth_model_fit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_uint, ctypes.c_void_p]),
     "resynth_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "resynth_score_frozen": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, _P(TFormatIndices),
@@ -213,8 +221,9 @@ def params_from_dict(d):
     return default_params(**d)
 
 
-def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False):
-    return ResynthOptions(mode, prng, seed, batch, int(bool(timing)))
+def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False, model=None, learned_probes=0):
+    return ResynthOptions(mode, prng, seed, batch, int(bool(timing)), model.h if model is not None else None,
+                          learned_probes)
 
 
 def format_indices(fmt):
@@ -243,7 +252,8 @@ class Result:
 
 
 def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, mask2=None,
-                width=None, mask_width=None, mask2_width=None, seed=0, batch=0, timing=False):
+                width=None, mask_width=None, mask2_width=None, seed=0, batch=0, timing=False, model=None,
+                learned_probes=0):
     """imageSynth / imageSynth2 (lib/imageSynth.c:62-217) on the GPU.
 
     image: uint8 (H, W, C) or a padded row buffer (H, rowBytes) with `width` given;
@@ -270,7 +280,7 @@ def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAN
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
-    opt = options(mode, prng, seed, batch, timing)
+    opt = options(mode, prng, seed, batch, timing, model, learned_probes)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().imageSynthEx(ctypes.byref(ib), ctypes.byref(mb), mb2p, fmt, pp, cb, None, ctypes.byref(cf),
                              ctypes.byref(opt), ctypes.byref(st))
@@ -299,14 +309,14 @@ def engine(params, ix, target_pixmap, corpus_pixmap, mode=MODE_BATCHED, prng=PRN
 
 def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, fmt, params=None,
                        mode=MODE_BATCHED, prng=PRNG_GRAND, d_mask2=None, mask2_row_bytes=0, seed=0, batch=0,
-                       timing=False, cancel=0):
+                       timing=False, cancel=0, model=None, learned_probes=0):
     """imageSynth on device-resident buffers (integer device addresses, e.g. torch data_ptr()).
     Synthesizes d_image in place.  Returns a Result."""
     log = []
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
-    opt = options(mode, prng, seed, batch, timing)
+    opt = options(mode, prng, seed, batch, timing, model, learned_probes)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().resynth_image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, d_mask2,
                                            mask2_row_bytes, fmt, pp, cb, None, ctypes.byref(cf), ctypes.byref(opt),
@@ -422,6 +432,33 @@ class Model:
         if rc:
             raise ValueError("resynth_model_train rejected its arguments")
         return losses
+
+
+def _model_log_pairs(self, image, mask, fmt, params=None, max_pairs=4096, seed=1):
+    """Runs the plain batched engine on a copy of image and logs up to max_pairs training
+    pairs.  Returns (error code, synthesized copy)."""
+    img = np.array(image, dtype=np.uint8, copy=True, order="C")
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    ib = ImageBuffer(img.ctypes.data, img.shape[1], img.shape[0], img.strides[0])
+    mb = ImageBuffer(m.ctypes.data, m.shape[1], m.shape[0], m.strides[0])
+    p = params if params is not None else default_params()
+    err = lib().resynth_model_log_pairs(self.h, ctypes.byref(ib), ctypes.byref(mb), fmt, ctypes.byref(p), max_pairs,
+                                        seed)
+    return err, img
+
+
+def _model_fit(self, steps, batch=1024, lr=1e-3, tau=0.1, seed=1):
+    losses = np.zeros(steps, np.float32)
+    rc = lib().resynth_model_fit(self.h, steps, batch, lr, tau, seed, losses.ctypes.data)
+    if rc:
+        raise ValueError("resynth_model_fit failed (no pairs or bad batch)")
+    return losses
+
+
+Model.log_pairs = _model_log_pairs
+Model.fit = _model_fit
+Model.pair_count = lambda self: lib().resynth_model_pair_count(self.h)
+Model.clear_pairs = lambda self: lib().resynth_model_clear_pairs(self.h)
 
 
 def gemm_bf16(a_bits, b_bits, ta=False, tb=False):

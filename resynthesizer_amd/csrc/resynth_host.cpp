@@ -4,6 +4,7 @@
 // All image-sized work runs on the GPU (resynth_kernels.hip).
 #include "../../include/resynth_mi355x.h"
 #include "resynth_internal.h"
+#include "resynth_learn.h"
 
 #include <hip/hip_runtime.h>
 
@@ -14,6 +15,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -278,6 +280,9 @@ struct Workspace {
   Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
       tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, firstIdx, wctl,
       bandStart;
+  // learned proposals: corpus keys, their embeddings and inverted file, query scratch
+  Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, centIdx, keyCl, bCount, bStart, bCursor,
+      bMembers, qX, qH1, qH2, qE, qZ, qCl, props, pairPts, pairSrc;
   PinnedBuf pinCounts, pinTables, pinPerm;
 };
 
@@ -585,6 +590,99 @@ int prepare_engine(Job &job, Prepared &pr)
   return 0;
 }
 
+// ------------------------------------------------------------------ learned proposals
+
+constexpr int kKeyBlock = 65536;   // corpus keys embedded per block
+
+// Inverted-file index of the corpus for learned proposals: keys at every corpus point of
+// even coordinates, their embeddings, centroids (every (nK/C)-th key) and the keys bucketed
+// by their nearest centroid.
+struct LearnedIndex {
+  bool ok = false;
+  unsigned nK = 0, C = 0;
+  int2 *keyPts = nullptr;
+  uint16_t *keyZ = nullptr, *cent = nullptr;
+  uint32_t *start = nullptr, *members = nullptr;
+};
+
+void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
+{
+  hipStream_t s = g_dev.stream;
+  Workspace &W = *job.W;
+  const ResynthModel *M = job.opt.model;
+  const Geometry &g = pr.g;
+  const int w2 = (g.cw + 1) / 2, h2 = (g.ch + 1) / 2;
+  const size_t nGrid = (size_t)w2 * h2;
+  uint8_t *flags = (uint8_t *)W.keyFlags.get(nGrid + 64);
+  HIP_TRY(learn_key_flags(s, job.wide, pr.b.cword, g.cw, g.ch, flags));
+  L.keyPts = (int2 *)W.keyPts.get(nGrid * sizeof(int2) + 64);
+  unsigned *dCount = (unsigned *)W.bCount.get(64 * 1024 * 4 + 64);
+  const size_t tb = compact_temp_bytes(nGrid);
+  HIP_TRY(launch_compact(s, flags, w2, h2, L.keyPts, dCount, W.keyTemp.get(tb + 64)));
+  unsigned nK = 0;
+  HIP_TRY(hipMemcpyAsync(&nK, dCount, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  L.nK = nK;
+  if (nK < 64) return;
+  const float *w = model_weights(M);
+  const uint16_t *wb = model_weights_bf16(M);
+  L.keyZ = (uint16_t *)W.keyZ.get((size_t)nK * kEmb * 2 + 64);
+  uint16_t *x = (uint16_t *)W.keyX.get((size_t)kKeyBlock * kIn * 2);
+  LearnScratch t;
+  t.h1 = (uint16_t *)W.keyH1.get((size_t)kKeyBlock * kH1 * 2);
+  t.h2 = (uint16_t *)W.keyH2.get((size_t)kKeyBlock * kH2 * 2);
+  t.e = (float *)W.keyE.get((size_t)kKeyBlock * kEmb * 4);
+  for (unsigned b0 = 0; b0 < nK; b0 += kKeyBlock) {
+    const int nb = (int)std::min<unsigned>(kKeyBlock, nK - b0);
+    HIP_TRY(learn_features(s, job.wide, pr.b.cword, g.cw, g.ch, 0, 0, g.nColors, pr.b.offsets, L.keyPts + b0, nb, true,
+                           true, 0, x));
+    HIP_TRY(mlp_embed(s, w, wb, x, nb, t, L.keyZ + (size_t)b0 * kEmb, nullptr));
+  }
+  // about 240 keys per cluster, at most 16384 clusters
+  unsigned C = std::min(16384u, std::max(64u, nK / 240u));
+  C = (C + 63) / 64 * 64;
+  C = std::min(C, nK);
+  L.C = C;
+  std::vector<uint32_t> ci(C);
+  for (unsigned c = 0; c < C; c++) ci[c] = (uint32_t)((uint64_t)c * nK / C);
+  uint32_t *dci = (uint32_t *)W.centIdx.get((size_t)C * 4 + 64);
+  HIP_TRY(hipMemcpyAsync(dci, ci.data(), (size_t)C * 4, hipMemcpyHostToDevice, s));
+  L.cent = (uint16_t *)W.cent.get((size_t)C * kEmb * 2 + 64);
+  HIP_TRY(gather_rows_bf16(s, L.keyZ, kEmb, dci, (int)C, L.cent));
+  int32_t *cl = (int32_t *)W.keyCl.get((size_t)nK * 4 + 64);
+  HIP_TRY(learn_topk_cols(s, L.keyZ, (int)nK, L.cent, (int)C, 1, cl));
+  uint32_t *cnt = (uint32_t *)W.bCount.get(((size_t)C + 64) * 4);
+  L.start = (uint32_t *)W.bStart.get(((size_t)C + 65) * 4);
+  uint32_t *cur = (uint32_t *)W.bCursor.get(((size_t)C + 64) * 4);
+  L.members = (uint32_t *)W.bMembers.get((size_t)nK * 4 + 64);
+  HIP_TRY(learn_bucket(s, cl, (int)nK, (int)C, cnt, L.start, cur, L.members));
+  HIP_TRY(hipStreamSynchronize(s));   // the model's weights are read on this stream
+  L.ok = true;
+}
+
+// Proposals (16 corpus points each) for targets k0..k1-1 from their current neighbourhoods.
+const int2 *learned_propose(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1)
+{
+  hipStream_t s = g_dev.stream;
+  Workspace &W = *job.W;
+  const Geometry &g = pr.g;
+  const int n = (int)(k1 - k0);
+  uint16_t *x = (uint16_t *)W.qX.get((size_t)n * kIn * 2 + 64);
+  LearnScratch t;
+  t.h1 = (uint16_t *)W.qH1.get((size_t)n * kH1 * 2 + 64);
+  t.h2 = (uint16_t *)W.qH2.get((size_t)n * kH2 * 2 + 64);
+  t.e = (float *)W.qE.get((size_t)n * kEmb * 4 + 64);
+  uint16_t *z = (uint16_t *)W.qZ.get((size_t)n * kEmb * 2 + 64);
+  int32_t *qcl = (int32_t *)W.qCl.get((size_t)n * 2 * 4 + 64);
+  int2 *props = (int2 *)W.props.get((size_t)n * 16 * sizeof(int2) + 64);
+  HIP_TRY(learn_features(s, job.wide, pr.b.tword, g.tw, g.th, g.tileH, g.tileV, g.nColors, pr.b.offsets,
+                         pr.b.targets + k0, n, false, false, 0, x));
+  HIP_TRY(mlp_embed(s, model_weights(job.opt.model), model_weights_bf16(job.opt.model), x, n, t, z, nullptr));
+  HIP_TRY(learn_topk_cols(s, z, n, L.cent, (int)L.C, 2, qcl));
+  HIP_TRY(learn_fine_top16(s, z, n, qcl, 2, L.keyZ, L.start, L.members, L.keyPts, props));
+  return props;
+}
+
 // The refiner (lib/refiner.h:43-122) over a prepared run: serial or batched passes with
 // the reference's progress, cancel and stop rule, then the run's statistics.
 int refine_engine(Job &job, Prepared &pr)
@@ -668,6 +766,14 @@ int refine_engine(Job &job, Prepared &pr)
     return (unsigned)std::max(64L, v);
   }();
   unsigned windowLaunches = 0, windowExpired = 0;
+  LearnedIndex learned;
+  if (job.opt.model && !serial) {
+    const double tl = now_ms();
+    learned_build(job, pr, learned);
+    job.st.learnedKeys = learned.nK;
+    job.st.learnedClusters = learned.ok ? learned.C : 0;
+    job.st.learnedIndexMs = now_ms() - tl;
+  }
   unsigned pass;
   unsigned passesRun = 0;
   unsigned bettersBefore = 0;
@@ -739,8 +845,16 @@ int refine_engine(Job &job, Prepared &pr)
           launches++;
           windowLaunches++;
         } else {
+          Geometry gl = g;
+          SynthBuffers bl = b;
+          if (learned.ok && k1 - k >= synth_wave_min()) {
+            bl.props = learned_propose(job, pr, learned, k, k1);
+            bl.propsK0 = k;
+            gl.maxProbe = job.opt.learnedProbes ? job.opt.learnedProbes : 32u;
+            job.st.learnedLaunches++;
+          }
           record_event(job, ev, true, k1 - k >= synth_wave_min());
-          HIP_TRY(launch_synth_batched(s, job.wide, g, b, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
+          HIP_TRY(launch_synth_batched(s, job.wide, gl, bl, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
           record_event(job, ev, false);
           launches++;
         }
@@ -847,9 +961,12 @@ bool remap_mask2(const ImageBuffer *image, const ImageBuffer *mask2, std::vector
   return true;
 }
 
+// `after` (may be null) runs under the library lock once the engine has succeeded, with the
+// run's device state still in the workspace.
 int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat format,
                      TImageSynthParameters *params, void (*cb)(int, void *), void *ctx, int *cancelFlag,
-                     const ResynthOptions *opt, ResynthStats *stats)
+                     const ResynthOptions *opt, ResynthStats *stats,
+                     const std::function<void(Job &, unsigned)> *after = nullptr)
 {
   // lib/imageSynth.c:62-137 / 140-217
   if (image->width != mask->width || image->height != mask->height) return IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
@@ -891,6 +1008,7 @@ int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, 
   }
   unsigned nT = 0;
   int err = npx ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
+  if (!err && after) (*after)(job, nT);
   if (!err && !(cancelFlag && *cancelFlag)) {
     HIP_TRY(launch_writeback_simple(s, job.wide, g_dev.ws.tword.p, (const int2 *)g_dev.ws.targets.p, nT, dImg,
                                     S.rowBytes, bpp, job.nColors, (int)w));
@@ -1437,6 +1555,41 @@ int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int devi
     force = &cfg;
   }
   return run_batch(requests, n, deviceBuffers != 0, options, force, stats);
+}
+
+int resynth_model_log_pairs(ResynthModel *model, ImageBuffer *image, ImageBuffer *mask, TImageFormat format,
+                            const TImageSynthParameters *params, unsigned int maxPairs, unsigned int seed)
+{
+  if (!model || !image || !mask) return -1;
+  ResynthOptions o;
+  resynth_default_options(&o);
+  o.mode = RESYNTH_MODE_BATCHED;
+  o.seed = seed;
+  TImageSynthParameters p;
+  if (params) p = *params;
+  else setDefaultParams(&p);
+  const std::function<void(Job &, unsigned)> log = [&](Job &job, unsigned nT) {
+    if (!maxPairs || !nT) return;
+    hipStream_t s = g_dev.stream;
+    Workspace &W = *job.W;
+    const int tw = job.src.tw, th = job.src.th;
+    const unsigned n = std::min(maxPairs, nT);
+    int2 *pts = (int2 *)W.pairPts.get((size_t)n * sizeof(int2) + 64);
+    int2 *src = (int2 *)W.pairSrc.get((size_t)n * sizeof(int2) + 64);
+    const uint64_t key = ((uint64_t)seed << 20) ^ 0x51ED27ull;
+    HIP_TRY(learn_sample_pairs(s, (const int2 *)W.targets.p, nT, (const int2 *)W.sourceOf.p, tw, (int)n, key, pts,
+                               src));
+    uint16_t *pq, *pk;
+    model_reserve_pairs(model, n, &pq, &pk);
+    HIP_TRY(learn_features(s, job.wide, W.tword.p, tw, th, p.isMakeSeamlesslyTileableHorizontally,
+                           p.isMakeSeamlesslyTileableVertically, job.nColors, (const int2 *)W.offsets.p, pts, (int)n,
+                           false, false, key + 1, pq));
+    HIP_TRY(learn_features(s, job.wide, W.cword.p, job.src.cw, job.src.ch, 0, 0, job.nColors,
+                           (const int2 *)W.offsets.p, src, (int)n, true, false, 0, pk));
+    HIP_TRY(hipStreamSynchronize(s));
+    model_commit_pairs(model, n);
+  };
+  return image_synth_host(image, mask, nullptr, format, &p, nullptr, nullptr, nullptr, &o, nullptr, &log);
 }
 
 unsigned int resynth_struct_sizes(size_t *out, unsigned int n)

@@ -67,6 +67,8 @@ struct SynthBuffers {
   const uint32_t *firstIdx; // [rhoMax+1] number of sorted offsets with x^2+y^2 < rho^2
   uint32_t *ctl;          // [WCTL_WORDS] sliding-window launch control
   const uint32_t *bandStart; // [kBands+1] index in corpusPts of each band's first point
+  const int2 *props;      // learned proposals of targets propsK0.. ([16] each, (-1,-1) = none)
+  unsigned propsK0;
 };
 
 // Control words of a sliding-window launch, each on its own 128-byte line.

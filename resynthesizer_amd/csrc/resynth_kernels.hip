@@ -666,6 +666,30 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   coherence_phase<Word>(g, cword, b.prober, k, count, sOff, sPix, sSrc, sT, sTm, best, winX, winY, perfect, scored,
                         record, cx, cy, evals);
   if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
+  // learned proposals: up to 16 corpus points, scored in rank order like probes
+  if (b.props && !perfect) {
+    int2 c = make_int2(-1, -1);
+    if (lane < 16) c = b.props[(size_t)(k - b.propsK0) * 16 + lane];
+    const bool valid = c.x >= 0;
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    if (valid) score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
+    int win = -1, stop = 64;
+    uint32_t bb = best;
+    resolve(valid && alive, M, F, bb, win, stop);
+    if (win >= 0) {
+      best = bb;
+      winX = __shfl(c.x, win);
+      winY = __shfl(c.y, win);
+      scored = true;
+    }
+    uint64_t used = __ballot(valid);
+    if (stop < 64) {
+      perfect = true;
+      used &= (stop == 63) ? ~0ull : ((2ull << stop) - 1ull);
+    }
+    evals += (unsigned)__popcll(used);
+  }
   // random probes from a per-target counter-based stream (seed, pass, target index,
   // attempt).  Dense corpora draw a uniform pixel of the corpus image and keep it when
   // its word marks a corpus point (0xFF: selected and opaque), which is uniform over

@@ -39,5 +39,48 @@ hipError_t mlp_infonce(hipStream_t s, const float *w, const uint16_t *wb, const 
                        float tau, LearnScratch &q, LearnScratch &k, float *grad, float *loss);
 hipError_t adam_step(hipStream_t s, float *w, const float *g, float *m, float *v, uint16_t *wb, float lr, int step);
 hipError_t params_to_bf16(hipStream_t s, const float *w, uint16_t *wb);
+hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst);
+hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uint64_t key);
+
+// ---- the model object (resynth_learn_host.cpp)
+}  // namespace rs
+struct ResynthModel;
+namespace rs {
+const float *model_weights(const ResynthModel *M);
+const uint16_t *model_weights_bf16(const ResynthModel *M);
+void model_reserve_pairs(ResynthModel *M, unsigned n, uint16_t **pq, uint16_t **pk);
+void model_commit_pairs(ResynthModel *M, unsigned n);
+hipStream_t model_stream(const ResynthModel *M);
+
+// ---- neighbourhood features, index and search (resynth_learn.hip)
+
+// Features of n points (x, y) of a pixel-word image (32- or 64-bit words, flag byte 0,
+// colours in bytes 1..nColors) into bf16 rows [n x kIn].  offsets: the engine's sorted
+// offsets (the first kNbr are used; offset 0 is the point itself and is never present).
+// isCorpus: a neighbour is present when it is selected in the corpus (flag 0xFE/0xFF);
+// otherwise (target) when its hasValue bit is set.  scale2: point coordinates are doubled
+// (half-resolution key grid).  dropKey != 0: each neighbour of a target point is dropped
+// with a per-point probability drawn from the hash of (dropKey, point).
+hipError_t learn_features(hipStream_t s, bool wide, const void *words, int w, int h, int tileH, int tileV, int nColors,
+                          const int2 *offsets, const int2 *pts, int n, bool isCorpus, bool scale2, uint64_t dropKey,
+                          uint16_t *out);
+// Flags of the half-resolution key grid: 1 where (2x, 2y) is a corpus point.
+hipError_t learn_key_flags(hipStream_t s, bool wide, const void *cword, int cw, int ch, uint8_t *flags);
+// Top-P (P = 1 or 2) columns by dot product for every row: rows [M x 64] and cols [N x 64]
+// bf16 (MFMA GEMM with the selection fused in), out [M x P] column indices.
+hipError_t learn_topk_cols(hipStream_t s, const uint16_t *rows, int M, const uint16_t *cols, int N, int P,
+                           int32_t *out);
+// Counting sort of n items by cluster id: start[C+1], members[n] (item indices).
+hipError_t learn_bucket(hipStream_t s, const int32_t *cluster, int n, int C, uint32_t *count, uint32_t *start,
+                        uint32_t *cursor, uint32_t *members);
+// Fine search: for each query, the top-16 keys (by dot product) among the members of its P
+// clusters; writes their corpus coordinates (key grid point * 2) or (-1, -1).
+// Pairs for training: pts[i] = targets[hash(key, i) % nT] and src[i] = sourceOf of that
+// target (row-major target image of width tw).
+hipError_t learn_sample_pairs(hipStream_t s, const int2 *targets, unsigned nT, const int2 *sourceOf, int tw, int n,
+                              uint64_t key, int2 *pts, int2 *src);
+hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qcl, int P,
+                            const uint16_t *keys, const uint32_t *start, const uint32_t *members, const int2 *keyPts,
+                            int2 *props);
 
 }  // namespace rs

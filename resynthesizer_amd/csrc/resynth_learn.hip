@@ -465,3 +465,354 @@ hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uin
 }
 
 }  // namespace rs
+
+namespace rs {
+
+// ------------------------------------------------------------------ features
+
+template <typename Word>
+__global__ __launch_bounds__(256) void k_features(const Word *words, int w, int h, int tileH, int tileV, int nColors,
+                                                  const int2 *offsets, const int2 *pts, int n, int isCorpus,
+                                                  int scale2, uint64_t dropKey, uint16_t *out)
+{
+  const int i = blockIdx.x * 8 + (threadIdx.x >> 5);
+  const int j = threadIdx.x & 31;
+  if (i >= n) return;
+  uint16_t v[4] = {0, 0, 0, 0};
+  if (j < kNbr && j > 0) {
+    int2 p = pts[i];
+    if (scale2) {
+      p.x *= 2;
+      p.y *= 2;
+    }
+    const int2 o = offsets[j];
+    int x = p.x + o.x, y = p.y + o.y;
+    bool in = true;
+    if (x < 0) { if (tileH) x += w; else in = false; }
+    else if (x >= w) { if (tileH) x -= w; else in = false; }
+    if (y < 0) { if (tileV) y += h; else in = false; }
+    else if (y >= h) { if (tileV) y -= h; else in = false; }
+    if (in && x >= 0 && x < w && y >= 0 && y < h) {
+      const Word wd = words[(size_t)y * (unsigned)w + (unsigned)x];
+      bool present = isCorpus ? ((wd & 0xFEu) == 0xFEu) : ((wd & 1u) != 0);
+      if (present && dropKey) {
+        // keep each neighbour with a per-point probability in [0.2, 1)
+        const uint64_t hp = lmix64(dropKey * 0x100000001B3ull + (uint64_t)i);
+        const float keep = 0.2f + 0.8f * (float)(hp >> 40) * (1.0f / 16777216.0f);
+        const uint64_t hj = lmix64(hp + (uint64_t)j * 0x9E3779B97F4A7C15ull);
+        present = (float)(hj >> 40) * (1.0f / 16777216.0f) < keep;
+      }
+      if (present) {
+        for (int c = 0; c < 3; c++)
+          if (c < nColors) v[c] = f2bf((float)((uint32_t)(wd >> (8 * (1 + c))) & 0xFFu) * (1.0f / 255.0f));
+        v[3] = 0x3F80;  // 1.0
+      }
+    }
+  }
+  // threads 30 and 31 write the 8 zero pads (inputs 120..127); thread 0 writes zeros too
+  uint2 packed = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+  *(uint2 *)(out + (size_t)i * kIn + j * 4) = packed;
+}
+
+hipError_t learn_features(hipStream_t s, bool wide, const void *words, int w, int h, int tileH, int tileV, int nColors,
+                          const int2 *offsets, const int2 *pts, int n, bool isCorpus, bool scale2, uint64_t dropKey,
+                          uint16_t *out)
+{
+  if (n <= 0) return hipSuccess;
+  dim3 grid((n + 7) / 8);
+  if (wide)
+    hipLaunchKernelGGL(k_features<uint64_t>, grid, dim3(256), 0, s, (const uint64_t *)words, w, h, tileH, tileV,
+                       nColors, offsets, pts, n, (int)isCorpus, (int)scale2, dropKey, out);
+  else
+    hipLaunchKernelGGL(k_features<uint32_t>, grid, dim3(256), 0, s, (const uint32_t *)words, w, h, tileH, tileV,
+                       nColors, offsets, pts, n, (int)isCorpus, (int)scale2, dropKey, out);
+  return hipGetLastError();
+}
+
+template <typename Word>
+__global__ void k_key_flags(const Word *cword, int cw, int ch, uint8_t *flags)
+{
+  const int w2 = (cw + 1) / 2, h2 = (ch + 1) / 2;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)w2 * h2) return;
+  const int x = (int)(i % w2) * 2, y = (int)(i / w2) * 2;
+  flags[i] = ((cword[(size_t)y * cw + x] & 0xFFu) == 0xFFu) ? 1 : 0;
+}
+
+hipError_t learn_key_flags(hipStream_t s, bool wide, const void *cword, int cw, int ch, uint8_t *flags)
+{
+  const size_t n = (size_t)((cw + 1) / 2) * ((ch + 1) / 2);
+  if (!n) return hipSuccess;
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (wide)
+    hipLaunchKernelGGL(k_key_flags<uint64_t>, grid, dim3(256), 0, s, (const uint64_t *)cword, cw, ch, flags);
+  else
+    hipLaunchKernelGGL(k_key_flags<uint32_t>, grid, dim3(256), 0, s, (const uint32_t *)cword, cw, ch, flags);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ GEMM with fused top-P
+
+template <int P>
+__device__ __forceinline__ void topk_insert(float (&v)[P], int (&ix)[P], float val, int idx)
+{
+  if (P == 1) {
+    if (val > v[0]) {
+      v[0] = val;
+      ix[0] = idx;
+    }
+  } else {
+    if (val > v[0]) {
+      v[1] = v[0];
+      ix[1] = ix[0];
+      v[0] = val;
+      ix[0] = idx;
+    } else if (val > v[1]) {
+      v[1] = val;
+      ix[1] = idx;
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_topk_cols(const uint16_t *rows, int M, const uint16_t *cols, int N,
+                                                   int32_t *out)
+{
+  __shared__ float sV[4][32][P];
+  __shared__ int sI[4][32][P];
+  const int m0 = blockIdx.x * GT;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  float bv[8][P];
+  int bi[8][P];
+  for (int q = 0; q < 8; q++)
+    for (int p = 0; p < P; p++) {
+      bv[q][p] = -3.4e38f;
+      bi[q][p] = -1;
+    }
+  for (int n0 = 0; n0 < N; n0 += GT) {
+    f32x4 acc[2][2];
+    gemm_tile<false, true>(M, N, kEmb, rows, kEmb, cols, kEmb, m0, n0, acc);
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        const int n = n0 + wn + b * 16 + (lane & 15);
+        if (n >= N) continue;
+        for (int r = 0; r < 4; r++) topk_insert<P>(bv[a * 4 + r], bi[a * 4 + r], acc[a][b][r], n);
+      }
+  }
+  // merge across the 16 lanes that share rows
+  for (int o = 1; o < 16; o <<= 1)
+    for (int q = 0; q < 8; q++) {
+      float ov[P];
+      int oi[P];
+      for (int p = 0; p < P; p++) {
+        ov[p] = __shfl_xor(bv[q][p], o);
+        oi[p] = __shfl_xor(bi[q][p], o);
+      }
+      for (int p = 0; p < P; p++)
+        if (oi[p] >= 0) topk_insert<P>(bv[q], bi[q], ov[p], oi[p]);
+    }
+  // remove duplicates a lane may have taken from its partner (insert keeps both copies
+  // of the same column); keep the list distinct
+  if (P == 2)
+    for (int q = 0; q < 8; q++)
+      if (bi[q][1] == bi[q][0]) {
+        bv[q][1] = -3.4e38f;
+        bi[q][1] = -1;
+      }
+  if ((lane & 15) == 0)
+    for (int a = 0; a < 2; a++)
+      for (int r = 0; r < 4; r++) {
+        const int rl = a * 16 + (lane >> 4) * 4 + r;
+        for (int p = 0; p < P; p++) {
+          sV[wave][rl][p] = bv[a * 4 + r][p];
+          sI[wave][rl][p] = bi[a * 4 + r][p];
+        }
+      }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int ml = threadIdx.x, w0 = (ml >= 32) ? 2 : 0, rl = ml & 31;
+    float v[P];
+    int ix[P];
+    for (int p = 0; p < P; p++) {
+      v[p] = sV[w0][rl][p];
+      ix[p] = sI[w0][rl][p];
+    }
+    for (int p = 0; p < P; p++)
+      if (sI[w0 + 1][rl][p] >= 0 && sI[w0 + 1][rl][p] != ix[0]) topk_insert<P>(v, ix, sV[w0 + 1][rl][p], sI[w0 + 1][rl][p]);
+    const int m = m0 + ml;
+    if (m < M)
+      for (int p = 0; p < P; p++) out[(size_t)m * P + p] = ix[p];
+  }
+}
+
+hipError_t learn_topk_cols(hipStream_t s, const uint16_t *rows, int M, const uint16_t *cols, int N, int P,
+                           int32_t *out)
+{
+  if (M <= 0 || N <= 0) return hipSuccess;
+  dim3 grid((M + GT - 1) / GT);
+  if (P == 1)
+    hipLaunchKernelGGL(k_topk_cols<1>, grid, dim3(256), 0, s, rows, M, cols, N, out);
+  else if (P == 2)
+    hipLaunchKernelGGL(k_topk_cols<2>, grid, dim3(256), 0, s, rows, M, cols, N, out);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ buckets
+
+__global__ void k_bucket_count(const int32_t *cluster, int n, uint32_t *count)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && cluster[i] >= 0) atomicAdd(&count[cluster[i]], 1u);
+}
+
+// Exclusive scan of C counts by one workgroup (each thread a contiguous run).
+__global__ __launch_bounds__(1024) void k_bucket_scan(const uint32_t *count, int C, uint32_t *start, uint32_t *cursor)
+{
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x, per = (C + 1023) / 1024;
+  const int b = t * per, e = min(C, b + per);
+  uint32_t s = 0;
+  for (int i = b; i < e; i++) s += count[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;
+  for (int i = b; i < e; i++) {
+    start[i] = run;
+    cursor[i] = 0;
+    run += count[i];
+  }
+  if (t == 1023) start[C] = part[1023];
+}
+
+__global__ void k_bucket_scatter(const int32_t *cluster, int n, const uint32_t *start, uint32_t *cursor,
+                                 uint32_t *members)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || cluster[i] < 0) return;
+  const int c = cluster[i];
+  members[start[c] + atomicAdd(&cursor[c], 1u)] = (uint32_t)i;
+}
+
+hipError_t learn_bucket(hipStream_t s, const int32_t *cluster, int n, int C, uint32_t *count, uint32_t *start,
+                        uint32_t *cursor, uint32_t *members)
+{
+  hipLaunchKernelGGL(k_zero, dim3((C + 255) / 256), dim3(256), 0, s, (float *)count, (size_t)C);
+  if (n > 0) hipLaunchKernelGGL(k_bucket_count, dim3((n + 255) / 256), dim3(256), 0, s, cluster, n, count);
+  hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, count, C, start, cursor);
+  if (n > 0)
+    hipLaunchKernelGGL(k_bucket_scatter, dim3((n + 255) / 256), dim3(256), 0, s, cluster, n, start, cursor, members);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ fine search
+
+// One wavefront per query: every member key of the query's P clusters is scored against
+// the query (64-dim bf16 dot products in fp32), each lane keeps its best two, and the
+// wave's top 16 are then taken one at a time.
+__global__ __launch_bounds__(256) void k_fine_top16(const uint16_t *qz, int nq, const int32_t *qcl, int P,
+                                                    const uint16_t *keys, const uint32_t *start,
+                                                    const uint32_t *members, const int2 *keyPts, int2 *props)
+{
+  __shared__ float sq[4][kEmb];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + wave;
+  if (q >= nq) return;
+  sq[wave][lane] = bf2f(qz[(size_t)q * kEmb + lane]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  float v0 = -3.4e38f, v1 = -3.4e38f;
+  int i0 = -1, i1 = -1;
+  for (int p = 0; p < P; p++) {
+    const int c = qcl[(size_t)q * P + p];
+    if (c < 0) continue;
+    const uint32_t b = start[c], e = start[c + 1];
+    for (uint32_t m = b + lane; m < e; m += 64) {
+      const uint32_t key = members[m];
+      const uint4 *kr = (const uint4 *)(keys + (size_t)key * kEmb);
+      float d = 0.f;
+      for (int g = 0; g < 8; g++) {
+        const uint4 u = kr[g];
+        const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
+        for (int t = 0; t < 4; t++) {
+          d += bf2f((uint16_t)(ws[t] & 0xFFFFu)) * sq[wave][g * 8 + t * 2];
+          d += bf2f((uint16_t)(ws[t] >> 16)) * sq[wave][g * 8 + t * 2 + 1];
+        }
+      }
+      if (d > v0) {
+        v1 = v0;
+        i1 = i0;
+        v0 = d;
+        i0 = (int)key;
+      } else if (d > v1) {
+        v1 = d;
+        i1 = (int)key;
+      }
+    }
+  }
+  int mine = -1;
+  for (int r = 0; r < 16; r++) {
+    // wave maximum of the lanes' current heads
+    float best = v0;
+    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+    if (best <= -3.0e38f) break;
+    const uint64_t hit = __ballot(v0 == best && i0 >= 0);
+    const int winner = (int)__builtin_ctzll(hit);
+    const int key = __shfl(i0, winner);
+    if (lane == r) mine = key;
+    if (lane == winner) {
+      v0 = v1;
+      i0 = i1;
+      v1 = -3.4e38f;
+      i1 = -1;
+    }
+  }
+  if (lane < 16) {
+    int2 o = make_int2(-1, -1);
+    if (mine >= 0) {
+      const int2 kp = keyPts[mine];
+      o = make_int2(kp.x * 2, kp.y * 2);
+    }
+    props[(size_t)q * 16 + lane] = o;
+  }
+}
+
+__global__ void k_sample_pairs(const int2 *targets, unsigned nT, const int2 *sourceOf, int tw, int n, uint64_t key,
+                               int2 *pts, int2 *src)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t h = lmix64(key * 0x100000001B3ull + (uint64_t)i);
+  const int2 t = targets[(unsigned)(((h >> 32) * (uint64_t)nT) >> 32)];
+  pts[i] = t;
+  src[i] = sourceOf[(size_t)t.y * (unsigned)tw + (unsigned)t.x];
+}
+
+hipError_t learn_sample_pairs(hipStream_t s, const int2 *targets, unsigned nT, const int2 *sourceOf, int tw, int n,
+                              uint64_t key, int2 *pts, int2 *src)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample_pairs, dim3((n + 255) / 256), dim3(256), 0, s, targets, nT, sourceOf, tw, n, key, pts,
+                     src);
+  return hipGetLastError();
+}
+
+hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qcl, int P,
+                            const uint16_t *keys, const uint32_t *start, const uint32_t *members, const int2 *keyPts,
+                            int2 *props)
+{
+  if (nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fine_top16, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, qcl, P, keys, start, members, keyPts,
+                     props);
+  return hipGetLastError();
+}
+
+}  // namespace rs

@@ -26,11 +26,6 @@ using namespace rs;
     }                                                                                                 \
   } while (0)
 
-namespace rs {
-hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst);
-hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uint64_t key);
-}  // namespace rs
-
 struct ResynthModel {
   hipStream_t s = nullptr;
   float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *loss = nullptr;
@@ -41,6 +36,9 @@ struct ResynthModel {
   LearnScratch q, k;
   uint16_t *bq = nullptr, *bk = nullptr;   // gathered batch inputs
   uint32_t *idx = nullptr;
+  // logged training pairs (bf16 rows of kIn), grow-only
+  uint16_t *pq = nullptr, *pk = nullptr;
+  unsigned nPairs = 0, capPairs = 0;
   std::vector<void *> owned;
   std::mutex mu;
 
@@ -138,6 +136,31 @@ int model_train_device(ResynthModel *M, const uint16_t *xq, const uint16_t *xk, 
   return 0;
 }
 
+// Room for n more pairs; returns where to write them (device).  The store grows by copying.
+void model_reserve_pairs(ResynthModel *M, unsigned n, uint16_t **pq, uint16_t **pk)
+{
+  if (M->nPairs + n > M->capPairs) {
+    const unsigned cap = std::max(M->nPairs + n, M->capPairs * 2);
+    uint16_t *q, *k;
+    LHIP(hipMalloc(&q, (size_t)cap * kIn * 2));
+    LHIP(hipMalloc(&k, (size_t)cap * kIn * 2));
+    if (M->nPairs) {
+      LHIP(hipMemcpyAsync(q, M->pq, (size_t)M->nPairs * kIn * 2, hipMemcpyDeviceToDevice, M->s));
+      LHIP(hipMemcpyAsync(k, M->pk, (size_t)M->nPairs * kIn * 2, hipMemcpyDeviceToDevice, M->s));
+    }
+    LHIP(hipStreamSynchronize(M->s));
+    if (M->pq) LHIP(hipFree(M->pq));
+    if (M->pk) LHIP(hipFree(M->pk));
+    M->pq = q;
+    M->pk = k;
+    M->capPairs = cap;
+  }
+  *pq = M->pq + (size_t)M->nPairs * kIn;
+  *pk = M->pk + (size_t)M->nPairs * kIn;
+}
+
+void model_commit_pairs(ResynthModel *M, unsigned n) { M->nPairs += n; }
+
 hipStream_t model_stream(const ResynthModel *M) { return M->s; }
 const float *model_weights(const ResynthModel *M) { return M->w; }
 const uint16_t *model_weights_bf16(const ResynthModel *M) { return M->wb; }
@@ -174,6 +197,8 @@ void resynth_model_free(ResynthModel *M)
 {
   if (!M) return;
   LHIP(hipStreamSynchronize(M->s));
+  if (M->pq) LHIP(hipFree(M->pq));
+  if (M->pk) LHIP(hipFree(M->pk));
   for (void *p : M->owned) LHIP(hipFree(p));
   LHIP(hipStreamDestroy(M->s));
   delete M;
@@ -262,6 +287,21 @@ int resynth_model_train(ResynthModel *M, const uint16_t *xq, const uint16_t *xk,
   LHIP(hipFree(dq));
   LHIP(hipFree(dk));
   return rc;
+}
+
+unsigned int resynth_model_pair_count(const ResynthModel *M) { return M ? M->nPairs : 0u; }
+
+void resynth_model_clear_pairs(ResynthModel *M)
+{
+  if (M) M->nPairs = 0;
+}
+
+int resynth_model_fit(ResynthModel *M, unsigned int steps, unsigned int batch, float lr, float tau, unsigned int seed,
+                      float *losses)
+{
+  if (!M || !M->nPairs) return -1;
+  std::lock_guard<std::mutex> lock(M->mu);
+  return model_train_device(M, M->pq, M->pk, M->nPairs, steps, batch, lr, tau, seed, losses);
 }
 
 int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, const uint16_t *B, float *C)

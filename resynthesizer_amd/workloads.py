@@ -160,3 +160,25 @@ def config3(n=96, base_seed=3000):
     """The batch of n uneven heal requests: lists of images, masks and patch sizes."""
     reqs = [config3_request(k, base_seed) for k in range(n)]
     return [r[0] for r in reqs], [r[1] for r in reqs], [r[2] for r in reqs]
+
+
+def config5(seed=5, size=4096, hole=None):
+    """Learned-proposal workload: size^2 RGB value noise + grating (config 2's generator,
+    3 channels) with a centred square hole of side size/4 (1024 at 4096).  Returns
+    (image, mask)."""
+    img = value_noise(size, size, 3, seed)
+    h = hole if hole is not None else size // 4
+    mask = np.zeros((size, size), np.uint8)
+    o = (size - h) // 2
+    mask[o:o + h, o:o + h] = 255
+    return img, mask
+
+
+def training_textures(count=200, size=512, base_seed=10000):
+    """The training set of the proposal network: config 2's generator (RGB) with a seeded
+    12-ellipse hole, one per seed base_seed .. base_seed+count-1.  Yields (image, mask)."""
+    for k in range(count):
+        img = value_noise(size, size, 3, base_seed + k)
+        s = size / 2048.0
+        m = ellipse_mask(size, size, 12, base_seed + 7919 * (k + 1), axes=(60 * s, 260 * s))
+        yield img, (m * 255).astype(np.uint8)
