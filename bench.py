@@ -101,6 +101,100 @@ def bench_config3(args, np, torch, R, W):
     print(json.dumps(out), flush=True)
 
 
+# Learned-mode settings of the config-5 line: proposals of every target computed once at
+# the start of pass 1 from the pass-0 result (1 inverted-file cluster searched, top 16),
+# 1 random probe beside them, 16 random probes in pass 0, at most 3 passes.
+LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=3)
+
+
+def bench_config5(args, np, torch, R, W):
+    """Learned proposals vs the plain method on a 4096x4096 RGB value-noise + grating texture
+    with a centred 1024x1024 hole (1 048 576 targets).  The network is trained first (pairs
+    logged by the plain engine on 200 seeded 512x512 textures of config 2's generator,
+    InfoNCE, Adam); training is reported, not timed.  Timed steps run the learned mode; the
+    plain method (200 probes) is timed the same way, and plain runs with more probes bound
+    the plain method's time at the learned error."""
+    t = time.time()
+    model = R.Model(seed=1)
+    for i, (img, mask) in enumerate(W.training_textures(200, 512)):
+        err, _ = model.log_pairs(img, mask, R.T_RGB, R.default_params(), max_pairs=4096, seed=i + 1)
+        assert err == 0, err
+    losses = model.fit(1500, batch=1024, lr=1e-3, tau=0.1, seed=3)
+    train_s = time.time() - t
+    img, mask = W.config5(size=args.size * 2 if args.size != 2048 else 4096)
+    h, w, c = img.shape
+    d_src = torch.from_numpy(img).cuda()
+    d_img = torch.empty_like(d_src)
+    d_mask = torch.from_numpy(mask).cuda()
+
+    def run(probes, learned, seed=0, timing=False):
+        d_img.copy_(d_src)
+        torch.cuda.synchronize()
+        kw = dict(LEARNED, model=model) if learned else {}
+        r = R.image_synth_device(d_img.data_ptr(), w, h, w * c, d_mask.data_ptr(), w, R.T_RGB,
+                                 R.default_params(maxProbeCount=probes), mode=R.MODE_BATCHED, seed=seed,
+                                 timing=timing, **kw)
+        assert r.err == 0, r.err
+        return r
+
+    def timed(probes, learned, steps, warmup):
+        for _ in range(warmup):
+            run(probes, learned)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        errs = []
+        for k in range(steps):
+            errs.append(run(probes, learned, seed=k + 1).meanError)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, float(np.mean(errs))
+
+    sec_l, err_l = timed(200, True, args.steps, args.warmup)
+    sec_p, err_p = timed(200, False, max(2, args.steps // 2), 1)
+    rl = run(200, True, timing=True)
+    rp = run(200, False, timing=True)
+    n_t = rl.nTargets
+    # the plain method with more probes, until its error reaches the learned error or the
+    # probe budget ends: its time there bounds the plain time at equal error from below
+    ladder = []
+    for probes in (1000, 4000):
+        sec, err = timed(probes, False, 1, 0)
+        ladder.append({"probes": probes, "ms": round(sec * 1e3, 1), "mean_match_error": round(err, 1)})
+        if err <= err_l:
+            break
+    reached = ladder[-1]["mean_match_error"] <= err_l
+    kern_ms = rl.waveKernelMs
+    alg = rl.wavePatchTerms * rl.pixelWordBytes
+    achieved = alg / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    out = {
+        "metric": METRIC, "value": round(n_t / sec_l, 1), "unit": "target px/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec_l * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32+bf16",
+        "data": "synthetic (seeded value noise + grating; training textures from seeds 10000..10199)",
+        "config": {"workload": "config 5: learned proposals, 4096x4096 RGB, 1024x1024 hole, %d target px" % n_t,
+                   "target_px": n_t, "learned": LEARNED,
+                   "learned_mean_match_error": round(err_l, 1),
+                   "plain200_px_per_s": round(n_t / sec_p, 1), "plain200_ms": round(sec_p * 1e3, 2),
+                   "plain200_mean_match_error": round(err_p, 1),
+                   "speedup_vs_plain200": round(sec_p / sec_l, 3),
+                   "error_ratio_learned_vs_plain200": round(err_l / err_p, 4),
+                   "plain_probe_ladder": ladder,
+                   "speedup_at_equal_error": (round(ladder[-1]["ms"] / (sec_l * 1e3), 2) if reached else None),
+                   "speedup_at_equal_error_lower_bound": round(ladder[-1]["ms"] / (sec_l * 1e3), 2),
+                   "train_s": round(train_s, 1), "train_pairs": int(model.pair_count()),
+                   "train_loss_first_last": [round(float(losses[:20].mean()), 3),
+                                             round(float(losses[-20:].mean()), 3)],
+                   "index_ms": round(rl.learnedIndexMs, 2), "keys": rl.learnedKeys,
+                   "clusters": rl.learnedClusters, "propose_ms": round(rl.learnedProposeMs, 2),
+                   "synth_kernels_ms": round(rl.synthKernelMs, 2), "passes": rl.passes,
+                   "plain_synth_kernels_ms": round(rp.synthKernelMs, 2)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_synth_wave",
+                     "launches": rl.waveLaunches,
+                     "avg_launch_ms": round(kern_ms / max(rl.waveLaunches, 1), 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,7 +204,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
-    ap.add_argument("--workload", default="config2", choices=["config2", "config3"],
+    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config5"],
                     help="config2: one 2048x2048 heal (the headline line); config3: 96 uneven heal "
                          "requests in one multi-request launch")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -136,6 +230,9 @@ def main():
     from resynthesizer_amd import workloads as W
     if args.workload == "config3":
         bench_config3(args, np, torch, R, W)
+        return
+    if args.workload == "config5":
+        bench_config5(args, np, torch, R, W)
         return
 
     img, mask = W.config2(size=args.size)

@@ -161,6 +161,17 @@ typedef struct {
                                  (NULL: the plain method) */
   unsigned int learnedProbes; /* random probes per visit that has proposals (0 = 32); visits
                                  of the first, small pass-0 chunks keep maxProbeCount */
+  unsigned int learnedPasses; /* passes whose visits get proposals (0 = all) */
+  unsigned int learnedClusters; /* inverted-file clusters searched per query: 1 or 2 (0 = 2) */
+  unsigned int learnedProbesLater; /* random probes per visit in passes without proposals when a
+                                      model is set (0 = maxProbeCount) */
+  unsigned int learnedRefresh;  /* 0: proposals for every chunk from its targets' current
+                                   neighbourhoods; else a bit mask of passes at whose start the
+                                   proposals of all targets are computed at once, then reused
+                                   by every later visit (passes before the first set bit have
+                                   none) */
+  unsigned int maxPasses;       /* batched mode: stop after this many passes even if the
+                                   refiner's rule would go on (0 = the rule alone) */
 } ResynthOptions;
 
 typedef struct {
@@ -186,6 +197,7 @@ typedef struct {
   unsigned int learnedKeys;           /* corpus keys embedded (0: no proposals) */
   unsigned int learnedClusters;       /* inverted-file clusters */
   double learnedIndexMs;              /* host wall clock of embedding the corpus and indexing */
+  double learnedProposeMs;            /* proposal kernels (features, embedding, search; timing != 0) */
 } ResynthStats;
 
 void resynth_set_default_mode(int mode);

@@ -59,7 +59,10 @@ class Map(ctypes.Structure):
 
 class ResynthOptions(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("prng", ctypes.c_int), ("seed", ctypes.c_uint), ("batch", ctypes.c_uint),
-                ("timing", ctypes.c_int), ("model", ctypes.c_void_p), ("learnedProbes", ctypes.c_uint)]
+                ("timing", ctypes.c_int), ("model", ctypes.c_void_p), ("learnedProbes", ctypes.c_uint),
+                ("learnedPasses", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
+                ("learnedProbesLater", ctypes.c_uint), ("learnedRefresh", ctypes.c_uint),
+                ("maxPasses", ctypes.c_uint)]
 
 
 class ResynthStats(ctypes.Structure):
@@ -73,7 +76,7 @@ class ResynthStats(ctypes.Structure):
                 ("waveLaunches", ctypes.c_uint), ("waveKernelMs", ctypes.c_double),
                 ("wavePatchTerms", ctypes.c_ulonglong), ("learnedLaunches", ctypes.c_uint),
                 ("learnedKeys", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
-                ("learnedIndexMs", ctypes.c_double)]
+                ("learnedIndexMs", ctypes.c_double), ("learnedProposeMs", ctypes.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -221,9 +224,11 @@ def params_from_dict(d):
     return default_params(**d)
 
 
-def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False, model=None, learned_probes=0):
+def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False, model=None, learned_probes=0,
+            learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0):
     return ResynthOptions(mode, prng, seed, batch, int(bool(timing)), model.h if model is not None else None,
-                          learned_probes)
+                          learned_probes, learned_passes, learned_clusters, learned_probes_later, learned_refresh,
+                          max_passes)
 
 
 def format_indices(fmt):
@@ -309,14 +314,16 @@ def engine(params, ix, target_pixmap, corpus_pixmap, mode=MODE_BATCHED, prng=PRN
 
 def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, fmt, params=None,
                        mode=MODE_BATCHED, prng=PRNG_GRAND, d_mask2=None, mask2_row_bytes=0, seed=0, batch=0,
-                       timing=False, cancel=0, model=None, learned_probes=0):
+                       timing=False, cancel=0, model=None, learned_probes=0, learned_passes=0,
+                       learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0):
     """imageSynth on device-resident buffers (integer device addresses, e.g. torch data_ptr()).
     Synthesizes d_image in place.  Returns a Result."""
     log = []
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
-    opt = options(mode, prng, seed, batch, timing, model, learned_probes)
+    opt = options(mode, prng, seed, batch, timing, model, learned_probes, learned_passes, learned_clusters,
+                  learned_probes_later, learned_refresh, max_passes)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().resynth_image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, d_mask2,
                                            mask2_row_bytes, fmt, pp, cb, None, ctypes.byref(cf), ctypes.byref(opt),

@@ -282,7 +282,7 @@ struct Workspace {
       bandStart;
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
   Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, centIdx, keyCl, bCount, bStart, bCursor,
-      bMembers, qX, qH1, qH2, qE, qZ, qCl, props, pairPts, pairSrc;
+      bMembers, qX, qH1, qH2, qE, qZ, qCl, props, propsAll, pairPts, pairSrc;
   PinnedBuf pinCounts, pinTables, pinPerm;
 };
 
@@ -661,7 +661,25 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
 }
 
 // Proposals (16 corpus points each) for targets k0..k1-1 from their current neighbourhoods.
-const int2 *learned_propose(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1)
+constexpr unsigned kQueryBlock = 65536;   // queries embedded and searched per block
+
+void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
+                           int2 *props);
+
+// Proposals for targets k0..k1-1, 16 each, written to out (or to the workspace when out is
+// null); returns where they are.
+const int2 *learned_propose(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
+                            int2 *out = nullptr)
+{
+  Workspace &W = *job.W;
+  if (!out) out = (int2 *)W.props.get((size_t)(k1 - k0) * 16 * sizeof(int2) + 64);
+  for (unsigned b = k0; b < k1; b += kQueryBlock)
+    learned_propose_block(job, pr, L, b, std::min(k1, b + kQueryBlock), P, out + (size_t)(b - k0) * 16);
+  return out;
+}
+
+void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
+                           int2 *props)
 {
   hipStream_t s = g_dev.stream;
   Workspace &W = *job.W;
@@ -674,13 +692,11 @@ const int2 *learned_propose(Job &job, const Prepared &pr, const LearnedIndex &L,
   t.e = (float *)W.qE.get((size_t)n * kEmb * 4 + 64);
   uint16_t *z = (uint16_t *)W.qZ.get((size_t)n * kEmb * 2 + 64);
   int32_t *qcl = (int32_t *)W.qCl.get((size_t)n * 2 * 4 + 64);
-  int2 *props = (int2 *)W.props.get((size_t)n * 16 * sizeof(int2) + 64);
   HIP_TRY(learn_features(s, job.wide, pr.b.tword, g.tw, g.th, g.tileH, g.tileV, g.nColors, pr.b.offsets,
                          pr.b.targets + k0, n, false, false, 0, x));
   HIP_TRY(mlp_embed(s, model_weights(job.opt.model), model_weights_bf16(job.opt.model), x, n, t, z, nullptr));
-  HIP_TRY(learn_topk_cols(s, z, n, L.cent, (int)L.C, 2, qcl));
-  HIP_TRY(learn_fine_top16(s, z, n, qcl, 2, L.keyZ, L.start, L.members, L.keyPts, props));
-  return props;
+  HIP_TRY(learn_topk_cols(s, z, n, L.cent, (int)L.C, P, qcl));
+  HIP_TRY(learn_fine_top16(s, z, n, qcl, P, L.keyZ, L.start, L.members, L.keyPts, props));
 }
 
 // The refiner (lib/refiner.h:43-122) over a prepared run: serial or batched passes with
@@ -767,6 +783,8 @@ int refine_engine(Job &job, Prepared &pr)
   }();
   unsigned windowLaunches = 0, windowExpired = 0;
   LearnedIndex learned;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> proposeEvents;
+  int2 *propsAll = nullptr;
   if (job.opt.model && !serial) {
     const double tl = now_ms();
     learned_build(job, pr, learned);
@@ -782,6 +800,21 @@ int refine_engine(Job &job, Prepared &pr)
     // every target has a value after pass 0; scans can skip empty rings only in pass 0,
     // and wrapped neighbourhoods (tiling) are not covered by the tile walk
     g.skipEmpty = (pass == 0 && !g.tileH && !g.tileV) ? 1 : 0;
+    if (learned.ok && (job.opt.learnedRefresh >> pass) & 1u) {
+      // proposals of every target from the state at the start of this pass
+      hipEvent_t pa = nullptr, pb = nullptr;
+      if (job.opt.timing) {
+        HIP_TRY(hipEventCreate(&pa));
+        HIP_TRY(hipEventCreate(&pb));
+        HIP_TRY(hipEventRecord(pa, s));
+      }
+      propsAll = (int2 *)W.propsAll.get((size_t)nT * 16 * sizeof(int2) + 64);
+      learned_propose(job, pr, learned, 0, nT, job.opt.learnedClusters == 1 ? 1 : 2, propsAll);
+      if (job.opt.timing) {
+        HIP_TRY(hipEventRecord(pb, s));
+        proposeEvents.push_back({pa, pb});
+      }
+    }
     // lib/synthesize.h:493-497: progress + cancel poll before target k when k & 4095 == 0
     unsigned k = 0;
     bool stop = false;
@@ -847,11 +880,31 @@ int refine_engine(Job &job, Prepared &pr)
         } else {
           Geometry gl = g;
           SynthBuffers bl = b;
-          if (learned.ok && k1 - k >= synth_wave_min()) {
-            bl.props = learned_propose(job, pr, learned, k, k1);
+          const bool learnPass = !job.opt.learnedPasses || pass < job.opt.learnedPasses;
+          if (learned.ok && job.opt.learnedRefresh && propsAll && k1 - k >= synth_wave_min()) {
+            bl.props = propsAll;
+            bl.propsK0 = 0;
+            gl.maxProbe = job.opt.learnedProbes ? job.opt.learnedProbes : 32u;
+            job.st.learnedLaunches++;
+          } else if (learned.ok && job.opt.learnedRefresh) {
+            if (job.opt.learnedProbesLater) gl.maxProbe = job.opt.learnedProbesLater;
+          } else if (learned.ok && learnPass && k1 - k >= synth_wave_min()) {
+            hipEvent_t pa = nullptr, pb = nullptr;
+            if (job.opt.timing) {
+              HIP_TRY(hipEventCreate(&pa));
+              HIP_TRY(hipEventCreate(&pb));
+              HIP_TRY(hipEventRecord(pa, s));
+            }
+            bl.props = learned_propose(job, pr, learned, k, k1, job.opt.learnedClusters == 1 ? 1 : 2);
+            if (job.opt.timing) {
+              HIP_TRY(hipEventRecord(pb, s));
+              proposeEvents.push_back({pa, pb});
+            }
             bl.propsK0 = k;
             gl.maxProbe = job.opt.learnedProbes ? job.opt.learnedProbes : 32u;
             job.st.learnedLaunches++;
+          } else if (learned.ok && !learnPass && job.opt.learnedProbesLater) {
+            gl.maxProbe = job.opt.learnedProbesLater;
           }
           record_event(job, ev, true, k1 - k >= synth_wave_min());
           HIP_TRY(launch_synth_batched(s, job.wide, gl, bl, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
@@ -877,6 +930,7 @@ int refine_engine(Job &job, Prepared &pr)
     job.st.visits += k;
     // lib/refiner.h:111-115
     if ((double)((float)betters / (float)nT) < 0.1) break;
+    if (!serial && job.opt.maxPasses && passesRun >= job.opt.maxPasses) break;
   }
   job.st.passes = passesRun;
   job.st.launches = launches;
@@ -917,6 +971,13 @@ int refine_engine(Job &job, Prepared &pr)
     }
     job.st.synthKernelMs = ms;
     job.st.waveKernelMs = wms;
+    for (auto &e : proposeEvents) {
+      float f = 0;
+      HIP_TRY(hipEventElapsedTime(&f, e.first, e.second));
+      job.st.learnedProposeMs += f;
+      HIP_TRY(hipEventDestroy(e.first));
+      HIP_TRY(hipEventDestroy(e.second));
+    }
   }
   job.st.synthMs = now_ms() - t1;
   return 0;
