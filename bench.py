@@ -1,18 +1,26 @@
 """Benchmark of the synthesis hot path on one MI355X.
 
-One step = one full engine run (imageSynth on device-resident buffers: adaptation,
-point lists, sorted offsets, target ordering, every refinement pass, write-back) over
-config 2 of the benchmark configurations: a 2048x2048 RGBA value-noise texture with a
-union of 12 seeded ellipses as the hole (531,632 target pixels), patchSize 30,
-maxProbeCount 200, sensitivity 0.117, random order, batched mode.
+One step = one full engine run (imageSynth on device-resident buffers: adaptation, point
+lists, sorted offsets, target ordering, every refinement pass, write-back).
 
-Prints one JSON line.  `value` is target pixels per second of the whole job with the
-inputs already in HBM.  `roofline` prices the dominant kernel (the batched visit kernel)
-by its algorithmic gather bytes: one corpus pixel word per patch neighbour per scored
-candidate.  `cpu_baseline` is the CPU oracle in the reference's threaded mode (8 threads,
-lib/refinerThreaded.h) on a bounded sample of the same workload.
+Default workload (config 5, learned proposals vs the plain method): a 4096x4096 RGB
+value-noise texture with a centred 1024x1024 hole (1 048 576 targets), patchSize 30,
+sensitivity 0.117, batched mode.  The proposal network is first trained on pairs the plain
+engine logs on 200 seeded textures (not timed); `value` is the learned mode's target px/s;
+the plain method (200 probes) is timed the same way, and plain runs with more probes give
+the plain method's time at the learned error (or a lower bound of it).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-quality]
+--workload config2: the plain method on a 2048x2048 RGBA value-noise texture with 12 seeded
+ellipses as the hole (531 632 targets), against the exact serial engine's error.
+--workload config3: 96 uneven heal requests in one multi-request launch.
+
+Prints one JSON line.  `value` is target pixels per second of the whole job with the inputs
+already in HBM.  `roofline` prices the dominant kernel (the wavefront-per-target visit
+kernel) by its algorithmic gather bytes: one corpus pixel word per patch neighbour per
+scored candidate.  `cpu_baseline` is the CPU oracle in the reference's threaded mode
+(8 threads, lib/refinerThreaded.h) on a bounded sample of the same workload.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--no-cpu] [--no-quality]
 """
 import argparse
 import json
@@ -121,7 +129,7 @@ def bench_config5(args, np, torch, R, W):
         assert err == 0, err
     losses = model.fit(1500, batch=1024, lr=1e-3, tau=0.1, seed=3)
     train_s = time.time() - t
-    img, mask = W.config5(size=args.size * 2 if args.size != 2048 else 4096)
+    img, mask = W.config5(size=args.size or 4096)
     h, w, c = img.shape
     d_src = torch.from_numpy(img).cuda()
     d_img = torch.empty_like(d_src)
@@ -192,7 +200,24 @@ def bench_config5(args, np, torch, R, W):
                      "launches": rl.waveLaunches,
                      "avg_launch_ms": round(kern_ms / max(rl.waveLaunches, 1), 4)},
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_config5()
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_config5(threads=8):
+    """Oracle in threaded-reference mode on config 5's generator at 1024x1024 with a centred
+    256x256 hole (the plain method, 200 probes): px/s of a whole engine run."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    from resynthesizer_amd import workloads as W
+    img, mask = W.config5(size=1024)
+    t = time.time()
+    r, _ = O.image_synth(img, mask, 0, O.default_params(), threads=threads)
+    dt = time.time() - t
+    return {"value": r.nTargets / dt, "unit": "target px/s", "cores": threads, "kind": "port",
+            "sample": "config 5 generator at 1024x1024 with a 256x256 hole (%d target px), one whole engine run "
+                      "(plain method), %.1f s, 8-thread reference mode of the CPU oracle" % (r.nTargets, dt)}
 
 
 def main():
@@ -200,12 +225,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--size", type=int, default=0, help="image side (default 2048 for config2, 4096 for config5)")
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
-    ap.add_argument("--workload", default="config2", choices=["config2", "config3", "config5"],
-                    help="config2: one 2048x2048 heal (the headline line); config3: 96 uneven heal "
+    ap.add_argument("--workload", default="config5", choices=["config2", "config3", "config5"],
+                    help="config5 (default): learned proposals vs the plain method on a 4096x4096 heal; "
+                         "config2: the plain method on a 2048x2048 heal; config3: 96 uneven heal "
                          "requests in one multi-request launch")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM counters)")
@@ -235,7 +261,7 @@ def main():
         bench_config5(args, np, torch, R, W)
         return
 
-    img, mask = W.config2(size=args.size)
+    img, mask = W.config2(size=args.size or 2048)
     h, w, c = img.shape
     d_src = torch.from_numpy(img).cuda()
     d_img = torch.empty_like(d_src)

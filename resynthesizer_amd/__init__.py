@@ -258,7 +258,8 @@ class Result:
 
 def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, mask2=None,
                 width=None, mask_width=None, mask2_width=None, seed=0, batch=0, timing=False, model=None,
-                learned_probes=0):
+                learned_probes=0, learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0,
+                max_passes=0):
     """imageSynth / imageSynth2 (lib/imageSynth.c:62-217) on the GPU.
 
     image: uint8 (H, W, C) or a padded row buffer (H, rowBytes) with `width` given;
@@ -285,7 +286,8 @@ def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAN
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
-    opt = options(mode, prng, seed, batch, timing, model, learned_probes)
+    opt = options(mode, prng, seed, batch, timing, model, learned_probes, learned_passes, learned_clusters,
+                  learned_probes_later, learned_refresh, max_passes)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().imageSynthEx(ctypes.byref(ib), ctypes.byref(mb), mb2p, fmt, pp, cb, None, ctypes.byref(cf),
                              ctypes.byref(opt), ctypes.byref(st))

@@ -282,7 +282,8 @@ struct Workspace {
       bandStart;
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
   Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, centIdx, keyCl, bCount, bStart, bCursor,
-      bMembers, qX, qH1, qH2, qE, qZ, qCl, props, propsAll, pairPts, pairSrc;
+      bMembers, qX, qH1, qH2, qE, qZ, qCl, props, propsAll, pairPts, pairSrc, superCent, superId, sCount, sStart,
+      sCursor, sMembers, subCent, tileStart, qSuper;
   PinnedBuf pinCounts, pinTables, pinPerm;
 };
 
@@ -599,12 +600,14 @@ constexpr int kKeyBlock = 65536;   // corpus keys embedded per block
 // by their nearest centroid.
 struct LearnedIndex {
   bool ok = false;
-  unsigned nK = 0, C = 0;
+  unsigned nK = 0, C = 0, S = 0, Q = 0;
   int2 *keyPts = nullptr;
-  uint16_t *keyZ = nullptr, *cent = nullptr;
+  uint16_t *keyZ = nullptr, *superCent = nullptr, *subCent = nullptr;
   uint32_t *start = nullptr, *members = nullptr;
 };
 
+// Two-level inverted file over the corpus keys: S super-centroids (every (nK/S)-th key),
+// Q = 128 sub-centroids inside each, about 240 keys per cluster.
 void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
 {
   hipStream_t s = g_dev.stream;
@@ -616,7 +619,7 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   uint8_t *flags = (uint8_t *)W.keyFlags.get(nGrid + 64);
   HIP_TRY(learn_key_flags(s, job.wide, pr.b.cword, g.cw, g.ch, flags));
   L.keyPts = (int2 *)W.keyPts.get(nGrid * sizeof(int2) + 64);
-  unsigned *dCount = (unsigned *)W.bCount.get(64 * 1024 * 4 + 64);
+  unsigned *dCount = (unsigned *)W.sCount.get(4096 + 64);
   const size_t tb = compact_temp_bytes(nGrid);
   HIP_TRY(launch_compact(s, flags, w2, h2, L.keyPts, dCount, W.keyTemp.get(tb + 64)));
   unsigned nK = 0;
@@ -625,43 +628,50 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   L.nK = nK;
   if (nK < 64) return;
   const float *w = model_weights(M);
-  const uint16_t *wb = model_weights_bf16(M);
+  const uint16_t *wt = model_weights_t(M, s);
   L.keyZ = (uint16_t *)W.keyZ.get((size_t)nK * kEmb * 2 + 64);
   uint16_t *x = (uint16_t *)W.keyX.get((size_t)kKeyBlock * kIn * 2);
-  LearnScratch t;
-  t.h1 = (uint16_t *)W.keyH1.get((size_t)kKeyBlock * kH1 * 2);
-  t.h2 = (uint16_t *)W.keyH2.get((size_t)kKeyBlock * kH2 * 2);
-  t.e = (float *)W.keyE.get((size_t)kKeyBlock * kEmb * 4);
   for (unsigned b0 = 0; b0 < nK; b0 += kKeyBlock) {
     const int nb = (int)std::min<unsigned>(kKeyBlock, nK - b0);
     HIP_TRY(learn_features(s, job.wide, pr.b.cword, g.cw, g.ch, 0, 0, g.nColors, pr.b.offsets, L.keyPts + b0, nb, true,
                            true, 0, x));
-    HIP_TRY(mlp_embed(s, w, wb, x, nb, t, L.keyZ + (size_t)b0 * kEmb, nullptr));
+    HIP_TRY(mlp_embed_fused(s, w, wt, x, nb, L.keyZ + (size_t)b0 * kEmb, nullptr));
   }
-  // about 240 keys per cluster, at most 16384 clusters
-  unsigned C = std::min(16384u, std::max(64u, nK / 240u));
-  C = (C + 63) / 64 * 64;
-  C = std::min(C, nK);
-  L.C = C;
-  std::vector<uint32_t> ci(C);
-  for (unsigned c = 0; c < C; c++) ci[c] = (uint32_t)((uint64_t)c * nK / C);
-  uint32_t *dci = (uint32_t *)W.centIdx.get((size_t)C * 4 + 64);
-  HIP_TRY(hipMemcpyAsync(dci, ci.data(), (size_t)C * 4, hipMemcpyHostToDevice, s));
-  L.cent = (uint16_t *)W.cent.get((size_t)C * kEmb * 2 + 64);
-  HIP_TRY(gather_rows_bf16(s, L.keyZ, kEmb, dci, (int)C, L.cent));
+  const unsigned Q = 128;
+  const unsigned S = std::max(1u, std::min(128u, nK / (Q * 240u)));
+  L.S = S;
+  L.Q = Q;
+  L.C = S * Q;
+  // super-centroids
+  std::vector<uint32_t> ci(S);
+  for (unsigned c = 0; c < S; c++) ci[c] = (uint32_t)((uint64_t)c * nK / S);
+  uint32_t *dci = (uint32_t *)W.centIdx.get((size_t)S * 4 + 64);
+  HIP_TRY(hipMemcpyAsync(dci, ci.data(), (size_t)S * 4, hipMemcpyHostToDevice, s));
+  L.superCent = (uint16_t *)W.superCent.get((size_t)S * kEmb * 2 + 64);
+  HIP_TRY(gather_rows_bf16(s, L.keyZ, kEmb, dci, (int)S, L.superCent));
+  int32_t *sid = (int32_t *)W.superId.get((size_t)nK * 4 + 64);
+  HIP_TRY(learn_topk_cols(s, L.keyZ, (int)nK, L.superCent, (int)S, 1, sid));
+  uint32_t *startS = (uint32_t *)W.sStart.get(((size_t)S + 65) * 4);
+  uint32_t *membersS = (uint32_t *)W.sMembers.get((size_t)nK * 4 + 64);
+  HIP_TRY(learn_bucket(s, sid, (int)nK, (int)S, dCount, startS, (uint32_t *)W.sCursor.get(((size_t)S + 64) * 4),
+                       membersS));
+  // sub-centroids and final clusters
+  L.subCent = (uint16_t *)W.subCent.get((size_t)L.C * kEmb * 2 + 64);
   int32_t *cl = (int32_t *)W.keyCl.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_topk_cols(s, L.keyZ, (int)nK, L.cent, (int)C, 1, cl));
-  uint32_t *cnt = (uint32_t *)W.bCount.get(((size_t)C + 64) * 4);
-  L.start = (uint32_t *)W.bStart.get(((size_t)C + 65) * 4);
-  uint32_t *cur = (uint32_t *)W.bCursor.get(((size_t)C + 64) * 4);
+  HIP_TRY(learn_sub_index(s, startS, membersS, (int)S, (int)Q, (int)nK, L.keyZ, L.subCent,
+                          (uint32_t *)W.tileStart.get(((size_t)S + 65) * 4), cl));
+  uint32_t *cnt = (uint32_t *)W.bCount.get(((size_t)L.C + 64) * 4);
+  L.start = (uint32_t *)W.bStart.get(((size_t)L.C + 65) * 4);
+  uint32_t *cur = (uint32_t *)W.bCursor.get(((size_t)L.C + 64) * 4);
   L.members = (uint32_t *)W.bMembers.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_bucket(s, cl, (int)nK, (int)C, cnt, L.start, cur, L.members));
+  HIP_TRY(learn_bucket(s, cl, (int)nK, (int)L.C, cnt, L.start, cur, L.members));
   HIP_TRY(hipStreamSynchronize(s));   // the model's weights are read on this stream
   L.ok = true;
 }
 
 // Proposals (16 corpus points each) for targets k0..k1-1 from their current neighbourhoods.
 constexpr unsigned kQueryBlock = 65536;   // queries embedded and searched per block
+constexpr unsigned kLearnMinChunk = 256;  // per-chunk proposals for chunks of at least this size
 
 void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
                            int2 *props);
@@ -686,16 +696,14 @@ void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, 
   const Geometry &g = pr.g;
   const int n = (int)(k1 - k0);
   uint16_t *x = (uint16_t *)W.qX.get((size_t)n * kIn * 2 + 64);
-  LearnScratch t;
-  t.h1 = (uint16_t *)W.qH1.get((size_t)n * kH1 * 2 + 64);
-  t.h2 = (uint16_t *)W.qH2.get((size_t)n * kH2 * 2 + 64);
-  t.e = (float *)W.qE.get((size_t)n * kEmb * 4 + 64);
   uint16_t *z = (uint16_t *)W.qZ.get((size_t)n * kEmb * 2 + 64);
   int32_t *qcl = (int32_t *)W.qCl.get((size_t)n * 2 * 4 + 64);
   HIP_TRY(learn_features(s, job.wide, pr.b.tword, g.tw, g.th, g.tileH, g.tileV, g.nColors, pr.b.offsets,
                          pr.b.targets + k0, n, false, false, 0, x));
-  HIP_TRY(mlp_embed(s, model_weights(job.opt.model), model_weights_bf16(job.opt.model), x, n, t, z, nullptr));
-  HIP_TRY(learn_topk_cols(s, z, n, L.cent, (int)L.C, P, qcl));
+  HIP_TRY(mlp_embed_fused(s, model_weights(job.opt.model), model_weights_t(job.opt.model, s), x, n, z, nullptr));
+  int32_t *qs = (int32_t *)W.qSuper.get((size_t)n * 4 + 64);
+  HIP_TRY(learn_topk_cols(s, z, n, L.superCent, (int)L.S, 1, qs));
+  HIP_TRY(learn_query_sub(s, z, n, qs, (int)L.Q, L.subCent, P, qcl));
   HIP_TRY(learn_fine_top16(s, z, n, qcl, P, L.keyZ, L.start, L.members, L.keyPts, props));
 }
 
@@ -881,14 +889,14 @@ int refine_engine(Job &job, Prepared &pr)
           Geometry gl = g;
           SynthBuffers bl = b;
           const bool learnPass = !job.opt.learnedPasses || pass < job.opt.learnedPasses;
-          if (learned.ok && job.opt.learnedRefresh && propsAll && k1 - k >= synth_wave_min()) {
+          if (learned.ok && job.opt.learnedRefresh && propsAll) {
             bl.props = propsAll;
             bl.propsK0 = 0;
             gl.maxProbe = job.opt.learnedProbes ? job.opt.learnedProbes : 32u;
             job.st.learnedLaunches++;
           } else if (learned.ok && job.opt.learnedRefresh) {
             if (job.opt.learnedProbesLater) gl.maxProbe = job.opt.learnedProbesLater;
-          } else if (learned.ok && learnPass && k1 - k >= synth_wave_min()) {
+          } else if (learned.ok && learnPass && k1 - k >= kLearnMinChunk) {
             hipEvent_t pa = nullptr, pb = nullptr;
             if (job.opt.timing) {
               HIP_TRY(hipEventCreate(&pa));
@@ -906,7 +914,7 @@ int refine_engine(Job &job, Prepared &pr)
           } else if (learned.ok && !learnPass && job.opt.learnedProbesLater) {
             gl.maxProbe = job.opt.learnedProbesLater;
           }
-          record_event(job, ev, true, k1 - k >= synth_wave_min());
+          record_event(job, ev, true, k1 - k >= synth_wave_min() || bl.props);
           HIP_TRY(launch_synth_batched(s, job.wide, gl, bl, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
           record_event(job, ev, false);
           launches++;

@@ -951,7 +951,8 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
 {
   constexpr int WAVES = 4;
   if (k1 <= k0) return hipSuccess;
-  if (k1 - k0 >= wave_min()) {
+  // learned proposals are scored by the wavefront-per-target kernel only
+  if (k1 - k0 >= wave_min() || b.props) {
     if (wide) launch_wave<uint64_t>(s, g, b, k0, k1, seed, pass);
     else launch_wave<uint32_t>(s, g, b, k0, k1, seed, pass);
     return hipGetLastError();

@@ -39,6 +39,12 @@ hipError_t mlp_infonce(hipStream_t s, const float *w, const uint16_t *wb, const 
                        float tau, LearnScratch &q, LearnScratch &k, float *grad, float *loss);
 hipError_t adam_step(hipStream_t s, float *w, const float *g, float *m, float *v, uint16_t *wb, float lr, int step);
 hipError_t params_to_bf16(hipStream_t s, const float *w, uint16_t *wb);
+// Transposed bf16 copies of the three weight matrices ([out x in] each, kParamCount-sized
+// scratch minus biases), for the fused embedding kernel.
+hipError_t transpose_weights(hipStream_t s, const float *w, uint16_t *wt);
+// Fused embedding of n rows (bf16 x [n x kIn]) into normalised z (bf16 and/or fp32).
+hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, const uint16_t *x, int n, uint16_t *zb,
+                           float *z);
 hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst);
 hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uint64_t key);
 
@@ -48,6 +54,7 @@ struct ResynthModel;
 namespace rs {
 const float *model_weights(const ResynthModel *M);
 const uint16_t *model_weights_bf16(const ResynthModel *M);
+const uint16_t *model_weights_t(const ResynthModel *M, hipStream_t s);
 void model_reserve_pairs(ResynthModel *M, unsigned n, uint16_t **pq, uint16_t **pk);
 void model_commit_pairs(ResynthModel *M, unsigned n);
 hipStream_t model_stream(const ResynthModel *M);
@@ -79,6 +86,13 @@ hipError_t learn_bucket(hipStream_t s, const int32_t *cluster, int n, int C, uin
 // target (row-major target image of width tw).
 hipError_t learn_sample_pairs(hipStream_t s, const int2 *targets, unsigned nT, const int2 *sourceOf, int tw, int n,
                               uint64_t key, int2 *pts, int2 *src);
+// Second level of the index: sub-centroids of every super-cluster (bucketed by startS /
+// membersS) and each key's cluster s*Q + q (tileStart: S+1 scratch words).
+hipError_t learn_sub_index(hipStream_t s, const uint32_t *startS, const uint32_t *membersS, int S, int Q, int nK,
+                           const uint16_t *keyZ, uint16_t *subCent, uint32_t *tileStart, int32_t *cluster);
+// Query routing inside the query's super-cluster: the best P sub-centroids as cluster ids.
+hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
+                           const uint16_t *subCent, int P, int32_t *qcl);
 hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qcl, int P,
                             const uint16_t *keys, const uint32_t *start, const uint32_t *members, const int2 *keyPts,
                             int2 *props);

@@ -351,6 +351,8 @@ hipError_t mlp_embed(hipStream_t s, const float *w, const uint16_t *wb, const ui
   return hipGetLastError();
 }
 
+__global__ void k_colsum_bf16_par(const uint16_t *g, int rows, int cols, float *out);
+
 // Backward of one tower given dE (fp32 [n x 64]); accumulates parameter gradients into
 // grad (which the caller zeroes).  x, h1, h2 are the forward's inputs and activations.
 static hipError_t mlp_backward(hipStream_t s, const uint16_t *wb, const uint16_t *x, const uint16_t *h1,
@@ -360,7 +362,8 @@ static hipError_t mlp_backward(hipStream_t s, const uint16_t *wb, const uint16_t
   // dW3 = h2^T dE, db3 = colsum dE
   if ((r = gemm<true, false, EPI_F32>(s, kH2, kEmb, n, h2, kH2, dEb, kEmb, t.gw, nullptr, kEmb, nullptr, 1.f))) return r;
   hipLaunchKernelGGL(k_axpy, dim3((kH2 * kEmb + 255) / 256), dim3(256), 0, s, t.gw, grad + OW3, (size_t)kH2 * kEmb);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(kEmb), 0, s, dEb, n, kEmb, t.gb);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, t.gb, (size_t)kEmb);
+  hipLaunchKernelGGL(k_colsum_bf16_par, dim3((kEmb + 63) / 64, (n + 255) / 256), dim3(256), 0, s, dEb, n, kEmb, t.gb);
   hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, s, t.gb, grad + OB3, (size_t)kEmb);
   // dH2 = dE W3^T -> through ReLU
   if ((r = gemm<false, true, EPI_F32>(s, n, kH2, kEmb, dEb, kEmb, wb + OW3, kEmb, t.g, nullptr, kH2, nullptr, 1.f)))
@@ -369,7 +372,8 @@ static hipError_t mlp_backward(hipStream_t s, const uint16_t *wb, const uint16_t
                      (size_t)n * kH2);
   if ((r = gemm<true, false, EPI_F32>(s, kH1, kH2, n, h1, kH1, t.gb16, kH2, t.gw, nullptr, kH2, nullptr, 1.f))) return r;
   hipLaunchKernelGGL(k_axpy, dim3((kH1 * kH2 + 255) / 256), dim3(256), 0, s, t.gw, grad + OW2, (size_t)kH1 * kH2);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(kH2), 0, s, t.gb16, n, kH2, t.gb);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, t.gb, (size_t)kH2);
+  hipLaunchKernelGGL(k_colsum_bf16_par, dim3((kH2 + 63) / 64, (n + 255) / 256), dim3(256), 0, s, t.gb16, n, kH2, t.gb);
   hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, s, t.gb, grad + OB2, (size_t)kH2);
   // dH1 = dH2pre W2^T -> through ReLU
   if ((r = gemm<false, true, EPI_F32>(s, n, kH1, kH2, t.gb16, kH2, wb + OW2, kH2, t.g, nullptr, kH1, nullptr, 1.f)))
@@ -378,7 +382,8 @@ static hipError_t mlp_backward(hipStream_t s, const uint16_t *wb, const uint16_t
                      (size_t)n * kH1);
   if ((r = gemm<true, false, EPI_F32>(s, kIn, kH1, n, x, kIn, t.gb16, kH1, t.gw, nullptr, kH1, nullptr, 1.f))) return r;
   hipLaunchKernelGGL(k_axpy, dim3((kIn * kH1 + 255) / 256), dim3(256), 0, s, t.gw, grad + OW1, (size_t)kIn * kH1);
-  hipLaunchKernelGGL(k_colsum_bf16, dim3(1), dim3(kH1), 0, s, t.gb16, n, kH1, t.gb);
+  hipLaunchKernelGGL(k_zero, dim3(1), dim3(256), 0, s, t.gb, (size_t)kH1);
+  hipLaunchKernelGGL(k_colsum_bf16_par, dim3((kH1 + 63) / 64, (n + 255) / 256), dim3(256), 0, s, t.gb16, n, kH1, t.gb);
   hipLaunchKernelGGL(k_axpy, dim3(1), dim3(256), 0, s, t.gb, grad + OB1, (size_t)kH1);
   return hipGetLastError();
 }
@@ -582,7 +587,7 @@ __global__ __launch_bounds__(256) void k_topk_cols(const uint16_t *rows, int M, 
   __shared__ int sI[4][32][P];
   const int m0 = blockIdx.x * GT;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wn = (wave & 1) * 32;   // waves 0,1 hold rows 0..31 of the tile, waves 2,3 rows 32..63
   float bv[8][P];
   int bi[8][P];
   for (int q = 0; q < 8; q++)
@@ -812,6 +817,356 @@ hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int
   if (nq <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fine_top16, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, qcl, P, keys, start, members, keyPts,
                      props);
+  return hipGetLastError();
+}
+
+}  // namespace rs
+
+namespace rs {
+
+// ------------------------------------------------------------------ two-level index
+//
+// S super-centroids (every (nK/S)-th key) split the keys; inside each super-cluster, Q
+// sub-centroids (evenly spaced members of its bucket) split them again, giving C = S*Q
+// clusters.  Assignment and query routing cost S + Q dot products per vector instead of C.
+
+// Block per super-cluster s: sub-centroid q copies the key at position q*size/Q of s's
+// bucket (key 0 when the bucket is empty).
+__global__ void k_pick_subcentroids(const uint32_t *startS, const uint32_t *membersS, int Q, const uint16_t *keyZ,
+                                    uint16_t *subCent)
+{
+  const int s = blockIdx.x;
+  const uint32_t b = startS[s], size = startS[s + 1] - b;
+  for (int q = threadIdx.x; q < Q; q += blockDim.x) {
+    const uint32_t key = size ? membersS[b + (uint32_t)(((uint64_t)q * size) / (uint64_t)Q)] : 0u;
+    const uint4 *src = (const uint4 *)(keyZ + (size_t)key * kEmb);
+    uint4 *dst = (uint4 *)(subCent + ((size_t)s * Q + q) * kEmb);
+    for (int g = 0; g < 8; g++) dst[g] = src[g];
+  }
+}
+
+// Tiles of 64 keys that do not straddle super-clusters: tileStart[s] = first tile of s.
+__global__ void k_super_tiles(const uint32_t *startS, int S, uint32_t *tileStart)
+{
+  if (threadIdx.x != 0) return;
+  uint32_t t = 0;
+  for (int s = 0; s < S; s++) {
+    tileStart[s] = t;
+    t += (startS[s + 1] - startS[s] + 63) / 64;
+  }
+  tileStart[S] = t;
+}
+
+// Block per tile of 64 keys of one super-cluster: scores against its Q sub-centroids on
+// MFMA (the key rows gathered through the bucket), keeps each key's best sub-centroid and
+// writes cluster = s*Q + q.
+__global__ __launch_bounds__(256) void k_sub_assign(const uint32_t *startS, const uint32_t *membersS,
+                                                    const uint32_t *tileStart, int S, int Q, const uint16_t *keyZ,
+                                                    const uint16_t *subCent, int32_t *cluster)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t As[GT][kEmb + GPAD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[GT][kEmb + GPAD];
+  __shared__ float sV[4][32];
+  __shared__ int sI[4][32];
+  __shared__ uint32_t sKey[GT];
+  const uint32_t t = blockIdx.x;
+  if (t >= tileStart[S]) return;
+  int lo = 0, hi = S;  // the super-cluster holding tile t
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (tileStart[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const int s = lo;
+  const uint32_t base = startS[s] + (t - tileStart[s]) * 64u, end = startS[s + 1];
+  const int rows = (int)min(64u, end - base);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  if (tid < GT) sKey[tid] = tid < rows ? membersS[base + tid] : 0u;
+  __syncthreads();
+  // A: 64 gathered key rows x 64 dims (two k-steps kept in LDS)
+  {
+    const int r = tid >> 2, c = (tid & 3) * 16;
+    uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0;
+    if (r < rows) {
+      const uint4 *src = (const uint4 *)(keyZ + (size_t)sKey[r] * kEmb + c);
+      v0 = src[0];
+      v1 = src[1];
+    }
+    *(uint4 *)&As[r][c] = v0;
+    *(uint4 *)&As[r][c + 8] = v1;
+  }
+  float bv[8];
+  int bi[8];
+  for (int q = 0; q < 8; q++) {
+    bv[q] = -3.4e38f;
+    bi[q] = -1;
+  }
+  const uint16_t *cent = subCent + (size_t)s * Q * kEmb;
+  for (int n0 = 0; n0 < Q; n0 += GT) {
+    __syncthreads();
+    {
+      const int r = tid >> 2, c = (tid & 3) * 16;
+      uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0;
+      if (n0 + r < Q) {
+        const uint4 *src = (const uint4 *)(cent + (size_t)(n0 + r) * kEmb + c);
+        v0 = src[0];
+        v1 = src[1];
+      }
+      *(uint4 *)&Bs[r][c] = v0;
+      *(uint4 *)&Bs[r][c + 8] = v1;
+    }
+    __syncthreads();
+    f32x4 acc[2][2];
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int rr = lane & 15, kq = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < kEmb; k0 += 32) {
+      bf16x8 fa[2], fb[2];
+      for (int a = 0; a < 2; a++) fa[a] = *(const bf16x8 *)&As[wm + a * 16 + rr][k0 + kq];
+      for (int b = 0; b < 2; b++) fb[b] = *(const bf16x8 *)&Bs[wn + b * 16 + rr][k0 + kq];
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+    }
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) {
+        const int n = n0 + wn + b * 16 + (lane & 15);
+        if (n >= Q) continue;
+        for (int r = 0; r < 4; r++)
+          if (acc[a][b][r] > bv[a * 4 + r]) {
+            bv[a * 4 + r] = acc[a][b][r];
+            bi[a * 4 + r] = n;
+          }
+      }
+  }
+  for (int o = 1; o < 16; o <<= 1)
+    for (int q = 0; q < 8; q++) {
+      const float ov = __shfl_xor(bv[q], o);
+      const int oi = __shfl_xor(bi[q], o);
+      if (ov > bv[q] || (ov == bv[q] && oi >= 0 && (bi[q] < 0 || oi < bi[q]))) {
+        bv[q] = ov;
+        bi[q] = oi;
+      }
+    }
+  if ((lane & 15) == 0)
+    for (int a = 0; a < 2; a++)
+      for (int r = 0; r < 4; r++) {
+        const int rl = a * 16 + (lane >> 4) * 4 + r;
+        sV[wave][rl] = bv[a * 4 + r];
+        sI[wave][rl] = bi[a * 4 + r];
+      }
+  __syncthreads();
+  if (tid < GT && tid < rows) {
+    const int w0 = (tid >= 32) ? 2 : 0, rl = tid & 31;
+    const int q = (sV[w0 + 1][rl] > sV[w0][rl]) ? sI[w0 + 1][rl] : sI[w0][rl];
+    cluster[sKey[tid]] = s * Q + (q < 0 ? 0 : q);
+  }
+}
+
+hipError_t learn_sub_index(hipStream_t s, const uint32_t *startS, const uint32_t *membersS, int S, int Q, int nK,
+                           const uint16_t *keyZ, uint16_t *subCent, uint32_t *tileStart, int32_t *cluster)
+{
+  hipLaunchKernelGGL(k_pick_subcentroids, dim3(S), dim3(256), 0, s, startS, membersS, Q, keyZ, subCent);
+  hipLaunchKernelGGL(k_super_tiles, dim3(1), dim3(64), 0, s, startS, S, tileStart);
+  const unsigned tiles = (unsigned)((nK + 63) / 64 + S);
+  hipLaunchKernelGGL(k_sub_assign, dim3(tiles), dim3(256), 0, s, startS, membersS, tileStart, S, Q, keyZ, subCent,
+                     cluster);
+  return hipGetLastError();
+}
+
+// Wavefront per query: its super-cluster's Q sub-centroids scored (lanes take q, q+64, ..),
+// then the best P become cluster ids s*Q + q.
+__global__ __launch_bounds__(256) void k_query_sub(const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
+                                                   const uint16_t *subCent, int P, int32_t *qcl)
+{
+  __shared__ float sq[4][kEmb];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + wave;
+  if (q >= nq) return;
+  sq[wave][lane] = bf2f(qz[(size_t)q * kEmb + lane]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  const int s = qsuper[q] < 0 ? 0 : qsuper[q];
+  const uint16_t *cent = subCent + (size_t)s * Q * kEmb;
+  float v0 = -3.4e38f, v1 = -3.4e38f;
+  int i0 = -1, i1 = -1;
+  for (int c = lane; c < Q; c += 64) {
+    const uint4 *kr = (const uint4 *)(cent + (size_t)c * kEmb);
+    float d = 0.f;
+    for (int g = 0; g < 8; g++) {
+      const uint4 u = kr[g];
+      const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
+      for (int t = 0; t < 4; t++) {
+        d += bf2f((uint16_t)(ws[t] & 0xFFFFu)) * sq[wave][g * 8 + t * 2];
+        d += bf2f((uint16_t)(ws[t] >> 16)) * sq[wave][g * 8 + t * 2 + 1];
+      }
+    }
+    if (d > v0) {
+      v1 = v0;
+      i1 = i0;
+      v0 = d;
+      i0 = c;
+    } else if (d > v1) {
+      v1 = d;
+      i1 = c;
+    }
+  }
+  for (int p = 0; p < P; p++) {
+    float best = v0;
+    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+    const uint64_t hit = __ballot(v0 == best && i0 >= 0);
+    int c = -1;
+    if (hit) {
+      const int winner = (int)__builtin_ctzll(hit);
+      c = __shfl(i0, winner);
+      if (lane == winner) {
+        v0 = v1;
+        i0 = i1;
+        v1 = -3.4e38f;
+        i1 = -1;
+      }
+    }
+    if (lane == 0) qcl[(size_t)q * P + p] = c < 0 ? -1 : s * Q + c;
+  }
+}
+
+hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
+                           const uint16_t *subCent, int P, int32_t *qcl)
+{
+  if (nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query_sub, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, qsuper, Q, subCent, P, qcl);
+  return hipGetLastError();
+}
+
+// Bias gradients: column sums of a bf16 [rows x cols] matrix into out (zeroed by the caller),
+// 64 columns x a chunk of rows per block.
+__global__ __launch_bounds__(256) void k_colsum_bf16_par(const uint16_t *g, int rows, int cols, float *out)
+{
+  __shared__ float part[4][64];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), rg = threadIdx.x >> 6;
+  const int r0 = blockIdx.y * 256, r1 = min(rows, r0 + 256);
+  float s = 0.f;
+  if (c < cols)
+    for (int r = r0 + rg; r < r1; r += 4) s += bf2f(g[(size_t)r * cols + c]);
+  part[rg][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (rg == 0 && c < cols) atomicAdd(&out[c], part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
+                                                  part[3][threadIdx.x]);
+}
+
+}  // namespace rs
+
+namespace rs {
+
+// ------------------------------------------------------------------ fused embedding
+//
+// k_mlp_embed: 64 rows per workgroup through all three layers and the normalisation, the
+// activations kept in LDS (two ping-pong buffers) and the weights read as transposed bf16
+// copies ([out x in], k-contiguous) so that every MFMA B operand is one 16-byte load.
+// Four waves split each layer's output columns.
+constexpr int kRowsE = 64, kLdsW = 264;   // rows per workgroup, LDS row stride (bf16)
+
+__global__ void k_transpose_w(const float *w, int in, int out, uint16_t *wt)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= in * out) return;
+  const int r = i / out, c = i % out;   // w[r][c] -> wt[c][r]
+  wt[(size_t)c * in + r] = f2bf(w[i]);
+}
+
+hipError_t transpose_weights(hipStream_t s, const float *w, uint16_t *wt)
+{
+  const size_t OW1 = 0, OB1 = OW1 + (size_t)kIn * kH1, OW2 = OB1 + kH1, OB2 = OW2 + (size_t)kH1 * kH2,
+               OW3 = OB2 + kH2;
+  hipLaunchKernelGGL(k_transpose_w, dim3((kIn * kH1 + 255) / 256), dim3(256), 0, s, w + OW1, kIn, kH1, wt);
+  hipLaunchKernelGGL(k_transpose_w, dim3((kH1 * kH2 + 255) / 256), dim3(256), 0, s, w + OW2, kH1, kH2,
+                     wt + (size_t)kIn * kH1);
+  hipLaunchKernelGGL(k_transpose_w, dim3((kH2 * kEmb + 255) / 256), dim3(256), 0, s, w + OW3, kH2, kEmb,
+                     wt + (size_t)kIn * kH1 + (size_t)kH1 * kH2);
+  return hipGetLastError();
+}
+
+// One layer: out[64 x N] = act(in[64 x K] W + b); this wave computes columns
+// [wave*N/4, (wave+1)*N/4).  CB = N/64 column blocks of 16 per wave.
+template <int K, int N, bool RELU>
+__device__ __forceinline__ void mlp_layer(const uint16_t (*in)[kLdsW], const uint16_t *wt, const float *bias,
+                                          uint16_t (*out)[kLdsW], float (*outF)[kEmb + 4])
+{
+  constexpr int CB = N / 64;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = wave * (N / 4);
+  const int rr = lane & 15, kq = (lane >> 4) * 8;
+  f32x4 acc[4][CB];
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < CB; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16x8 fb[CB];
+    for (int b = 0; b < CB; b++) fb[b] = *(const bf16x8 *)(wt + (size_t)(c0 + b * 16 + rr) * K + k0 + kq);
+    for (int a = 0; a < 4; a++) {
+      const bf16x8 fa = *(const bf16x8 *)&in[a * 16 + rr][k0 + kq];
+      for (int b = 0; b < CB; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[b], acc[a][b], 0, 0, 0);
+    }
+  }
+  for (int b = 0; b < CB; b++) {
+    const int col = c0 + b * 16 + (lane & 15);
+    const float bs = bias[col];
+    for (int a = 0; a < 4; a++)
+      for (int r = 0; r < 4; r++) {
+        const int row = a * 16 + (lane >> 4) * 4 + r;
+        float v = acc[a][b][r] + bs;
+        if (RELU) {
+          v = v > 0.f ? v : 0.f;
+          out[row][col] = f2bf(v);
+        } else {
+          outF[row][col] = v;
+        }
+      }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_embed(const uint16_t *x, int n, const uint16_t *wt, const float *w,
+                                                   uint16_t *zb, float *z)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t bufA[kRowsE][kLdsW];
+  __shared__ __attribute__((aligned(16))) uint16_t bufB[kRowsE][kLdsW];
+  const size_t OB1 = (size_t)kIn * kH1, OW2 = OB1 + kH1, OB2 = OW2 + (size_t)kH1 * kH2, OW3 = OB2 + kH2,
+               OB3 = OW3 + (size_t)kH2 * kEmb;
+  const uint16_t *wt1 = wt, *wt2 = wt + (size_t)kIn * kH1, *wt3 = wt2 + (size_t)kH1 * kH2;
+  const int r0 = blockIdx.x * kRowsE, tid = threadIdx.x;
+  // x rows -> bufA (128 bf16 each: 16 uint4)
+  for (int i = tid; i < kRowsE * 16; i += 256) {
+    const int r = i >> 4, c = (i & 15) * 8;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (r0 + r < n) v = *(const uint4 *)(x + (size_t)(r0 + r) * kIn + c);
+    *(uint4 *)&bufA[r][c] = v;
+  }
+  __syncthreads();
+  mlp_layer<kIn, kH1, true>(bufA, wt1, w + OB1, bufB, nullptr);
+  __syncthreads();
+  mlp_layer<kH1, kH2, true>(bufB, wt2, w + OB2, bufA, nullptr);
+  __syncthreads();
+  float (*eF)[kEmb + 4] = (float (*)[kEmb + 4])&bufB[0][0];   // 64 x 68 floats fit in bufB
+  mlp_layer<kH2, kEmb, false>(bufA, wt3, w + OB3, nullptr, eF);
+  __syncthreads();
+  // normalise: wave w takes rows w*16 .. w*16+15, lane = column
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int r = wave * 16; r < wave * 16 + 16; r++) {
+    if (r0 + r >= n) break;
+    const float v = eF[r][lane];
+    float s2 = v * v;
+    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
+    const float zz = v / sqrtf(fmaxf(s2, 1e-12f));
+    if (zb) zb[(size_t)(r0 + r) * kEmb + lane] = f2bf(zz);
+    if (z) z[(size_t)(r0 + r) * kEmb + lane] = zz;
+  }
+}
+
+hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, const uint16_t *x, int n, uint16_t *zb,
+                           float *z)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_embed, dim3((n + kRowsE - 1) / kRowsE), dim3(256), 0, s, x, n, wt, w, zb, z);
   return hipGetLastError();
 }
 

@@ -30,7 +30,9 @@ struct ResynthModel {
   hipStream_t s = nullptr;
   float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *loss = nullptr;
   uint16_t *wb = nullptr;
+  uint16_t *wt = nullptr;   // transposed bf16 weights for the fused embedding kernel
   int step = 0;
+  int version = 0, wtVersion = -1;   // parameter version and the version wt was made from
   // training scratch for batches of up to cap rows
   int cap = 0;
   LearnScratch q, k;
@@ -124,6 +126,7 @@ int model_train_device(ResynthModel *M, const uint16_t *xq, const uint16_t *xk, 
     LHIP(gather_rows_bf16(s, xk, kIn, M->idx, (int)batch, M->bk));
     LHIP(mlp_infonce(s, M->w, M->wb, M->bq, M->bk, (int)batch, tau, M->q, M->k, M->grad, M->loss + t % 1024));
     M->step++;
+    M->version++;
     LHIP(adam_step(s, M->w, M->grad, M->m, M->v, M->wb, lr, M->step));
     if (losses && (t % 1024 == 1023 || t + 1 == steps)) {
       const unsigned base = t - t % 1024, cnt = t - base + 1;
@@ -165,6 +168,18 @@ hipStream_t model_stream(const ResynthModel *M) { return M->s; }
 const float *model_weights(const ResynthModel *M) { return M->w; }
 const uint16_t *model_weights_bf16(const ResynthModel *M) { return M->wb; }
 
+// Transposed weights, refreshed on stream s when the parameters changed since last time.
+const uint16_t *model_weights_t(const ResynthModel *Mc, hipStream_t s)
+{
+  ResynthModel *M = const_cast<ResynthModel *>(Mc);
+  if (M->wtVersion != M->version) {
+    LHIP(hipStreamSynchronize(M->s));
+    LHIP(transpose_weights(s, M->w, M->wt));
+    M->wtVersion = M->version;
+  }
+  return M->wt;
+}
+
 }  // namespace rs
 
 extern "C" {
@@ -183,6 +198,7 @@ ResynthModel *resynth_model_create(unsigned int seed)
   M->grad = (float *)M->alloc(kParamCount * 4);
   M->loss = (float *)M->alloc(1024 * 4);
   M->wb = (uint16_t *)M->alloc(kParamCount * 2);
+  M->wt = (uint16_t *)M->alloc(kParamCount * 2);
   std::vector<float> p;
   model_init_params(seed, p);
   LHIP(hipMemcpyAsync(M->w, p.data(), kParamCount * 4, hipMemcpyHostToDevice, M->s));
@@ -221,6 +237,7 @@ int resynth_model_set_params(ResynthModel *M, const float *in)
   LHIP(hipMemsetAsync(M->m, 0, kParamCount * 4, M->s));
   LHIP(hipMemsetAsync(M->v, 0, kParamCount * 4, M->s));
   M->step = 0;
+  M->version++;
   LHIP(hipStreamSynchronize(M->s));
   return 0;
 }
@@ -234,7 +251,6 @@ int resynth_model_embed(const ResynthModel *Mc, const uint16_t *x, unsigned int 
   hipStream_t s = M->s;
   uint16_t *dx;
   float *dz;
-  LearnScratch t;
   std::vector<void *> tmp;
   auto A = [&](size_t b) {
     void *p;
@@ -244,11 +260,9 @@ int resynth_model_embed(const ResynthModel *Mc, const uint16_t *x, unsigned int 
   };
   dx = (uint16_t *)A((size_t)n * kIn * 2);
   dz = (float *)A((size_t)n * kEmb * 4);
-  t.h1 = (uint16_t *)A((size_t)n * kH1 * 2);
-  t.h2 = (uint16_t *)A((size_t)n * kH2 * 2);
-  t.e = (float *)A((size_t)n * kEmb * 4);
   LHIP(hipMemcpyAsync(dx, x, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
-  LHIP(mlp_embed(s, M->w, M->wb, dx, (int)n, t, nullptr, dz));
+  // the fused kernel the engine embeds with
+  LHIP(mlp_embed_fused(s, M->w, model_weights_t(M, s), dx, (int)n, nullptr, dz));
   LHIP(hipMemcpyAsync(out, dz, (size_t)n * kEmb * 4, hipMemcpyDeviceToHost, s));
   LHIP(hipStreamSynchronize(s));
   for (void *p : tmp) LHIP(hipFree(p));
