@@ -283,7 +283,8 @@ struct Workspace {
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
   Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, centIdx, keyCl, bCount, bStart, bCursor,
       bMembers, qX, qH1, qH2, qE, qZ, qCl, props, propsAll, pairPts, pairSrc, superCent, superId, sCount, sStart,
-      sCursor, sMembers, subCent, tileStart, qSuper;
+      sCursor, sMembers, subCent, tileStart, qSuper, sortK0, sortV0, sortK1, sortV1, sortTemp, keyZs, keyPtsS, qK0,
+      qV0, qK1, qV1, qSortTemp;
   PinnedBuf pinCounts, pinTables, pinPerm;
 };
 
@@ -601,10 +602,18 @@ constexpr int kKeyBlock = 65536;   // corpus keys embedded per block
 struct LearnedIndex {
   bool ok = false;
   unsigned nK = 0, C = 0, S = 0, Q = 0;
-  int2 *keyPts = nullptr;
-  uint16_t *keyZ = nullptr, *superCent = nullptr, *subCent = nullptr;
-  uint32_t *start = nullptr, *members = nullptr;
+  int2 *keyPts = nullptr, *keyPtsS = nullptr;      // keyPtsS: in cluster order
+  uint16_t *keyZ = nullptr, *keyZs = nullptr;      // keyZs: embeddings in cluster order
+  uint16_t *superCent = nullptr, *subCent = nullptr;
+  uint32_t *start = nullptr;                       // [C+1] cluster c = keyZs rows start[c]..start[c+1]
 };
+
+unsigned bits_for(unsigned v)
+{
+  unsigned b = 1;
+  while (b < 32 && (v >> b)) b++;
+  return b;
+}
 
 // Two-level inverted file over the corpus keys: S super-centroids (every (nK/S)-th key),
 // Q = 128 sub-centroids inside each, about 240 keys per cluster.
@@ -651,20 +660,27 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   HIP_TRY(gather_rows_bf16(s, L.keyZ, kEmb, dci, (int)S, L.superCent));
   int32_t *sid = (int32_t *)W.superId.get((size_t)nK * 4 + 64);
   HIP_TRY(learn_topk_cols(s, L.keyZ, (int)nK, L.superCent, (int)S, 1, sid));
+  // keys sorted by super-cluster, then by cluster (radix sorts: no atomics on hot counters)
+  uint32_t *k0 = (uint32_t *)W.sortK0.get((size_t)nK * 4 + 64), *v0 = (uint32_t *)W.sortV0.get((size_t)nK * 4 + 64);
+  uint32_t *k1 = (uint32_t *)W.sortK1.get((size_t)nK * 4 + 64), *v1 = (uint32_t *)W.sortV1.get((size_t)nK * 4 + 64);
+  const size_t stb = sort_pairs_temp_bytes(nK);
+  void *stemp = W.sortTemp.get(stb);
+  HIP_TRY(learn_sort_keys(s, sid, 1, (int)nK, k0, v0));
+  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, nK, bits_for(S), stemp, stb));
   uint32_t *startS = (uint32_t *)W.sStart.get(((size_t)S + 65) * 4);
-  uint32_t *membersS = (uint32_t *)W.sMembers.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_bucket(s, sid, (int)nK, (int)S, dCount, startS, (uint32_t *)W.sCursor.get(((size_t)S + 64) * 4),
-                       membersS));
+  HIP_TRY(learn_bucket_starts(s, k1, (int)nK, (int)S, startS));
   // sub-centroids and final clusters
   L.subCent = (uint16_t *)W.subCent.get((size_t)L.C * kEmb * 2 + 64);
   int32_t *cl = (int32_t *)W.keyCl.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_sub_index(s, startS, membersS, (int)S, (int)Q, (int)nK, L.keyZ, L.subCent,
+  HIP_TRY(learn_sub_index(s, startS, v1, (int)S, (int)Q, (int)nK, L.keyZ, L.subCent,
                           (uint32_t *)W.tileStart.get(((size_t)S + 65) * 4), cl));
-  uint32_t *cnt = (uint32_t *)W.bCount.get(((size_t)L.C + 64) * 4);
+  HIP_TRY(learn_sort_keys(s, cl, 1, (int)nK, k0, v0));
+  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, nK, bits_for(L.C), stemp, stb));
   L.start = (uint32_t *)W.bStart.get(((size_t)L.C + 65) * 4);
-  uint32_t *cur = (uint32_t *)W.bCursor.get(((size_t)L.C + 64) * 4);
-  L.members = (uint32_t *)W.bMembers.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_bucket(s, cl, (int)nK, (int)L.C, cnt, L.start, cur, L.members));
+  HIP_TRY(learn_bucket_starts(s, k1, (int)nK, (int)L.C, L.start));
+  L.keyZs = (uint16_t *)W.keyZs.get((size_t)nK * kEmb * 2 + 64);
+  L.keyPtsS = (int2 *)W.keyPtsS.get((size_t)nK * sizeof(int2) + 64);
+  HIP_TRY(learn_permute_keys(s, v1, (int)nK, L.keyZ, L.keyPts, L.keyZs, L.keyPtsS));
   HIP_TRY(hipStreamSynchronize(s));   // the model's weights are read on this stream
   L.ok = true;
 }
@@ -704,7 +720,13 @@ void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, 
   int32_t *qs = (int32_t *)W.qSuper.get((size_t)n * 4 + 64);
   HIP_TRY(learn_topk_cols(s, z, n, L.superCent, (int)L.S, 1, qs));
   HIP_TRY(learn_query_sub(s, z, n, qs, (int)L.Q, L.subCent, P, qcl));
-  HIP_TRY(learn_fine_top16(s, z, n, qcl, P, L.keyZ, L.start, L.members, L.keyPts, props));
+  // visit the queries in cluster order so that each cluster's keys are read from L2
+  uint32_t *qk0 = (uint32_t *)W.qK0.get((size_t)n * 4 + 64), *qv0 = (uint32_t *)W.qV0.get((size_t)n * 4 + 64);
+  uint32_t *qk1 = (uint32_t *)W.qK1.get((size_t)n * 4 + 64), *qv1 = (uint32_t *)W.qV1.get((size_t)n * 4 + 64);
+  const size_t qtb = sort_pairs_temp_bytes(n);
+  HIP_TRY(learn_sort_keys(s, qcl, P, n, qk0, qv0));
+  HIP_TRY(launch_sort_pairs(s, qk0, qk1, qv0, qv1, (size_t)n, 32, W.qSortTemp.get(qtb), qtb));
+  HIP_TRY(learn_fine_top16_sorted(s, z, n, qv1, qcl, P, L.keyZs, L.start, L.keyPtsS, props));
 }
 
 // The refiner (lib/refiner.h:43-122) over a prepared run: serial or batched passes with

@@ -126,6 +126,9 @@ size_t compact_temp_bytes(size_t n);
 hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int2 *out, unsigned *count,
                           void *temp);
 size_t offsets_temp_bytes(size_t nO);
+size_t sort_pairs_temp_bytes(size_t n);
+hipError_t launch_sort_pairs(hipStream_t s, const uint32_t *kIn, uint32_t *kOut, const uint32_t *vIn, uint32_t *vOut,
+                             size_t n, unsigned bits, void *temp, size_t tempBytes);
 hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 hipError_t launch_band_starts(hipStream_t s, const int2 *corpusPts, const unsigned *nC, int ch, uint32_t *out);

@@ -1536,6 +1536,23 @@ hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *t
   return hipGetLastError();
 }
 
+// Radix sort of (key, value) u32 pairs on the low `bits` bits of the keys.
+size_t sort_pairs_temp_bytes(size_t n)
+{
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs<rocprim::default_config>(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                           (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
+  return bytes + 256;
+}
+
+hipError_t launch_sort_pairs(hipStream_t s, const uint32_t *kIn, uint32_t *kOut, const uint32_t *vIn, uint32_t *vOut,
+                             size_t n, unsigned bits, void *temp, size_t tempBytes)
+{
+  if (!n) return hipSuccess;
+  return rocprim::radix_sort_pairs<rocprim::default_config>(temp, tempBytes, const_cast<uint32_t *>(kIn), kOut,
+                                                            const_cast<uint32_t *>(vIn), vOut, n, 0, bits, s);
+}
+
 // ------------------------------------------------------------------ small kernels
 
 // First index in the row-major corpus point list of every band's first row.

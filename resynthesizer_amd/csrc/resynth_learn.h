@@ -93,6 +93,14 @@ hipError_t learn_sub_index(hipStream_t s, const uint32_t *startS, const uint32_t
 // Query routing inside the query's super-cluster: the best P sub-centroids as cluster ids.
 hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
                            const uint16_t *subCent, int P, int32_t *qcl);
+// Sort keys from cluster ids (every stride-th entry) with values 0..n-1.
+hipError_t learn_sort_keys(hipStream_t s, const int32_t *cluster, int stride, int n, uint32_t *keys, uint32_t *vals);
+hipError_t learn_bucket_starts(hipStream_t s, const uint32_t *sorted, int n, int C, uint32_t *start);
+hipError_t learn_permute_keys(hipStream_t s, const uint32_t *members, int n, const uint16_t *keyZ, const int2 *keyPts,
+                              uint16_t *keyZs, int2 *keyPtsS);
+hipError_t learn_fine_top16_sorted(hipStream_t s, const uint16_t *qz, int nq, const uint32_t *order,
+                                   const int32_t *qcl, int P, const uint16_t *keyZs, const uint32_t *start,
+                                   const int2 *keyPtsS, int2 *props);
 hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qcl, int P,
                             const uint16_t *keys, const uint32_t *start, const uint32_t *members, const int2 *keyPts,
                             int2 *props);

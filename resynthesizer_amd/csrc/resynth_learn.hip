@@ -1171,3 +1171,140 @@ hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, co
 }
 
 }  // namespace rs
+
+namespace rs {
+
+// ------------------------------------------------------------------ sorted buckets
+
+// keys[i] = (u32) cluster[i * stride] (negative ids sort last), vals[i] = i.
+__global__ void k_sort_keys(const int32_t *cluster, int stride, int n, uint32_t *keys, uint32_t *vals)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = (uint32_t)cluster[(size_t)i * stride];
+  vals[i] = (uint32_t)i;
+}
+
+hipError_t learn_sort_keys(hipStream_t s, const int32_t *cluster, int stride, int n, uint32_t *keys, uint32_t *vals)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_keys, dim3((n + 255) / 256), dim3(256), 0, s, cluster, stride, n, keys, vals);
+  return hipGetLastError();
+}
+
+// start[c] = first position of cluster c in the sorted ids (n for c = C).
+__global__ void k_bucket_starts(const uint32_t *sorted, int n, int C, uint32_t *start)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > C) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (sorted[mid] < (uint32_t)c) lo = mid + 1;
+    else hi = mid;
+  }
+  start[c] = (uint32_t)lo;
+}
+
+hipError_t learn_bucket_starts(hipStream_t s, const uint32_t *sorted, int n, int C, uint32_t *start)
+{
+  hipLaunchKernelGGL(k_bucket_starts, dim3((C + 256) / 256), dim3(256), 0, s, sorted, n, C, start);
+  return hipGetLastError();
+}
+
+// Keys and their corpus points copied into cluster order.
+__global__ void k_permute_keys(const uint32_t *members, int n, const uint16_t *keyZ, const int2 *keyPts,
+                               uint16_t *keyZs, int2 *keyPtsS)
+{
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * 8) return;
+  const int i = (int)(t >> 3), g = (int)(t & 7);
+  const uint32_t m = members[i];
+  ((uint4 *)(keyZs + (size_t)i * kEmb))[g] = ((const uint4 *)(keyZ + (size_t)m * kEmb))[g];
+  if (g == 0) keyPtsS[i] = keyPts[m];
+}
+
+hipError_t learn_permute_keys(hipStream_t s, const uint32_t *members, int n, const uint16_t *keyZ, const int2 *keyPts,
+                              uint16_t *keyZs, int2 *keyPtsS)
+{
+  if (n <= 0) return hipSuccess;
+  const size_t work = (size_t)n * 8;
+  hipLaunchKernelGGL(k_permute_keys, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, members, n, keyZ, keyPts,
+                     keyZs, keyPtsS);
+  return hipGetLastError();
+}
+
+// Fine search over clusters stored contiguously (keyZs, keyPtsS in cluster order); queries
+// visited in `order` (sorted by cluster, so that a cluster's keys are read by consecutive
+// waves and stay in L2).  Same selection as k_fine_top16.
+__global__ __launch_bounds__(256) void k_fine_top16_sorted(const uint16_t *qz, int nq, const uint32_t *order,
+                                                           const int32_t *qcl, int P, const uint16_t *keyZs,
+                                                           const uint32_t *start, const int2 *keyPtsS, int2 *props)
+{
+  __shared__ float sq[4][kEmb];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + wave;
+  if (w >= nq) return;
+  const int q = (int)order[w];
+  sq[wave][lane] = bf2f(qz[(size_t)q * kEmb + lane]);
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_wave_barrier();
+  float v0 = -3.4e38f, v1 = -3.4e38f;
+  int i0 = -1, i1 = -1;
+  for (int p = 0; p < P; p++) {
+    const int c = qcl[(size_t)q * P + p];
+    if (c < 0) continue;
+    const uint32_t b = start[c], e = start[c + 1];
+    for (uint32_t m = b + lane; m < e; m += 64) {
+      const uint4 *kr = (const uint4 *)(keyZs + (size_t)m * kEmb);
+      float d = 0.f;
+      for (int g = 0; g < 8; g++) {
+        const uint4 u = kr[g];
+        const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
+        for (int t = 0; t < 4; t++) {
+          d += bf2f((uint16_t)(ws[t] & 0xFFFFu)) * sq[wave][g * 8 + t * 2];
+          d += bf2f((uint16_t)(ws[t] >> 16)) * sq[wave][g * 8 + t * 2 + 1];
+        }
+      }
+      if (d > v0) {
+        v1 = v0;
+        i1 = i0;
+        v0 = d;
+        i0 = (int)m;
+      } else if (d > v1) {
+        v1 = d;
+        i1 = (int)m;
+      }
+    }
+  }
+  int mine = -1;
+  for (int r = 0; r < 16; r++) {
+    float best = v0;
+    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+    if (best <= -3.0e38f) break;
+    const uint64_t hit = __ballot(v0 == best && i0 >= 0);
+    const int winner = (int)__builtin_ctzll(hit);
+    const int key = __shfl(i0, winner);
+    if (lane == r) mine = key;
+    if (lane == winner) {
+      v0 = v1;
+      i0 = i1;
+      v1 = -3.4e38f;
+      i1 = -1;
+    }
+  }
+  if (lane < 16) props[(size_t)q * 16 + lane] = mine >= 0 ? make_int2(keyPtsS[mine].x * 2, keyPtsS[mine].y * 2)
+                                                          : make_int2(-1, -1);
+}
+
+hipError_t learn_fine_top16_sorted(hipStream_t s, const uint16_t *qz, int nq, const uint32_t *order,
+                                   const int32_t *qcl, int P, const uint16_t *keyZs, const uint32_t *start,
+                                   const int2 *keyPtsS, int2 *props)
+{
+  if (nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fine_top16_sorted, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, order, qcl, P, keyZs, start,
+                     keyPtsS, props);
+  return hipGetLastError();
+}
+
+}  // namespace rs
