@@ -111,8 +111,10 @@ def bench_config3(args, np, torch, R, W):
 
 # Learned-mode settings of the config-5 line: proposals of every target computed once at
 # the start of pass 1 from the pass-0 result (1 inverted-file cluster searched, top 16),
-# 1 random probe beside them, 16 random probes in pass 0, at most 3 passes.
-LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=3)
+# 1 random probe beside them, 16 random probes in pass 0 whose chunks grow by 30% of the
+# visited targets, 2 passes.
+LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
+               ramp_permille=300)
 
 
 def bench_config5(args, np, torch, R, W):

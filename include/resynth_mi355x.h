@@ -172,6 +172,8 @@ typedef struct {
                                    none) */
   unsigned int maxPasses;       /* batched mode: stop after this many passes even if the
                                    refiner's rule would go on (0 = the rule alone) */
+  unsigned int rampPermille;    /* batched mode: pass-0 chunks hold this many permille of the
+                                   targets visited so far (0 = 60) */
 } ResynthOptions;
 
 typedef struct {

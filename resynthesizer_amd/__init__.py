@@ -62,7 +62,7 @@ class ResynthOptions(ctypes.Structure):
                 ("timing", ctypes.c_int), ("model", ctypes.c_void_p), ("learnedProbes", ctypes.c_uint),
                 ("learnedPasses", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
                 ("learnedProbesLater", ctypes.c_uint), ("learnedRefresh", ctypes.c_uint),
-                ("maxPasses", ctypes.c_uint)]
+                ("maxPasses", ctypes.c_uint), ("rampPermille", ctypes.c_uint)]
 
 
 class ResynthStats(ctypes.Structure):
@@ -225,10 +225,11 @@ def params_from_dict(d):
 
 
 def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False, model=None, learned_probes=0,
-            learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0):
+            learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0,
+            ramp_permille=0):
     return ResynthOptions(mode, prng, seed, batch, int(bool(timing)), model.h if model is not None else None,
                           learned_probes, learned_passes, learned_clusters, learned_probes_later, learned_refresh,
-                          max_passes)
+                          max_passes, ramp_permille)
 
 
 def format_indices(fmt):
@@ -317,7 +318,8 @@ def engine(params, ix, target_pixmap, corpus_pixmap, mode=MODE_BATCHED, prng=PRN
 def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, fmt, params=None,
                        mode=MODE_BATCHED, prng=PRNG_GRAND, d_mask2=None, mask2_row_bytes=0, seed=0, batch=0,
                        timing=False, cancel=0, model=None, learned_probes=0, learned_passes=0,
-                       learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0):
+                       learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0,
+                       ramp_permille=0):
     """imageSynth on device-resident buffers (integer device addresses, e.g. torch data_ptr()).
     Synthesizes d_image in place.  Returns a Result."""
     log = []
@@ -325,7 +327,7 @@ def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
     opt = options(mode, prng, seed, batch, timing, model, learned_probes, learned_passes, learned_clusters,
-                  learned_probes_later, learned_refresh, max_passes)
+                  learned_probes_later, learned_refresh, max_passes, ramp_permille)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().resynth_image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, d_mask2,
                                            mask2_row_bytes, fmt, pp, cb, None, ctypes.byref(cf), ctypes.byref(opt),

@@ -792,11 +792,12 @@ int refine_engine(Job &job, Prepared &pr)
   // +6% at nT/8 over the serial engine on 512x512 heals).
   const unsigned cap = job.opt.batch ? job.opt.batch : std::min(65536u, std::max(64u, nT / 32u));
   const unsigned minChunk = std::min(cap, std::max(1u, std::min(64u, nT / 2048u)));
-  static const unsigned rampPermille = [] {
+  static const unsigned rampDefault = [] {
     const char *e = std::getenv("RESYNTH_RAMP_PERMILLE");
     long v = e ? std::atol(e) : 60;
     return (unsigned)std::min(1000L, std::max(1L, v));
   }();
+  const unsigned rampPermille = job.opt.rampPermille ? std::min(1000u, job.opt.rampPermille) : rampDefault;
   // Sliding-window launches (RESYNTH_WINDOW=1 turns them on): once a chunk would reach
   // RESYNTH_WINDOW_MIN targets, the rest of the pass runs as one persistent launch whose
   // window is half the chunk the schedule above would use (a visit in a chunk misses half
