@@ -997,6 +997,9 @@ __device__ __forceinline__ unsigned long long first_lane64(unsigned long long v)
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// Tickets a wave takes at once: one counter update per group instead of per visit.
+constexpr unsigned kTicketGroup = 4;
+
 template <typename Word, int WAVES, int G0, int GL>
 __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls,
                                                            unsigned nReq, uint32_t *global)
@@ -1013,9 +1016,6 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__rest
   const Tables *loaded = nullptr;
   unsigned idleRound = 0, idleTotal = 0;
   for (;;) {
-    unsigned act = 0;
-    if (lane == 0) act = load_agent(active);
-    if (__builtin_amdgcn_readfirstlane(act) == 0) break;
     r = __builtin_amdgcn_readfirstlane(r);
     const ReqDesc &d = descs[r];
     ReqCtl &c = ctls[r];
@@ -1025,12 +1025,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__rest
       const unsigned long long cur = load_agent64(&c.ctr);
       unsigned p = (unsigned)(cur >> 32);
       if (p < 6 && (unsigned)cur < d.sizes[p]) {
-        t64 = atomicAdd(&c.ctr, 1ull);
+        t64 = atomicAdd(&c.ctr, (unsigned long long)kTicketGroup);
         p = (unsigned)(t64 >> 32);
         got = p < 6 && (unsigned)t64 < d.sizes[p];
       }
     }
     if ((__ballot(got) & 1ull) == 0) {
+      // no ticket here: leave when every request has finished, else try the next one
+      unsigned act = 0;
+      if (lane == 0) act = load_agent(active);
+      if (__builtin_amdgcn_readfirstlane(act) == 0) break;
       r = (r + 1 == nReq) ? 0u : r + 1;
       if (++idleRound >= nReq) {
         idleRound = 0;
@@ -1044,15 +1048,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__rest
     }
     idleRound = 0;
     t64 = first_lane64(t64);
-    const unsigned pass = (unsigned)(t64 >> 32), tk = (unsigned)t64;
+    const unsigned pass = (unsigned)(t64 >> 32), tk0 = (unsigned)t64;
+    const unsigned nGot = min(kTicketGroup, d.sizes[pass] - tk0);
     unsigned W = d.capW;
     if (pass == 0) {
-      const unsigned rr = (unsigned)((uint64_t)tk * d.ramp / 1000u);
+      const unsigned rr = (unsigned)((uint64_t)tk0 * d.ramp / 1000u);
       W = rr > d.minW ? rr : d.minW;
       if (W > d.capW) W = d.capW;
     }
-    if (tk > W) {
-      const unsigned need = tk - W;
+    if (tk0 > W) {
+      const unsigned need = tk0 - W;
       if (lane == 0) {
         unsigned seen = (unsigned)load_agent64(&c.pass[pass].word);
         unsigned spins = 0;
@@ -1078,14 +1083,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__rest
     g.skipEmpty = (pass == 0 && d.g.skipEmpty) ? 1 : 0;
     unsigned long long ev = 0, tm = 0;
     unsigned bet = 0;
-    visit_wave<Word, G0, GL, true>(g, d.b, tk, d.seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT[wave], sTm[wave], ev,
-                                   tm, bet);
+    for (unsigned j = 0; j < nGot; j++)
+      visit_wave<Word, G0, GL, true>(g, d.b, tk0 + j, d.seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT[wave],
+                                     sTm[wave], ev, tm, bet);
     if (lane == 0) {
-      const unsigned slot = tk & (CNT_SLOTS - 1);
+      const unsigned slot = tk0 & (CNT_SLOTS - 1);
       if (ev) atomicAdd(&d.b.counters->candidates[slot], ev);
       if (tm) atomicAdd(&d.b.counters->terms[slot], tm);
-      const unsigned long long old = atomicAdd(&c.pass[pass].word, ((unsigned long long)bet << 32) | 1ull);
-      if ((unsigned)old + 1u == d.sizes[pass]) {
+      const unsigned long long old = atomicAdd(&c.pass[pass].word, ((unsigned long long)bet << 32) | nGot);
+      if ((unsigned)old + nGot == d.sizes[pass]) {
+        // the pass is complete: lib/refiner.h:111-115 decides whether another follows
         const unsigned betters = (unsigned)(old >> 32) + bet;
         const bool more = pass < 5 && d.sizes[pass + 1] > 0 && !((double)((float)betters / (float)d.g.nT) < 0.1);
         if (more) {
