@@ -85,6 +85,43 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
   }
 }
 
+// The same accumulation without branches, for NC colour channels and no map channels:
+// every term is computed (table indices stay in [1, 511] whatever the loaded word holds)
+// and selected, so the wave does not diverge per neighbour.  The sums are identical.
+template <typename Word, int G, int NC>
+__device__ __forceinline__ void score_group_rgb(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
+                                                unsigned base, unsigned count, const int2 *nbOff, const Word *nbPix,
+                                                const uint16_t *T, uint32_t B0, uint32_t &acc, uint32_t &mx,
+                                                bool &dead)
+{
+  Word q[G];
+  bool in[G];
+#pragma unroll
+  for (int u = 0; u < G; u++) {
+    const unsigned i = base + u;
+    const bool live = i < count;
+    const int2 o = nbOff[live ? i : 0u];
+    const int qx = cx + o.x, qy = cy + o.y;
+    in[u] = live && (unsigned)qx < (unsigned)g.cw && (unsigned)qy < (unsigned)g.ch;
+    q[u] = in[u] ? cword[(size_t)qy * (unsigned)g.cw + (unsigned)qx] : (Word)0;
+  }
+#pragma unroll
+  for (int u = 0; u < G; u++) {
+    const unsigned i = base + u;
+    const Word t = nbPix[i < count ? i : 0u];
+    uint32_t cs = 0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) cs += T[256u + chan(t, c) - chan(q[u], c)];
+    cs = (i == 0) ? 0u : cs;
+    const bool valid = in[u] && (q[u] & 0xFEu) == 0xFEu;
+    const uint32_t term = valid ? cs : g.penalty;
+    const bool act = i < count && !dead;
+    acc = act ? acc + term : acc;
+    mx = acc > mx ? acc : mx;
+    dead = dead || (act && acc >= B0);
+  }
+}
+
 // Scores candidate (cx,cy) against the staged patch.  The lane stops (dead) at the first
 // prefix sum >= B0, where B0 is the best sum before this batch of candidates: such a
 // candidate cannot be accepted by the sequential update, whose best only decreases.
@@ -94,7 +131,8 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
 // map channels, so neighbour 0 (the candidate itself, colours not compared) adds 0; with
 // B0 > 0 its early-out test cannot fire either.  G0 is the size of the first group of
 // loads, which is where most random candidates die.
-template <typename Word, int G0 = 8, int GL = 8>
+// NC > 0: the branch-free path for NC colours and no maps (the caller guarantees it).
+template <typename Word, int G0 = 8, int GL = 8, int NC = 0>
 __device__ __forceinline__ void score_candidate(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
                                                 unsigned count, const int2 *nbOff, const Word *nbPix,
                                                 const uint16_t *T, const uint32_t *Tm, uint32_t B0, uint32_t &F,
@@ -103,9 +141,15 @@ __device__ __forceinline__ void score_candidate(const Geometry &g, const Word *_
   uint32_t acc = 0, mx = 0;
   bool dead = false;
   unsigned base = (skip0 && B0 > 0) ? 1u : 0u;
-  score_group<Word, G0>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
-  for (base += G0; base < count && !dead; base += GL)
-    score_group<Word, GL>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+  if constexpr (NC > 0) {
+    score_group_rgb<Word, G0, NC>(g, cword, cx, cy, base, count, nbOff, nbPix, T, B0, acc, mx, dead);
+    for (base += G0; base < count && !dead; base += GL)
+      score_group_rgb<Word, GL, NC>(g, cword, cx, cy, base, count, nbOff, nbPix, T, B0, acc, mx, dead);
+  } else {
+    score_group<Word, G0>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+    for (base += G0; base < count && !dead; base += GL)
+      score_group<Word, GL>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+  }
   F = acc;
   M = mx;
   alive = !dead;
@@ -232,7 +276,7 @@ __device__ unsigned gather_neighbours(const Geometry &g, const Word *tword, cons
 // Candidates clipped/masked in the corpus, already probed by this target index
 // (heuristic 2) or repeating an earlier lane's candidate are skipped; the rest are scored
 // in lane order.  Returns the lanes whose candidate must be recorded in the prober map.
-template <typename Word>
+template <typename Word, int NC = 0>
 __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *__restrict__ cword,
                                                 const uint32_t *prober, unsigned k, unsigned count,
                                                 const int2 *nbOff, const Word *nbPix, const int2 *nbSrc,
@@ -269,7 +313,8 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
   uint32_t F = 0, M = 0;
   bool alive = false;
   if (active)
-    score_candidate<Word>(g, cword, cx, cy, count, nbOff, nbPix, T, Tm, 0xFFFFFFFFu, F, M, alive, g.nMaps == 0);
+    score_candidate<Word, 8, 8, NC>(g, cword, cx, cy, count, nbOff, nbPix, T, Tm, 0xFFFFFFFFu, F, M, alive,
+                                    g.nMaps == 0);
   uint64_t amask = __ballot(active);
   evals += (unsigned)__popcll(amask);
   int win = -1, stop = 64;
@@ -645,7 +690,7 @@ __device__ __forceinline__ void put(T *p, T v)
 }
 
 // One batched visit of target k by one wavefront (lib/synthesize.h:426-642).
-template <typename Word, int G0, int GL, bool PUBLISH>
+template <typename Word, int G0, int GL, bool PUBLISH, int NC = 0>
 __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers &b, unsigned k, unsigned seed,
                                            unsigned pass, int2 *sOff, Word *sPix, int2 *sSrc, const uint16_t *sT,
                                            const uint32_t *sTm, unsigned long long &myEval,
@@ -663,8 +708,8 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   bool perfect = false, scored = false, record = false;
   int cx, cy;
   unsigned evals = 0;
-  coherence_phase<Word>(g, cword, b.prober, k, count, sOff, sPix, sSrc, sT, sTm, best, winX, winY, perfect, scored,
-                        record, cx, cy, evals);
+  coherence_phase<Word, NC>(g, cword, b.prober, k, count, sOff, sPix, sSrc, sT, sTm, best, winX, winY, perfect,
+                            scored, record, cx, cy, evals);
   if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
   // learned proposals: up to 16 corpus points, scored in rank order like probes
   if (b.props && !perfect) {
@@ -673,7 +718,8 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
     const bool valid = c.x >= 0;
     uint32_t F = 0, M = 0;
     bool alive = false;
-    if (valid) score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
+    if (valid)
+      score_candidate<Word, G0, GL, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
     int win = -1, stop = 64;
     uint32_t bb = best;
     resolve(valid && alive, M, F, bb, win, stop);
@@ -714,12 +760,12 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
       // the candidate's own word (neighbour 0, loaded with the first group) carries the
       // corpus-point flag
       const Word w0 = cword[p];
-      score_candidate<Word, G0 + 1, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, false);
+      score_candidate<Word, G0 + 1, GL, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, false);
       valid = (w0 & 0xFFu) == 0xFFu;
     } else {
       c = b.corpusPts[probe_index(g, b, h)];
       valid = true;
-      score_candidate<Word, G0, GL>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
+      score_candidate<Word, G0, GL, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
     }
     const uint64_t vm = __ballot(valid);
     const bool active = valid && drawn + (unsigned)__popcll(vm & lanemask_lt()) < g.maxProbe;
@@ -795,7 +841,7 @@ __device__ __forceinline__ void flush_counters(const SynthBuffers &b, unsigned l
 
 // Batched visits, one wavefront per target (large chunks: every wave stays busy on its
 // own target instead of idling through another wave's gather and heuristic-1 phases).
-template <typename Word, int WAVES, int G0, int GL>
+template <typename Word, int WAVES, int G0, int GL, int NC = 0>
 __global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
                                                            unsigned seed, unsigned pass)
 {
@@ -815,8 +861,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuff
   unsigned long long myEval = 0, myTerm = 0;
   unsigned myBet = 0;
   for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES)
-    visit_wave<Word, G0, GL, false>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval, myTerm,
-                                    myBet);
+    visit_wave<Word, G0, GL, false, NC>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval,
+                                        myTerm, myBet);
   flush_counters<Word, WAVES, true>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
 }
 
@@ -906,6 +952,13 @@ static unsigned wave_min()
 }
 unsigned synth_wave_min() { return wave_min(); }
 
+// RESYNTH_RGB_FAST=0 keeps the generic scorer everywhere (A/B comparisons).
+static bool rgb_fast_path()
+{
+  const char *e = std::getenv("RESYNTH_RGB_FAST");
+  return e ? std::atoi(e) != 0 : true;
+}
+
 static int first_group()
 {
   static int v = [] {
@@ -926,10 +979,19 @@ static void launch_wave_g(hipStream_t s, dim3 grid, const Geometry &g, const Syn
     const char *e = std::getenv("RESYNTH_LATER_GROUP");
     return (e && std::atoi(e) == 8) ? 8 : 4;
   }();
-  if (later == 8)
-    hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 8>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
-  else
-    hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 4>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
+  // RGB without maps (32-bit words) takes the branch-free scorer
+  const bool rgb = sizeof(Word) == 4 && g.nColors == 3 && g.nMaps == 0 && rgb_fast_path();
+  if (later == 8) {
+    if (rgb)
+      hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 8, 3>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
+    else
+      hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 8>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
+  } else {
+    if (rgb)
+      hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 4, 3>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
+    else
+      hipLaunchKernelGGL((k_synth_wave<Word, WAVES, G0, 4>), grid, dim3(64 * WAVES), 0, s, g, b, k0, k1, seed, pass);
+  }
 }
 
 template <typename Word>
@@ -1746,7 +1808,7 @@ hipError_t launch_mean_error(hipStream_t s, const uint32_t *lastBest, const int2
 
 // ------------------------------------------------------------------ frozen-state scorer
 
-template <typename Word>
+template <typename Word, int NC = 0>
 __global__ __launch_bounds__(64) void k_score_frozen(Geometry g, const Word *cword, const Tables *tables,
                                                      const unsigned *nbCount, const int2 *nbOffset,
                                                      const Word *nbWord, const unsigned *candCount, unsigned maxCands,
@@ -1781,7 +1843,7 @@ __global__ __launch_bounds__(64) void k_score_frozen(Geometry g, const Word *cwo
     bool alive = false;
     if (active) {
       int2 c = cands[(size_t)prob * maxCands + j];
-      score_candidate<Word>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive);
+      score_candidate<Word, 8, 8, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive);
     }
     int win = -1, stop = 64;
     uint32_t bb = best;
@@ -1818,6 +1880,10 @@ hipError_t launch_score_frozen(hipStream_t s, bool wide, int cw, int ch, int nCo
   if (wide)
     hipLaunchKernelGGL((k_score_frozen<uint64_t>), dim3(nProblems), dim3(64), 0, s, g, (const uint64_t *)cword,
                        tables, nbCount, nbOffset, (const uint64_t *)nbWord, candCount, maxCands, cands, winner,
+                       bestSum, stopped);
+  else if (nColors == 3 && nMaps == 0 && rgb_fast_path())
+    hipLaunchKernelGGL((k_score_frozen<uint32_t, 3>), dim3(nProblems), dim3(64), 0, s, g, (const uint32_t *)cword,
+                       tables, nbCount, nbOffset, (const uint32_t *)nbWord, candCount, maxCands, cands, winner,
                        bestSum, stopped);
   else
     hipLaunchKernelGGL((k_score_frozen<uint32_t>), dim3(nProblems), dim3(64), 0, s, g, (const uint32_t *)cword,
