@@ -1283,7 +1283,9 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
     HIP_TRY(launch_fill_u32(s, (uint32_t *)dC, 0u, (size_t)m * sizeof(ReqCtl) / 4));
     HIP_TRY(launch_fill_u32(s, dG, 0u, MG_WORDS));
     HIP_TRY(launch_fill_u32(s, dG + MG_ACTIVE, m, 1));
-    HIP_TRY(launch_synth_many(s, wide != 0, cfg, dD, dC, m, dG));
+    bool allRgb = true;
+    for (unsigned j = 0; j < m; j++) allRgb = allRgb && hD[j].g.nColors == 3 && hD[j].g.nMaps == 0;
+    HIP_TRY(launch_synth_many(s, wide != 0, cfg, dD, dC, m, dG, allRgb));
     HIP_TRY(hipStreamEndCapture(s, &graph));
     HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     hipEvent_t e0, e1;

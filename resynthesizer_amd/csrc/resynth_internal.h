@@ -142,7 +142,7 @@ hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, cons
                                unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,
                                unsigned cap);
 hipError_t launch_synth_many(hipStream_t s, bool wide, const ManyConfig &cfg, const ReqDesc *descs, ReqCtl *ctls,
-                             unsigned nReq, uint32_t *global);
+                             unsigned nReq, uint32_t *global, bool allRgb);
 bool many_config_valid(const ManyConfig &cfg);
 hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, int prngKind, const uint32_t *stream, uint32_t streamLen,

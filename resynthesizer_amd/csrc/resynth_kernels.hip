@@ -272,6 +272,8 @@ __device__ unsigned gather_neighbours(const Geometry &g, const Word *tword, cons
   return count;
 }
 
+static bool rgb_fast_path();
+
 // Heuristic 1 (lib/synthesize.h:537-580): lane i proposes source(neighbour i) - offset i.
 // Candidates clipped/masked in the corpus, already probed by this target index
 // (heuristic 2) or repeating an earlier lane's candidate are skipped; the rest are scored
@@ -520,7 +522,7 @@ __device__ void resolve_slots_wave0(VisitLDS<Word, WAVES> &S, unsigned nSlots, i
 // probes from the exact PRNG stream (prngKind, stream, streamLen, S.streamPos) and aborts
 // without side effects when the stream runs dry; otherwise probes come from a per-target
 // counter-based stream.  Global writes happen only in the commit at the end.
-template <typename Word, int WAVES, bool SERIAL>
+template <typename Word, int WAVES, bool SERIAL, int NC = 0>
 __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word, WAVES> &S, unsigned k,
                          unsigned seed, unsigned pass, int prngKind, const uint32_t *stream, uint32_t streamLen,
                          uint32_t maxvalue, unsigned long long &evalTot, unsigned long long &termTot,
@@ -540,7 +542,7 @@ __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word
     int winX = 0, winY = 0;
     bool perfect = false, scored = false;
     unsigned evals = 0;
-    coherence_phase<Word>(g, cword, b.prober, k, count, S.off, S.pix, S.src, S.T, S.Tm, best, winX, winY, perfect,
+    coherence_phase<Word, NC>(g, cword, b.prober, k, count, S.off, S.pix, S.src, S.T, S.Tm, best, winX, winY, perfect,
                           scored, record, cx, cy, evals);
     if (lane == 0) {
       S.best = best;
@@ -589,7 +591,8 @@ __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word
     int2 c = make_int2(0, 0);
     if (active) {
       c = b.corpusPts[idx];
-      score_candidate<Word>(g, cword, c.x, c.y, count, S.off, S.pix, S.T, S.Tm, S.best, F, M, alive, g.nMaps == 0);
+      score_candidate<Word, 8, 8, NC>(g, cword, c.x, c.y, count, S.off, S.pix, S.T, S.Tm, S.best, F, M, alive,
+                                      g.nMaps == 0);
     }
     S.cand[tid] = c;
     S.candM[tid] = M;
@@ -656,7 +659,7 @@ __device__ __forceinline__ void load_tables(const SynthBuffers &b, VisitLDS<Word
 }
 
 // Batched visits: one workgroup per target of [k0, k1).
-template <typename Word, int WAVES>
+template <typename Word, int WAVES, int NC = 0>
 __global__ __launch_bounds__(64 * WAVES) void k_synth_batched(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
                                                               unsigned seed, unsigned pass)
 {
@@ -665,7 +668,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_batched(Geometry g, SynthB
   unsigned long long ev = 0, tm = 0;
   unsigned bt = 0;
   for (unsigned k = k0 + blockIdx.x; k < k1; k += gridDim.x) {
-    visit_wg<Word, WAVES, false>(g, b, S, k, seed, pass, 0, nullptr, 0, 0, ev, tm, bt);
+    visit_wg<Word, WAVES, false, NC>(g, b, S, k, seed, pass, 0, nullptr, 0, 0, ev, tm, bt);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -1020,10 +1023,13 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
     return hipGetLastError();
   }
   unsigned blocks = k1 - k0;
+  const bool rgb = !wide && g.nColors == 3 && g.nMaps == 0 && rgb_fast_path();
   if (blocks <= 256) {
     // few targets: a 1024-lane workgroup per target shortens each visit's latency
     if (wide)
       hipLaunchKernelGGL((k_synth_batched<uint64_t, 16>), dim3(blocks), dim3(1024), 0, s, g, b, k0, k1, seed, pass);
+    else if (rgb)
+      hipLaunchKernelGGL((k_synth_batched<uint32_t, 16, 3>), dim3(blocks), dim3(1024), 0, s, g, b, k0, k1, seed, pass);
     else
       hipLaunchKernelGGL((k_synth_batched<uint32_t, 16>), dim3(blocks), dim3(1024), 0, s, g, b, k0, k1, seed, pass);
     return hipGetLastError();
@@ -1031,6 +1037,9 @@ hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, con
   if (wide)
     hipLaunchKernelGGL((k_synth_batched<uint64_t, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
                        pass);
+  else if (rgb)
+    hipLaunchKernelGGL((k_synth_batched<uint32_t, WAVES, 3>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1,
+                       seed, pass);
   else
     hipLaunchKernelGGL((k_synth_batched<uint32_t, WAVES>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1, seed,
                        pass);
@@ -1062,7 +1071,7 @@ __device__ __forceinline__ unsigned long long first_lane64(unsigned long long v)
 // Tickets a wave takes at once: one counter update per group instead of per visit.
 constexpr unsigned kTicketGroup = 4;
 
-template <typename Word, int WAVES, int G0, int GL>
+template <typename Word, int WAVES, int G0, int GL, int NC = 0>
 __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls,
                                                            unsigned nReq, uint32_t *global)
 {
@@ -1146,8 +1155,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__rest
     unsigned long long ev = 0, tm = 0;
     unsigned bet = 0;
     for (unsigned j = 0; j < nGot; j++)
-      visit_wave<Word, G0, GL, true>(g, d.b, tk0 + j, d.seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT[wave],
-                                     sTm[wave], ev, tm, bet);
+      visit_wave<Word, G0, GL, true, NC>(g, d.b, tk0 + j, d.seed, pass, sOff[wave], sPix[wave], sSrc[wave],
+                                         sT[wave], sTm[wave], ev, tm, bet);
     if (lane == 0) {
       const unsigned slot = tk0 & (CNT_SLOTS - 1);
       if (ev) atomicAdd(&d.b.counters->candidates[slot], ev);
@@ -1176,21 +1185,31 @@ bool many_config_valid(const ManyConfig &c)
          (c.laterGroup == 4 || c.laterGroup == 8);
 }
 
-template <typename Word, int WAVES, int G0, int GL>
-static hipError_t launch_many_t(hipStream_t s, const ReqDesc *descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
+template <typename Word, int WAVES, int G0, int GL, int NC>
+static hipError_t launch_many_nc(hipStream_t s, const ReqDesc *descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
 {
   static int resident = [] {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 1024;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_synth_many<Word, WAVES, G0, GL>, 64 * WAVES, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_synth_many<Word, WAVES, G0, GL, NC>, 64 * WAVES, 0) !=
             hipSuccess || per < 1)
       per = 1;
     return cus * per;
   }();
-  hipLaunchKernelGGL((k_synth_many<Word, WAVES, G0, GL>), dim3(resident), dim3(64 * WAVES), 0, s, descs, ctls, nReq,
-                     global);
+  hipLaunchKernelGGL((k_synth_many<Word, WAVES, G0, GL, NC>), dim3(resident), dim3(64 * WAVES), 0, s, descs, ctls,
+                     nReq, global);
   return hipGetLastError();
+}
+
+// Every request of the launch RGB without maps (32-bit words): the branch-free scorer.
+static thread_local bool g_manyRgb = false;
+
+template <typename Word, int WAVES, int G0, int GL>
+static hipError_t launch_many_t(hipStream_t s, const ReqDesc *descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
+{
+  if (sizeof(Word) == 4 && g_manyRgb) return launch_many_nc<Word, WAVES, G0, GL, 3>(s, descs, ctls, nReq, global);
+  return launch_many_nc<Word, WAVES, G0, GL, 0>(s, descs, ctls, nReq, global);
 }
 
 template <typename Word, int WAVES>
@@ -1217,10 +1236,11 @@ static hipError_t launch_many_word(hipStream_t s, const ManyConfig &c, const Req
 
 // The caller zeroes ctls, sets each ctr to pass 0 and global[MG_ACTIVE] to nReq.
 hipError_t launch_synth_many(hipStream_t s, bool wide, const ManyConfig &cfg, const ReqDesc *descs, ReqCtl *ctls,
-                             unsigned nReq, uint32_t *global)
+                             unsigned nReq, uint32_t *global, bool allRgb)
 {
   if (!nReq) return hipSuccess;
   if (!many_config_valid(cfg)) return hipErrorInvalidValue;
+  g_manyRgb = allRgb && rgb_fast_path();
   return wide ? launch_many_word<uint64_t>(s, cfg, descs, ctls, nReq, global)
               : launch_many_word<uint32_t>(s, cfg, descs, ctls, nReq, global);
 }
@@ -1270,7 +1290,7 @@ hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, cons
 
 // Serial visits: one workgroup visits targets k0..k1-1 in order (lib/refiner.h:75-100),
 // drawing probes from the exact PRNG stream.
-template <typename Word, int WAVES>
+template <typename Word, int WAVES, int NC = 0>
 __global__ __launch_bounds__(64 * WAVES) void k_synth_serial(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
                                                              int prngKind, const uint32_t *__restrict__ stream,
                                                              uint32_t streamLen, StreamState *state)
@@ -1294,7 +1314,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_serial(Geometry g, SynthBu
   for (; k < k1; k++) {
     const uint32_t posBefore = S.streamPos;
     __syncthreads();
-    visit_wg<Word, WAVES, true>(g, b, S, k, 0, 0, prngKind, stream, streamLen, maxvalue, ev, tm, bt);
+    visit_wg<Word, WAVES, true, NC>(g, b, S, k, 0, 0, prngKind, stream, streamLen, maxvalue, ev, tm, bt);
     __syncthreads();
     if (S.aborted) {
       __syncthreads();
@@ -1321,6 +1341,9 @@ hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, cons
   constexpr int WAVES = 4;
   if (wide)
     hipLaunchKernelGGL((k_synth_serial<uint64_t, WAVES>), dim3(1), dim3(64 * WAVES), 0, s, g, b, k0, k1, prngKind,
+                       stream, streamLen, state);
+  else if (g.nColors == 3 && g.nMaps == 0 && rgb_fast_path())
+    hipLaunchKernelGGL((k_synth_serial<uint32_t, WAVES, 3>), dim3(1), dim3(64 * WAVES), 0, s, g, b, k0, k1, prngKind,
                        stream, streamLen, state);
   else
     hipLaunchKernelGGL((k_synth_serial<uint32_t, WAVES>), dim3(1), dim3(64 * WAVES), 0, s, g, b, k0, k1, prngKind,
