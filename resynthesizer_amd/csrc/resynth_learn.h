@@ -77,11 +77,6 @@ hipError_t learn_key_flags(hipStream_t s, bool wide, const void *cword, int cw, 
 // bf16 (MFMA GEMM with the selection fused in), out [M x P] column indices.
 hipError_t learn_topk_cols(hipStream_t s, const uint16_t *rows, int M, const uint16_t *cols, int N, int P,
                            int32_t *out);
-// Counting sort of n items by cluster id: start[C+1], members[n] (item indices).
-hipError_t learn_bucket(hipStream_t s, const int32_t *cluster, int n, int C, uint32_t *count, uint32_t *start,
-                        uint32_t *cursor, uint32_t *members);
-// Fine search: for each query, the top-16 keys (by dot product) among the members of its P
-// clusters; writes their corpus coordinates (key grid point * 2) or (-1, -1).
 // Pairs for training: pts[i] = targets[hash(key, i) % nT] and src[i] = sourceOf of that
 // target (row-major target image of width tw).
 hipError_t learn_sample_pairs(hipStream_t s, const int2 *targets, unsigned nT, const int2 *sourceOf, int tw, int n,
@@ -95,14 +90,16 @@ hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const int3
                            const uint16_t *subCent, int P, int32_t *qcl);
 // Sort keys from cluster ids (every stride-th entry) with values 0..n-1.
 hipError_t learn_sort_keys(hipStream_t s, const int32_t *cluster, int stride, int n, uint32_t *keys, uint32_t *vals);
+// start[c] = first position of cluster c in the sorted ids (start[C] = n).
 hipError_t learn_bucket_starts(hipStream_t s, const uint32_t *sorted, int n, int C, uint32_t *start);
+// Key embeddings and their corpus points copied into cluster order.
 hipError_t learn_permute_keys(hipStream_t s, const uint32_t *members, int n, const uint16_t *keyZ, const int2 *keyPts,
                               uint16_t *keyZs, int2 *keyPtsS);
+// Fine search: for each query, the top-16 keys (by dot product) among the keys of its P
+// clusters (keys in cluster order); writes their corpus coordinates (key grid point * 2)
+// or (-1, -1).  order: the queries sorted by their first cluster.
 hipError_t learn_fine_top16_sorted(hipStream_t s, const uint16_t *qz, int nq, const uint32_t *order,
                                    const int32_t *qcl, int P, const uint16_t *keyZs, const uint32_t *start,
                                    const int2 *keyPtsS, int2 *props);
-hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qcl, int P,
-                            const uint16_t *keys, const uint32_t *start, const uint32_t *members, const int2 *keyPts,
-                            int2 *props);
 
 }  // namespace rs

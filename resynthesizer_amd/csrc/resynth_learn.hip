@@ -271,26 +271,6 @@ __global__ void k_relu_bwd(const float *g, const uint16_t *act, uint16_t *out, s
   if (i < n) out[i] = (act[i] != 0 && !(act[i] & 0x8000u)) ? f2bf(g[i]) : (uint16_t)0;
 }
 
-// Column sums of a rows x cols fp32 matrix into out (bias gradients).
-__global__ void k_colsum(const float *g, int rows, int cols, float *out)
-{
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  float s = 0.f;
-  for (int r = 0; r < rows; r++) s += g[(size_t)r * cols + c];
-  out[c] = s;
-}
-
-// Column sums of a bf16 matrix (bias gradient of a layer whose output gradient is bf16).
-__global__ void k_colsum_bf16(const uint16_t *g, int rows, int cols, float *out)
-{
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  float s = 0.f;
-  for (int r = 0; r < rows; r++) s += bf2f(g[(size_t)r * cols + c]);
-  out[c] = s;
-}
-
 __global__ void k_sum(const float *x, int n, float *out)
 {
   __shared__ float red[256];
@@ -665,130 +645,7 @@ hipError_t learn_topk_cols(hipStream_t s, const uint16_t *rows, int M, const uin
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ buckets
-
-__global__ void k_bucket_count(const int32_t *cluster, int n, uint32_t *count)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && cluster[i] >= 0) atomicAdd(&count[cluster[i]], 1u);
-}
-
-// Exclusive scan of C counts by one workgroup (each thread a contiguous run).
-__global__ __launch_bounds__(1024) void k_bucket_scan(const uint32_t *count, int C, uint32_t *start, uint32_t *cursor)
-{
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x, per = (C + 1023) / 1024;
-  const int b = t * per, e = min(C, b + per);
-  uint32_t s = 0;
-  for (int i = b; i < e; i++) s += count[i];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const uint32_t v = t >= o ? part[t - o] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  uint32_t run = part[t] - s;
-  for (int i = b; i < e; i++) {
-    start[i] = run;
-    cursor[i] = 0;
-    run += count[i];
-  }
-  if (t == 1023) start[C] = part[1023];
-}
-
-__global__ void k_bucket_scatter(const int32_t *cluster, int n, const uint32_t *start, uint32_t *cursor,
-                                 uint32_t *members)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || cluster[i] < 0) return;
-  const int c = cluster[i];
-  members[start[c] + atomicAdd(&cursor[c], 1u)] = (uint32_t)i;
-}
-
-hipError_t learn_bucket(hipStream_t s, const int32_t *cluster, int n, int C, uint32_t *count, uint32_t *start,
-                        uint32_t *cursor, uint32_t *members)
-{
-  hipLaunchKernelGGL(k_zero, dim3((C + 255) / 256), dim3(256), 0, s, (float *)count, (size_t)C);
-  if (n > 0) hipLaunchKernelGGL(k_bucket_count, dim3((n + 255) / 256), dim3(256), 0, s, cluster, n, count);
-  hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, count, C, start, cursor);
-  if (n > 0)
-    hipLaunchKernelGGL(k_bucket_scatter, dim3((n + 255) / 256), dim3(256), 0, s, cluster, n, start, cursor, members);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------------------ fine search
-
-// One wavefront per query: every member key of the query's P clusters is scored against
-// the query (64-dim bf16 dot products in fp32), each lane keeps its best two, and the
-// wave's top 16 are then taken one at a time.
-__global__ __launch_bounds__(256) void k_fine_top16(const uint16_t *qz, int nq, const int32_t *qcl, int P,
-                                                    const uint16_t *keys, const uint32_t *start,
-                                                    const uint32_t *members, const int2 *keyPts, int2 *props)
-{
-  __shared__ float sq[4][kEmb];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = blockIdx.x * 4 + wave;
-  if (q >= nq) return;
-  sq[wave][lane] = bf2f(qz[(size_t)q * kEmb + lane]);
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
-  float v0 = -3.4e38f, v1 = -3.4e38f;
-  int i0 = -1, i1 = -1;
-  for (int p = 0; p < P; p++) {
-    const int c = qcl[(size_t)q * P + p];
-    if (c < 0) continue;
-    const uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t m = b + lane; m < e; m += 64) {
-      const uint32_t key = members[m];
-      const uint4 *kr = (const uint4 *)(keys + (size_t)key * kEmb);
-      float d = 0.f;
-      for (int g = 0; g < 8; g++) {
-        const uint4 u = kr[g];
-        const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
-        for (int t = 0; t < 4; t++) {
-          d += bf2f((uint16_t)(ws[t] & 0xFFFFu)) * sq[wave][g * 8 + t * 2];
-          d += bf2f((uint16_t)(ws[t] >> 16)) * sq[wave][g * 8 + t * 2 + 1];
-        }
-      }
-      if (d > v0) {
-        v1 = v0;
-        i1 = i0;
-        v0 = d;
-        i0 = (int)key;
-      } else if (d > v1) {
-        v1 = d;
-        i1 = (int)key;
-      }
-    }
-  }
-  int mine = -1;
-  for (int r = 0; r < 16; r++) {
-    // wave maximum of the lanes' current heads
-    float best = v0;
-    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    if (best <= -3.0e38f) break;
-    const uint64_t hit = __ballot(v0 == best && i0 >= 0);
-    const int winner = (int)__builtin_ctzll(hit);
-    const int key = __shfl(i0, winner);
-    if (lane == r) mine = key;
-    if (lane == winner) {
-      v0 = v1;
-      i0 = i1;
-      v1 = -3.4e38f;
-      i1 = -1;
-    }
-  }
-  if (lane < 16) {
-    int2 o = make_int2(-1, -1);
-    if (mine >= 0) {
-      const int2 kp = keyPts[mine];
-      o = make_int2(kp.x * 2, kp.y * 2);
-    }
-    props[(size_t)q * 16 + lane] = o;
-  }
-}
 
 __global__ void k_sample_pairs(const int2 *targets, unsigned nT, const int2 *sourceOf, int tw, int n, uint64_t key,
                                int2 *pts, int2 *src)
@@ -810,15 +667,6 @@ hipError_t learn_sample_pairs(hipStream_t s, const int2 *targets, unsigned nT, c
   return hipGetLastError();
 }
 
-hipError_t learn_fine_top16(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qcl, int P,
-                            const uint16_t *keys, const uint32_t *start, const uint32_t *members, const int2 *keyPts,
-                            int2 *props)
-{
-  if (nq <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fine_top16, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, qcl, P, keys, start, members, keyPts,
-                     props);
-  return hipGetLastError();
-}
 
 }  // namespace rs
 
@@ -1236,7 +1084,8 @@ hipError_t learn_permute_keys(hipStream_t s, const uint32_t *members, int n, con
 
 // Fine search over clusters stored contiguously (keyZs, keyPtsS in cluster order); queries
 // visited in `order` (sorted by cluster, so that a cluster's keys are read by consecutive
-// waves and stay in L2).  Same selection as k_fine_top16.
+// waves and stay in L2).  Each lane keeps its best two, and the wave's top 16 are then taken
+// one at a time.
 __global__ __launch_bounds__(256) void k_fine_top16_sorted(const uint16_t *qz, int nq, const uint32_t *order,
                                                            const int32_t *qcl, int P, const uint16_t *keyZs,
                                                            const uint32_t *start, const int2 *keyPtsS, int2 *props)
