@@ -260,7 +260,7 @@ class Result:
 def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, mask2=None,
                 width=None, mask_width=None, mask2_width=None, seed=0, batch=0, timing=False, model=None,
                 learned_probes=0, learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0,
-                max_passes=0):
+                max_passes=0, ramp_permille=0):
     """imageSynth / imageSynth2 (lib/imageSynth.c:62-217) on the GPU.
 
     image: uint8 (H, W, C) or a padded row buffer (H, rowBytes) with `width` given;
@@ -288,7 +288,7 @@ def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAN
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
     opt = options(mode, prng, seed, batch, timing, model, learned_probes, learned_passes, learned_clusters,
-                  learned_probes_later, learned_refresh, max_passes)
+                  learned_probes_later, learned_refresh, max_passes, ramp_permille)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().imageSynthEx(ctypes.byref(ib), ctypes.byref(mb), mb2p, fmt, pp, cb, None, ctypes.byref(cf),
                              ctypes.byref(opt), ctypes.byref(st))
@@ -296,7 +296,7 @@ def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAN
 
 
 def engine(params, ix, target_pixmap, corpus_pixmap, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, seed=0,
-           batch=0, timing=False):
+           batch=0, timing=False, max_passes=0, ramp_permille=0):
     """engine() (lib/engine.c:539-690) on engine-layout pixmaps uint8 (H, W, depth).
     Returns (Result, target_out)."""
     t = np.array(target_pixmap, dtype=np.uint8, copy=True, order="C")
@@ -309,7 +309,7 @@ def engine(params, ix, target_pixmap, corpus_pixmap, mode=MODE_BATCHED, prng=PRN
     cb = PROGRESS(lambda pct, ctx: log.append(pct))
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
-    opt = options(mode, prng, seed, batch, timing)
+    opt = options(mode, prng, seed, batch, timing, max_passes=max_passes, ramp_permille=ramp_permille)
     err = lib().engineEx(params, ctypes.byref(ix), ctypes.byref(tm), ctypes.byref(cm), cb, None, ctypes.byref(cf),
                          ctypes.byref(opt), ctypes.byref(st))
     return Result(err, st, log), t

@@ -277,6 +277,37 @@ def test_batched_invariants_and_quality_config1():
     assert ratio <= 1.05, (bat, ser)
 
 
+@pytest.mark.parametrize("mtype,th,tv,bound", [(0, 1, 1, 1.6), (2, 0, 0, 1.2), (5, 0, 0, 1.2), (8, 1, 0, 1.2),
+                                               (3, 0, 1, 1.2)])
+def test_batched_orderings_and_tiling(mtype, th, tv, bound):
+    """Batched passes under the target orderings and seamless tiling that the heal,
+    uncrop and render-texture callers select (lib/orderTarget.h:269-343,
+    lib/synthesize.h:81-113): context untouched, corpus colours only, and a mean error
+    close to the serial engine's under the same parameters.  At 256x256 one run's error
+    varies between PRNG seeds by 6-14% with context and by about 50% without it (type 0),
+    so each side averages 3 images x 4 seeds and the bound leaves room for that spread
+    (measured ratios: 1.01-1.05 for types 2, 3, 8 and 1.06-1.11 for type 5)."""
+    p = R.default_params(matchContextType=mtype, tileH=th, tileV=tv)
+    ser, bat = [], []
+    for img_seed in (1, 2, 3):
+        img, mask = W.config1(img_seed)
+        sel = mask != 0
+        corpus_colours = {tuple(c) for c in img[~sel].reshape(-1, 3)}
+        for s in range(4):
+            r, _ = R.image_synth(img, mask, R.T_RGB, p, mode=R.MODE_SERIAL, seed=s)
+            assert r.err == 0
+            ser.append(r.meanError)
+            r, out = R.image_synth(img, mask, R.T_RGB, p, mode=R.MODE_BATCHED, seed=s)
+            assert r.err == 0
+            assert np.array_equal(out[~sel], img[~sel])
+            assert all(tuple(c) in corpus_colours for c in out[sel].reshape(-1, 3)[::3])
+            bat.append(r.meanError)
+    ratio = np.mean(bat) / np.mean(ser)
+    print("type %d tile %d%d mean error: batched %.0f serial %.0f ratio %.4f"
+          % (mtype, th, tv, np.mean(bat), np.mean(ser), ratio))
+    assert ratio <= bound, (bat, ser)
+
+
 def test_batched_config2_768():
     """Config 2's generators at 768x768 (about 75k targets, chunks up to the 2.3k cap):
     context pixels stay untouched, every target takes a corpus colour, and the mean
@@ -344,6 +375,83 @@ def test_batched_map_transfer_config4():
     ratio = np.mean(bat) / np.mean(ser)
     print("config 4 crop mean error: batched %.0f serial %.0f ratio %.4f" % (np.mean(bat), np.mean(ser), ratio))
     assert ratio <= 1.08, (bat, ser)
+
+
+FORMAT_MATRIX = [
+    # colour channels, map channels, alpha in target, alpha in corpus, patch, probes, mapWeight
+    (3, 3, True, True, 30, 200, 0.5),       # RGBA with RGB maps: total_bpp 8, the widest pixel
+    (3, 3, False, False, 20, 100, 0.3),
+    (1, 1, True, True, 16, 64, 0.5),
+    (1, 3, False, True, 30, 120, 0.8),      # grey image, RGB maps, alpha only in the corpus
+    (3, 1, True, False, 12, 50, 0.5),       # alpha only in the target
+    (1, 0, True, True, 10, 60, 0.5),        # grey+alpha, no map
+]
+
+
+def _engine_pixmaps(colors, maps, alpha_t, alpha_s, seed, tsize=(40, 52), csize=(36, 44)):
+    """Target and corpus pixmaps in the engine layout [mask, colours, alpha?, maps]
+    (lib/imageFormat.c:128-150): the target selection holds a hole, a partially selected
+    pixel and unselected context; the corpus has an unselected band, partially selected
+    pixels (not corpus) and, where the format has alpha, fully transparent pixels."""
+    alpha = alpha_t or alpha_s
+    depth = 1 + colors + int(alpha) + maps
+    rng = np.random.RandomState(seed)
+
+    def build(h, w, s):
+        p = np.zeros((h, w, depth), np.uint8)
+        p[..., 1:1 + colors] = W.grating_noise(h, w, colors, s)
+        if alpha:
+            p[..., 1 + colors] = rng.choice([0, 0, 9, 255, 255, 255, 255, 255], size=(h, w))
+        if maps:
+            p[..., 1 + colors + int(alpha):] = W.grating_noise(h, w, maps, s + 1)
+        return p
+
+    th, tw = tsize
+    tp = build(th, tw, seed)
+    tp[th // 4: th // 4 + th // 2, tw // 3: tw // 3 + tw // 2, 0] = 255
+    tp[th // 5, tw // 2, 0] = 77
+    ch, cw = csize
+    cp = build(ch, cw, seed + 10)
+    cp[..., 0] = 255
+    cp[: ch // 6, :, 0] = 0
+    cp[ch // 2, : cw // 3, 0] = 100
+    return tp, cp
+
+
+@pytest.mark.parametrize("case", range(len(FORMAT_MATRIX)))
+def test_engine_format_matrix(case):
+    """engine() over the pixel formats the plugin hands it (lib/imageFormat.c:117-205):
+    grey or RGB colours, none/one/three map channels, alpha in the target, the corpus, both
+    or neither.  The serial mode equals the oracle byte for byte; the batched mode writes
+    only colour channels of target pixels, and only colours of corpus pixels."""
+    colors, maps, at, as_, patch, probes, mw = FORMAT_MATRIX[case]
+    tp, cp = _engine_pixmaps(colors, maps, at, as_, 500 + case)
+    p = R.default_params(patchSize=patch, maxProbeCount=probes, mapWeight=mw)
+    ix = R.prepare_indices(colors, maps, at, as_, maps > 0)
+    assert ix.total_bpp == tp.shape[2]
+    print("format case", case, FORMAT_MATRIX[case], "total_bpp", ix.total_bpp, flush=True)
+    rg, og = R.engine(p, ix, tp, cp, mode=R.MODE_SERIAL)
+    rc, oc = O.engine(_oparams(p), O.prepare_indices(colors, maps, at, as_, maps > 0), tp, cp)
+    assert rg.err == rc.err == 0
+    assert rg.nTargets == rc.nTargets > 0
+    assert rg.passes == rc.passes and rg.betters == rc.betters
+    assert rg.candidates == rc.scorerCalls
+    assert rg.pixelWordBytes == (4 if colors + maps <= 3 else 8)
+    assert np.array_equal(og, oc)
+
+    rb, ob = R.engine(p, ix, tp, cp, mode=R.MODE_BATCHED, seed=3)
+    assert rb.err == 0 and rb.nTargets == rg.nTargets
+    target = tp[..., 0] != 0
+    assert np.array_equal(ob[~target], tp[~target])
+    keep = [0] + list(range(1 + colors, tp.shape[2]))      # mask, alpha, maps are never written
+    assert np.array_equal(ob[..., keep], tp[..., keep])
+    # Random probes come from opaque selected corpus pixels, but heuristic-1 candidates are
+    # checked against the selection only (lib/engine.c:505-517), so a transparent selected
+    # corpus pixel can lend its colour too.
+    corpus = cp[..., 0] == 255
+    colours = {tuple(c) for c in cp[corpus][:, 1:1 + colors]}
+    assert all(tuple(c) in colours for c in ob[target][:, 1:1 + colors])
+    assert rb.meanError <= 1.3 * rg.meanError, (rb.meanError, rg.meanError)
 
 
 # ---------------------------------------------------------------- many requests in one launch
