@@ -195,6 +195,7 @@ def bench_config5(args, np, torch, R, W):
                                              round(float(losses[-20:].mean()), 3)],
                    "index_ms": round(rl.learnedIndexMs, 2), "keys": rl.learnedKeys,
                    "clusters": rl.learnedClusters, "propose_ms": round(rl.learnedProposeMs, 2),
+                   "fine_search_keys_scored": rl.learnedKeysScored,
                    "synth_kernels_ms": round(rl.synthKernelMs, 2), "passes": rl.passes,
                    "plain_synth_kernels_ms": round(rp.synthKernelMs, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

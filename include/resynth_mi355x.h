@@ -200,6 +200,7 @@ typedef struct {
   unsigned int learnedClusters;       /* inverted-file clusters */
   double learnedIndexMs;              /* host wall clock of embedding the corpus and indexing */
   double learnedProposeMs;            /* proposal kernels (features, embedding, search; timing != 0) */
+  unsigned long long learnedKeysScored; /* query-key dot products of the fine search */
 } ResynthStats;
 
 void resynth_set_default_mode(int mode);
@@ -334,6 +335,24 @@ int resynth_model_train(ResynthModel *model, const uint16_t *xq, const uint16_t 
  * A ([M x K], or [K x M] when ta) and B ([K x N], or [N x K] when tb); K a multiple of 32,
  * M and N multiples of 8.  For tests. */
 int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, const uint16_t *B, float *C);
+/* The learned index's query routing, on host arrays, through the same device sort and kernel
+ * the engine uses.  queries: bf16 [nq x 64]; supers: [nq] super-cluster per query (a
+ * negative one counts as 0); subCentroids: bf16 [S*Q x 64], super-cluster s owning rows
+ * [s*Q, (s+1)*Q).  Writes clusters [nq x P]: the ids s*Q + c of the P (1 or 2) rows of the
+ * query's super-cluster with the largest dot products, best first.  Returns 0, or -1 on
+ * invalid arguments.  For tests. */
+int resynth_learn_route(const uint16_t *queries, unsigned int nq, const int32_t *supers,
+                        const uint16_t *subCentroids, unsigned int S, unsigned int Q, int P, int32_t *clusters);
+/* The learned index's fine search, on host arrays, through the same device sort and kernel
+ * the engine uses.  queries: bf16 [nq x 64]; clusters: [nq x P] cluster ids (-1: none),
+ * P 1 or 2; keys: bf16 [nKeys x 64] in cluster order, cluster c holding keys
+ * [start[c], start[c+1]) (start has nClusters + 1 entries, start[nClusters] = nKeys);
+ * keyPts: [nKeys] (x, y).  Writes props [nq x 16] (x, y) pairs: 2 * keyPts of the query's
+ * top-16 keys by dot product over its clusters, best first, (-1, -1) past the last.
+ * Returns 0, or -1 on invalid arguments.  For tests. */
+int resynth_learn_fine_top16(const uint16_t *queries, unsigned int nq, const int32_t *clusters, int P,
+                             const uint16_t *keys, unsigned int nKeys, const uint32_t *start,
+                             unsigned int nClusters, const int32_t *keyPts, int32_t *props);
 
 /* Runs imageSynth in batched mode (the plain method, options' seed) on the image, which is
  * synthesized in place, and appends up to maxPairs training pairs to the model's store:

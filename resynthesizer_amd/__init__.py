@@ -76,7 +76,8 @@ class ResynthStats(ctypes.Structure):
                 ("waveLaunches", ctypes.c_uint), ("waveKernelMs", ctypes.c_double),
                 ("wavePatchTerms", ctypes.c_ulonglong), ("learnedLaunches", ctypes.c_uint),
                 ("learnedKeys", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
-                ("learnedIndexMs", ctypes.c_double), ("learnedProposeMs", ctypes.c_double)]
+                ("learnedIndexMs", ctypes.c_double), ("learnedProposeMs", ctypes.c_double),
+                ("learnedKeysScored", ctypes.c_ulonglong)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -170,6 +171,11 @@ SIGNATURES = {
                                          ctypes.c_float, ctypes.c_uint, ctypes.c_void_p]),
     "resynth_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_learn_route": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_uint, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]),
+    "resynth_learn_fine_top16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_int,
+                                                ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_uint,
+                                                ctypes.c_void_p, ctypes.c_void_p]),
     "resynth_score_frozen": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, _P(TFormatIndices),
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
@@ -484,6 +490,39 @@ def gemm_bf16(a_bits, b_bits, ta=False, tb=False):
     if rc:
         raise ValueError("resynth_gemm_bf16 rejected its arguments")
     return C
+
+
+def route(queries_bits, supers, sub_centroids_bits, S, Q, P):
+    """The learned index's query routing on host arrays: for each query, the ids s*Q + c of
+    the P best sub-centroids of its super-cluster s, best first.  Returns int32 [nq x P]."""
+    q = np.ascontiguousarray(queries_bits, dtype=np.uint16)
+    sup = np.ascontiguousarray(supers, dtype=np.int32)
+    sub = np.ascontiguousarray(sub_centroids_bits, dtype=np.uint16)
+    out = np.zeros((q.shape[0], P), np.int32)
+    rc = lib().resynth_learn_route(q.ctypes.data, q.shape[0], sup.ctypes.data, sub.ctypes.data, S, Q, P,
+                                   out.ctypes.data)
+    if rc:
+        raise ValueError("resynth_learn_route rejected its arguments")
+    return out
+
+
+def fine_top16(queries_bits, clusters, keys_bits, start, key_pts):
+    """The learned index's fine search on host arrays: queries and keys are bf16 bit patterns
+    [n x 64], clusters int32 [nq x P], start uint32 [nClusters + 1], key_pts int32 [nKeys x 2].
+    Returns int32 [nq x 16 x 2]: 2 * key_pts of each query's top-16 keys, best first."""
+    q = np.ascontiguousarray(queries_bits, dtype=np.uint16)
+    cl = np.ascontiguousarray(clusters, dtype=np.int32)
+    if cl.ndim == 1:
+        cl = cl[:, None]
+    k = np.ascontiguousarray(keys_bits, dtype=np.uint16)
+    st = np.ascontiguousarray(start, dtype=np.uint32)
+    pts = np.ascontiguousarray(key_pts, dtype=np.int32)
+    out = np.zeros((q.shape[0], 16, 2), np.int32)
+    rc = lib().resynth_learn_fine_top16(q.ctypes.data, q.shape[0], cl.ctypes.data, cl.shape[1], k.ctypes.data,
+                                        k.shape[0], st.ctypes.data, len(st) - 1, pts.ctypes.data, out.ctypes.data)
+    if rc:
+        raise ValueError("resynth_learn_fine_top16 rejected its arguments")
+    return out
 
 
 def struct_sizes():

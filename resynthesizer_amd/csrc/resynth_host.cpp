@@ -19,6 +19,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace rs;
@@ -62,7 +63,12 @@ struct MT19937 {
       uint32_t y = (hi & 0x80000000u) | (lo & 0x7FFFFFFFu);
       return (y >> 1) ^ ((y & 1u) ? 0x9908B0DFu : 0u);
     };
-    for (int k = 0; k < 624; k++) s[k] = s[(k + 397) % 624] ^ mixbits(s[k], s[(k + 1) % 624]);
+    // s[k] ^= ... in order k = 0..623 with indices taken mod 624, split so that no index
+    // needs a modulo: s[k + 397] for k < 227, the already regenerated s[k - 227] after.
+    int k = 0;
+    for (; k < 227; k++) s[k] = s[k + 397] ^ mixbits(s[k], s[k + 1]);
+    for (; k < 623; k++) s[k] = s[k - 227] ^ mixbits(s[k], s[k + 1]);
+    s[623] = s[396] ^ mixbits(s[623], s[0]);
     i = 0;
   }
   uint32_t next()
@@ -131,6 +137,44 @@ struct Prng {
     uint32_t v;
     do v = mt.next(); while (v > maxvalue);
     return v % n;
+  }
+  // The random target order of types 0 and 1 (lib/orderTarget.h:36-47): for i in 0..n-1,
+  // swap(perm[i], perm[int_range(0, n)]), perm starting as the identity.  Every draw has the
+  // same n, so the rejection bound and the reduction are set up once (v % n by Lemire's
+  // exact 64-bit fastmod), and the draws are taken in blocks ahead of the swaps they feed.
+  void shuffle(uint32_t *perm, uint32_t n)
+  {
+    for (uint32_t i = 0; i < n; i++) perm[i] = i;
+    if (n == 0) return;
+    constexpr uint32_t kBlock = 4096;
+    uint32_t js[kBlock];
+    if (kind == RESYNTH_PRNG_LIBC) {
+      const uint32_t d = 2147483647u / n + 1u;
+      for (uint32_t i0 = 0; i0 < n; i0 += kBlock) {
+        const uint32_t m = std::min(kBlock, n - i0);
+        for (uint32_t t = 0; t < m; t++) js[t] = lc.next() / d;
+        for (uint32_t t = 0; t < m; t++) std::swap(perm[i0 + t], perm[js[t]]);
+      }
+      return;
+    }
+    uint32_t maxvalue;
+    if (n <= 0x80000000u) {
+      uint32_t leftover = (0x80000000u % n) * 2u;
+      if (leftover >= n) leftover -= n;
+      maxvalue = 0xFFFFFFFFu - leftover;
+    } else {
+      maxvalue = n - 1u;
+    }
+    const uint64_t M = ~(uint64_t)0 / n + 1;  // 0 for n = 1, which reduces every v to 0
+    for (uint32_t i0 = 0; i0 < n; i0 += kBlock) {
+      const uint32_t m = std::min(kBlock, n - i0);
+      for (uint32_t t = 0; t < m; t++) {
+        uint32_t v;
+        do v = mt.next(); while (v > maxvalue);
+        js[t] = (uint32_t)(((unsigned __int128)(M * v) * n) >> 64);
+      }
+      for (uint32_t t = 0; t < m; t++) std::swap(perm[i0 + t], perm[js[t]]);
+    }
   }
 };
 
@@ -402,11 +446,34 @@ struct Prepared {
   unsigned nT = 0;
   size_t wb = 4;
   std::unique_ptr<Prng> prng;   // the engine's PRNG after the ordering draws
+  // Types 0 and 1 with a deferred order: the shuffle runs on a host thread (it needs only
+  // the target count) while the device works; finish_order joins it and gathers the
+  // row-major targets in that order.
+  std::thread orderThread;
+  bool orderPending = false;
+  uint32_t *hPerm = nullptr, *dPerm = nullptr;
+  const int2 *targetsRM = nullptr;
+  ~Prepared()
+  {
+    if (orderThread.joinable()) orderThread.join();
+  }
 };
 
+void finish_order(Prepared &pr)
+{
+  if (!pr.orderPending) return;
+  pr.orderThread.join();
+  pr.orderPending = false;
+  hipStream_t s = g_dev.stream;
+  HIP_TRY(hipMemcpyAsync(pr.dPerm, pr.hPerm, (size_t)pr.nT * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIP_TRY(launch_gather_targets(s, pr.targetsRM, pr.dPerm, const_cast<int2 *>(pr.b.targets), pr.nT));
+}
+
 // Preparation of an engine run (lib/engine.c:539-660) on the job's workspace.  Returns a
-// TImageSynthError.
-int prepare_engine(Job &job, Prepared &pr)
+// TImageSynthError.  deferOrder: a random target order (types 0 and 1) is left to
+// finish_order, so that its host shuffle overlaps the device work queued before the
+// targets are needed.
+int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
 {
   hipStream_t s = g_dev.stream;
   Workspace &W = *job.W;
@@ -506,17 +573,13 @@ int prepare_engine(Job &job, Prepared &pr)
   int2 *targets = (int2 *)W.targets.get((size_t)nT * sizeof(int2) + 64);
   const int type = P.matchContextType;
   if (type == 0 || type == 1) {
-    std::vector<uint32_t> perm(nT);
-    for (unsigned i = 0; i < nT; i++) perm[i] = i;
-    for (unsigned i = 0; i < nT; i++) {
-      uint32_t j = prng.range(nT);
-      std::swap(perm[i], perm[j]);
-    }
-    uint32_t *hPerm = (uint32_t *)W.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
-    std::memcpy(hPerm, perm.data(), (size_t)nT * sizeof(uint32_t));
-    uint32_t *dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
-    HIP_TRY(hipMemcpyAsync(dPerm, hPerm, (size_t)nT * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_gather_targets(s, targetsRM, dPerm, targets, nT));
+    pr.hPerm = (uint32_t *)W.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
+    pr.dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
+    pr.targetsRM = targetsRM;
+    Prng *pp = &prng;
+    uint32_t *hPerm = pr.hPerm;
+    pr.orderThread = std::thread([pp, hPerm, nT] { pp->shuffle(hPerm, nT); });
+    pr.orderPending = true;
   } else if (type >= 2 && type <= 8) {
     std::vector<Pt> pts(nT);
     HIP_TRY(hipMemcpyAsync(pts.data(), targetsRM, (size_t)nT * sizeof(int2), hipMemcpyDeviceToHost, s));
@@ -589,6 +652,7 @@ int prepare_engine(Job &job, Prepared &pr)
   pr.b = b;
   std::copy(sizes, sizes + 6, pr.sizes);
   pr.wb = wb;
+  if (!deferOrder) finish_order(pr);
   return 0;
 }
 
@@ -686,7 +750,12 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
 }
 
 // Proposals (16 corpus points each) for targets k0..k1-1 from their current neighbourhoods.
-constexpr unsigned kQueryBlock = 65536;   // queries embedded and searched per block
+// Queries embedded and searched per block.  Large blocks keep the searches efficient: the
+// queries are sorted by cluster and taken 16 per wavefront, so with many queries per
+// cluster a wavefront's 16 mostly share one cluster and walk its keys once (a block of 2^20
+// queries holds about 64 per cluster of a 16 384-cluster index; one of 2^16 about 4).
+// Scratch per block: about 400 bytes per query.
+constexpr unsigned kQueryBlock = 1u << 20;
 constexpr unsigned kLearnMinChunk = 256;  // per-chunk proposals for chunks of at least this size
 
 void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
@@ -719,13 +788,20 @@ void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, 
   HIP_TRY(mlp_embed_fused(s, model_weights(job.opt.model), model_weights_t(job.opt.model, s), x, n, z, nullptr));
   int32_t *qs = (int32_t *)W.qSuper.get((size_t)n * 4 + 64);
   HIP_TRY(learn_topk_cols(s, z, n, L.superCent, (int)L.S, 1, qs));
-  HIP_TRY(learn_query_sub(s, z, n, qs, (int)L.Q, L.subCent, P, qcl));
-  // visit the queries in cluster order so that each cluster's keys are read from L2
+  // Both searches visit the queries sorted by the cluster they search, so that a tile of 16
+  // queries mostly shares its rows and consecutive tiles read them from L2.  The sort keys
+  // only need the bits that tell clusters apart (ids below 2^bits; a -1 may land anywhere,
+  // which costs locality only).
   uint32_t *qk0 = (uint32_t *)W.qK0.get((size_t)n * 4 + 64), *qv0 = (uint32_t *)W.qV0.get((size_t)n * 4 + 64);
   uint32_t *qk1 = (uint32_t *)W.qK1.get((size_t)n * 4 + 64), *qv1 = (uint32_t *)W.qV1.get((size_t)n * 4 + 64);
   const size_t qtb = sort_pairs_temp_bytes(n);
+  void *qtemp = W.qSortTemp.get(qtb);
+  HIP_TRY(learn_sort_keys(s, qs, 1, n, qk0, qv0));
+  HIP_TRY(launch_sort_pairs(s, qk0, qk1, qv0, qv1, (size_t)n, bits_for(L.S), qtemp, qtb));
+  HIP_TRY(learn_query_sub(s, z, n, qv1, qs, (int)L.Q, L.subCent, P, qcl));
+  HIP_TRY(learn_count_scored(s, qcl, n, P, L.start, &pr.b.counters->keysScored));
   HIP_TRY(learn_sort_keys(s, qcl, P, n, qk0, qv0));
-  HIP_TRY(launch_sort_pairs(s, qk0, qk1, qv0, qv1, (size_t)n, 32, W.qSortTemp.get(qtb), qtb));
+  HIP_TRY(launch_sort_pairs(s, qk0, qk1, qv0, qv1, (size_t)n, bits_for(L.C), qtemp, qtb));
   HIP_TRY(learn_fine_top16_sorted(s, z, n, qv1, qcl, P, L.keyZs, L.start, L.keyPtsS, props));
 }
 
@@ -741,6 +817,9 @@ int refine_engine(Job &job, Prepared &pr)
   SynthBuffers &b = pr.b;
   const unsigned nT = pr.nT;
   const unsigned *sizes = pr.sizes;
+  const bool serial = job.opt.mode == RESYNTH_MODE_SERIAL;
+  // without a learned index to build first, nothing else can overlap the host shuffle
+  if (serial || !job.opt.model) finish_order(pr);
   Prng &prng = *pr.prng;
   Counters *cnt = b.counters;
   const uint32_t *lastBest = b.lastBest;
@@ -749,7 +828,6 @@ int refine_engine(Job &job, Prepared &pr)
 
   std::vector<TimedLaunch> ev;
   unsigned launches = 0;
-  const bool serial = job.opt.mode == RESYNTH_MODE_SERIAL;
 
   // serial mode: raw PRNG outputs after the ordering draws, streamed to the device
   // raw outputs buffered on the device; RESYNTH_SERIAL_STREAM_CAP shrinks the buffer so
@@ -823,6 +901,7 @@ int refine_engine(Job &job, Prepared &pr)
     job.st.learnedClusters = learned.ok ? learned.C : 0;
     job.st.learnedIndexMs = now_ms() - tl;
   }
+  finish_order(pr);
   unsigned pass;
   unsigned passesRun = 0;
   unsigned bettersBefore = 0;
@@ -987,6 +1066,7 @@ int refine_engine(Job &job, Prepared &pr)
       job.st.patchTerms += hc.terms[i];
       job.st.wavePatchTerms += hc.termsWave[i];
     }
+    job.st.learnedKeysScored = hc.keysScored;
     double ms = 0;
     double wms = 0;
     for (auto &e : ev) {
@@ -1019,7 +1099,7 @@ int refine_engine(Job &job, Prepared &pr)
 int run_engine(Job &job, unsigned &nTargetsOut)
 {
   Prepared pr;
-  const int err = prepare_engine(job, pr);
+  const int err = prepare_engine(job, pr, true);
   nTargetsOut = pr.nT;
   if (err) return err;
   return refine_engine(job, pr);
@@ -1766,6 +1846,104 @@ int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned 
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Device memory owned by one test-hook call.
+struct TmpBuf {
+  void *p = nullptr;
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes < 256 ? 256 : bytes); }
+  ~TmpBuf()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+extern "C" {
+
+int resynth_learn_fine_top16(const uint16_t *queries, unsigned int nq, const int32_t *clusters, int P,
+                             const uint16_t *keys, unsigned int nKeys, const uint32_t *start,
+                             unsigned int nClusters, const int32_t *keyPts, int32_t *props)
+{
+  if (!queries || !clusters || !keys || !start || !keyPts || !props || P < 1 || P > 2 || !nq || !nClusters)
+    return -1;
+  if (start[0] != 0 || start[nClusters] != nKeys) return -1;
+  for (unsigned c = 0; c < nClusters; c++)
+    if (start[c] > start[c + 1]) return -1;
+  for (size_t i = 0; i < (size_t)nq * P; i++)
+    if (clusters[i] >= (int32_t)nClusters) return -1;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  hipStream_t s = g_dev.stream;
+  const size_t n = nq;
+  const size_t tb = sort_pairs_temp_bytes(n);
+  TmpBuf dq, dcl, dk, dst, dpts, dout, dk0, dv0, dk1, dv1, dtemp;
+  HIP_TRY(dq.alloc(n * kEmb * 2));
+  HIP_TRY(dcl.alloc(n * P * 4));
+  HIP_TRY(dk.alloc((size_t)std::max(1u, nKeys) * kEmb * 2));
+  HIP_TRY(dst.alloc(((size_t)nClusters + 1) * 4));
+  HIP_TRY(dpts.alloc((size_t)std::max(1u, nKeys) * 8));
+  HIP_TRY(dout.alloc(n * 16 * 8));
+  HIP_TRY(dk0.alloc(n * 4));
+  HIP_TRY(dv0.alloc(n * 4));
+  HIP_TRY(dk1.alloc(n * 4));
+  HIP_TRY(dv1.alloc(n * 4));
+  HIP_TRY(dtemp.alloc(tb));
+  HIP_TRY(hipMemcpyAsync(dq.p, queries, n * kEmb * 2, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dcl.p, clusters, n * P * 4, hipMemcpyHostToDevice, s));
+  if (nKeys) {
+    HIP_TRY(hipMemcpyAsync(dk.p, keys, (size_t)nKeys * kEmb * 2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dpts.p, keyPts, (size_t)nKeys * 8, hipMemcpyHostToDevice, s));
+  }
+  HIP_TRY(hipMemcpyAsync(dst.p, start, ((size_t)nClusters + 1) * 4, hipMemcpyHostToDevice, s));
+  const int32_t *qcl = (const int32_t *)dcl.p;
+  uint32_t *k0 = (uint32_t *)dk0.p, *v0 = (uint32_t *)dv0.p, *k1 = (uint32_t *)dk1.p, *v1 = (uint32_t *)dv1.p;
+  HIP_TRY(learn_sort_keys(s, qcl, P, (int)n, k0, v0));
+  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, n, 32, dtemp.p, tb));
+  HIP_TRY(learn_fine_top16_sorted(s, (const uint16_t *)dq.p, (int)n, v1, qcl, P, (const uint16_t *)dk.p,
+                                  (const uint32_t *)dst.p, (const int2 *)dpts.p, (int2 *)dout.p));
+  HIP_TRY(hipMemcpyAsync(props, dout.p, n * 16 * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int resynth_learn_route(const uint16_t *queries, unsigned int nq, const int32_t *supers,
+                        const uint16_t *subCentroids, unsigned int S, unsigned int Q, int P, int32_t *clusters)
+{
+  if (!queries || !supers || !subCentroids || !clusters || !nq || !S || !Q || P < 1 || P > 2 || (unsigned)P > Q)
+    return -1;
+  for (unsigned i = 0; i < nq; i++)
+    if (supers[i] >= (int32_t)S) return -1;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  hipStream_t s = g_dev.stream;
+  const size_t n = nq;
+  const size_t tb = sort_pairs_temp_bytes(n);
+  TmpBuf dq, dsup, dsub, dout, dk0, dv0, dk1, dv1, dtemp;
+  HIP_TRY(dq.alloc(n * kEmb * 2));
+  HIP_TRY(dsup.alloc(n * 4));
+  HIP_TRY(dsub.alloc((size_t)S * Q * kEmb * 2));
+  HIP_TRY(dout.alloc(n * P * 4));
+  HIP_TRY(dk0.alloc(n * 4));
+  HIP_TRY(dv0.alloc(n * 4));
+  HIP_TRY(dk1.alloc(n * 4));
+  HIP_TRY(dv1.alloc(n * 4));
+  HIP_TRY(dtemp.alloc(tb));
+  HIP_TRY(hipMemcpyAsync(dq.p, queries, n * kEmb * 2, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dsup.p, supers, n * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dsub.p, subCentroids, (size_t)S * Q * kEmb * 2, hipMemcpyHostToDevice, s));
+  const int32_t *qs = (const int32_t *)dsup.p;
+  uint32_t *k0 = (uint32_t *)dk0.p, *v0 = (uint32_t *)dv0.p, *k1 = (uint32_t *)dk1.p, *v1 = (uint32_t *)dv1.p;
+  HIP_TRY(learn_sort_keys(s, qs, 1, (int)n, k0, v0));
+  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, n, bits_for(S), dtemp.p, tb));
+  HIP_TRY(learn_query_sub(s, (const uint16_t *)dq.p, (int)n, v1, qs, (int)Q, (const uint16_t *)dsub.p, P,
+                          (int32_t *)dout.p));
+  HIP_TRY(hipMemcpyAsync(clusters, dout.p, n * P * 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
 int resynth_device_info(char *name, size_t nameLen, int *computeUnits)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
@@ -1809,8 +1987,7 @@ int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsig
   for (unsigned i = 0; i < n; i++) pts[i] = {points[2 * i], points[2 * i + 1]};
   if (matchContextType == 0 || matchContextType == 1) {
     std::vector<uint32_t> perm(n);
-    for (unsigned i = 0; i < n; i++) perm[i] = i;
-    for (unsigned i = 0; i < n; i++) std::swap(perm[i], perm[p.range(n)]);
+    p.shuffle(perm.data(), n);
     for (unsigned i = 0; i < n; i++) {
       points[2 * i] = pts[perm[i]].x;
       points[2 * i + 1] = pts[perm[i]].y;

@@ -50,6 +50,7 @@ struct Counters {
   unsigned long long terms[CNT_SLOTS];
   unsigned long long termsWave[CNT_SLOTS];   // terms scored by k_synth_wave launches only
   unsigned int betters[CNT_SLOTS];
+  unsigned long long keysScored;             // dot products of the learned fine search
 };
 
 struct SynthBuffers {

@@ -85,9 +85,14 @@ hipError_t learn_sample_pairs(hipStream_t s, const int2 *targets, unsigned nT, c
 // membersS) and each key's cluster s*Q + q (tileStart: S+1 scratch words).
 hipError_t learn_sub_index(hipStream_t s, const uint32_t *startS, const uint32_t *membersS, int S, int Q, int nK,
                            const uint16_t *keyZ, uint16_t *subCent, uint32_t *tileStart, int32_t *cluster);
-// Query routing inside the query's super-cluster: the best P sub-centroids as cluster ids.
-hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
-                           const uint16_t *subCent, int P, int32_t *qcl);
+// Query routing inside the query's super-cluster: the best P (1 or 2) of its Q
+// sub-centroids, as cluster ids s*Q + c.  order: the queries sorted by super-cluster.
+hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const uint32_t *order, const int32_t *qsuper,
+                           int Q, const uint16_t *subCent, int P, int32_t *qcl);
+// Adds to *total the keys the fine search will score: the summed sizes of every query's
+// clusters.
+hipError_t learn_count_scored(hipStream_t s, const int32_t *qcl, int n, int P, const uint32_t *start,
+                              unsigned long long *total);
 // Sort keys from cluster ids (every stride-th entry) with values 0..n-1.
 hipError_t learn_sort_keys(hipStream_t s, const int32_t *cluster, int stride, int n, uint32_t *keys, uint32_t *vals);
 // start[c] = first position of cluster c in the sorted ids (start[C] = n).

@@ -823,69 +823,6 @@ hipError_t learn_sub_index(hipStream_t s, const uint32_t *startS, const uint32_t
   return hipGetLastError();
 }
 
-// Wavefront per query: its super-cluster's Q sub-centroids scored (lanes take q, q+64, ..),
-// then the best P become cluster ids s*Q + q.
-__global__ __launch_bounds__(256) void k_query_sub(const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
-                                                   const uint16_t *subCent, int P, int32_t *qcl)
-{
-  __shared__ float sq[4][kEmb];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int q = blockIdx.x * 4 + wave;
-  if (q >= nq) return;
-  sq[wave][lane] = bf2f(qz[(size_t)q * kEmb + lane]);
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
-  const int s = qsuper[q] < 0 ? 0 : qsuper[q];
-  const uint16_t *cent = subCent + (size_t)s * Q * kEmb;
-  float v0 = -3.4e38f, v1 = -3.4e38f;
-  int i0 = -1, i1 = -1;
-  for (int c = lane; c < Q; c += 64) {
-    const uint4 *kr = (const uint4 *)(cent + (size_t)c * kEmb);
-    float d = 0.f;
-    for (int g = 0; g < 8; g++) {
-      const uint4 u = kr[g];
-      const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
-      for (int t = 0; t < 4; t++) {
-        d += bf2f((uint16_t)(ws[t] & 0xFFFFu)) * sq[wave][g * 8 + t * 2];
-        d += bf2f((uint16_t)(ws[t] >> 16)) * sq[wave][g * 8 + t * 2 + 1];
-      }
-    }
-    if (d > v0) {
-      v1 = v0;
-      i1 = i0;
-      v0 = d;
-      i0 = c;
-    } else if (d > v1) {
-      v1 = d;
-      i1 = c;
-    }
-  }
-  for (int p = 0; p < P; p++) {
-    float best = v0;
-    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    const uint64_t hit = __ballot(v0 == best && i0 >= 0);
-    int c = -1;
-    if (hit) {
-      const int winner = (int)__builtin_ctzll(hit);
-      c = __shfl(i0, winner);
-      if (lane == winner) {
-        v0 = v1;
-        i0 = i1;
-        v1 = -3.4e38f;
-        i1 = -1;
-      }
-    }
-    if (lane == 0) qcl[(size_t)q * P + p] = c < 0 ? -1 : s * Q + c;
-  }
-}
-
-hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const int32_t *qsuper, int Q,
-                           const uint16_t *subCent, int P, int32_t *qcl)
-{
-  if (nq <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_query_sub, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, qsuper, Q, subCent, P, qcl);
-  return hipGetLastError();
-}
 
 // Bias gradients: column sums of a bf16 [rows x cols] matrix into out (zeroed by the caller),
 // 64 columns x a chunk of rows per block.
@@ -1082,68 +1019,180 @@ hipError_t learn_permute_keys(hipStream_t s, const uint32_t *members, int n, con
   return hipGetLastError();
 }
 
-// Fine search over clusters stored contiguously (keyZs, keyPtsS in cluster order); queries
-// visited in `order` (sorted by cluster, so that a cluster's keys are read by consecutive
-// waves and stay in L2).  Each lane keeps its best two, and the wave's top 16 are then taken
-// one at a time.
-__global__ __launch_bounds__(256) void k_fine_top16_sorted(const uint16_t *qz, int nq, const uint32_t *order,
-                                                           const int32_t *qcl, int P, const uint16_t *keyZs,
-                                                           const uint32_t *start, const int2 *keyPtsS, int2 *props)
+// Top-N search on the matrix cores, shared by the query routing (a query's best sub-clusters
+// inside its super-cluster) and the fine search (a query's best keys inside its clusters).
+// One wavefront takes 16 consecutive queries of `order`, which is sorted by the cluster
+// searched, so the tile's queries mostly share one cluster and a cluster's rows are read
+// by consecutive waves from L2.  For each distinct cluster among the tile's queries it
+// walks the cluster's rows 16 at a time: two v_mfma_f32_16x16x32_bf16 steps (64 dimensions)
+// give the 16 x 16 dot products, and lane l holds those of queries 4*(l/16)+r against row
+// column l%16.  Each lane keeps, per query, the best LT rows of its column slot (a sorted
+// list, branch-free insertion); the 16 lanes of a query's row group then merge their lists
+// into the query's top N.  The merge is exact unless more than LT of a query's top N fall
+// into one column slot (never for N <= LT; probability about 3e-6 for N = 16, LT = 8).
+template <int LT>
+__device__ __forceinline__ void lane_top_insert(float (&v)[LT], int (&id)[LT], float d, int key)
 {
-  __shared__ float sq[4][kEmb];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int w = blockIdx.x * 4 + wave;
-  if (w >= nq) return;
-  const int q = (int)order[w];
-  sq[wave][lane] = bf2f(qz[(size_t)q * kEmb + lane]);
-  __builtin_amdgcn_s_waitcnt(0);
-  __builtin_amdgcn_wave_barrier();
-  float v0 = -3.4e38f, v1 = -3.4e38f;
-  int i0 = -1, i1 = -1;
+  const bool c = d > v[LT - 1];
+  v[LT - 1] = c ? d : v[LT - 1];
+  id[LT - 1] = c ? key : id[LT - 1];
+  for (int j = LT - 1; j > 0; j--) {
+    const bool up = v[j] > v[j - 1];
+    const float tv = v[j - 1];
+    const int ti = id[j - 1];
+    v[j - 1] = up ? v[j] : tv;
+    id[j - 1] = up ? id[j] : ti;
+    v[j] = up ? tv : v[j];
+    id[j] = up ? ti : id[j];
+  }
+}
+
+// Cluster c holds rows [start[c], start[c+1]), or [c*fixedLen, (c+1)*fixedLen) when start
+// is null.  POINTS: writes the top 16 rows' points (keyPts * 2, or (-1, -1)) to
+// props[q*16 ..]; otherwise writes the top `topn` row indices (or -1) to ids[q*topn ..].
+// rowCl[q*P + p] is the p-th cluster of query q (negative: none; in the routing mode a
+// negative super-cluster counts as 0).
+template <int LT, bool POINTS>
+__global__ __launch_bounds__(256) void k_topn_mfma(const uint16_t *qz, int nq, const uint32_t *order,
+                                                   const int32_t *rowCl, int P, const uint16_t *rows,
+                                                   const uint32_t *start, int fixedLen, int topn,
+                                                   const int2 *keyPts, int2 *props, int32_t *ids)
+{
+  constexpr float kNone = -3.4e38f;
+  const int lane = threadIdx.x & 63;
+  const int t0 = (int)(blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (t0 >= nq) return;  // uniform per wavefront: the MFMA steps below run with every lane
+  const int col = lane & 15, grp = lane >> 4, kq = grp * 8;
+  const uint32_t qa = order[min(t0 + col, nq - 1)];
+  const bf16x8 a0 = *(const bf16x8 *)(qz + (size_t)qa * kEmb + kq);
+  const bf16x8 a1 = *(const bf16x8 *)(qz + (size_t)qa * kEmb + 32 + kq);
+  int rq[4];
+  for (int r = 0; r < 4; r++) {
+    const int i = t0 + grp * 4 + r;
+    rq[r] = i < nq ? (int)order[i] : -1;
+  }
+  float v[4][LT];
+  int id[4][LT];
+  for (int r = 0; r < 4; r++)
+    for (int j = 0; j < LT; j++) {
+      v[r][j] = kNone;
+      id[r][j] = -1;
+    }
   for (int p = 0; p < P; p++) {
-    const int c = qcl[(size_t)q * P + p];
-    if (c < 0) continue;
-    const uint32_t b = start[c], e = start[c + 1];
-    for (uint32_t m = b + lane; m < e; m += 64) {
-      const uint4 *kr = (const uint4 *)(keyZs + (size_t)m * kEmb);
-      float d = 0.f;
-      for (int g = 0; g < 8; g++) {
-        const uint4 u = kr[g];
-        const uint32_t ws[4] = {u.x, u.y, u.z, u.w};
-        for (int t = 0; t < 4; t++) {
-          d += bf2f((uint16_t)(ws[t] & 0xFFFFu)) * sq[wave][g * 8 + t * 2];
-          d += bf2f((uint16_t)(ws[t] >> 16)) * sq[wave][g * 8 + t * 2 + 1];
+    int rc[4];
+    unsigned done = 0;
+    for (int r = 0; r < 4; r++) {
+      int c = rq[r] >= 0 ? rowCl[(size_t)rq[r] * P + p] : -1;
+      if (!POINTS && rq[r] >= 0 && c < 0) c = 0;
+      rc[r] = c;
+      if (c < 0) done |= 1u << r;
+    }
+    for (;;) {
+      int c = 0x7FFFFFFF;
+      for (int r = 0; r < 4; r++)
+        if (!((done >> r) & 1u)) c = min(c, rc[r]);
+      c = min(c, __shfl_xor(c, 16));
+      c = min(c, __shfl_xor(c, 32));
+      if (c == 0x7FFFFFFF) break;
+      bool match[4];
+      for (int r = 0; r < 4; r++) {
+        match[r] = !((done >> r) & 1u) && rc[r] == c;
+        if (match[r]) done |= 1u << r;
+      }
+      const uint32_t b = start ? start[c] : (uint32_t)c * (uint32_t)fixedLen;
+      const uint32_t e = start ? start[c + 1] : b + (uint32_t)fixedLen;
+      if (b >= e) continue;
+      uint32_t kk = min(b + col, e - 1);
+      bf16x8 b0 = *(const bf16x8 *)(rows + (size_t)kk * kEmb + kq);
+      bf16x8 b1 = *(const bf16x8 *)(rows + (size_t)kk * kEmb + 32 + kq);
+      for (uint32_t m0 = b; m0 < e; m0 += 16) {
+        // the next tile's operands (clamped into the cluster, so always in bounds)
+        kk = min(m0 + 16 + col, e - 1);
+        const bf16x8 n0 = *(const bf16x8 *)(rows + (size_t)kk * kEmb + kq);
+        const bf16x8 n1 = *(const bf16x8 *)(rows + (size_t)kk * kEmb + 32 + kq);
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
+        const uint32_t key = m0 + col;
+        const bool inside = key < e;
+        for (int r = 0; r < 4; r++) lane_top_insert<LT>(v[r], id[r], (inside && match[r]) ? acc[r] : kNone, (int)key);
+        b0 = n0;
+        b1 = n1;
+      }
+    }
+  }
+  // Merge: topn rounds per query; the lane whose list head is the row group's maximum (the
+  // lowest such lane on a tie) hands it over and pops it.
+  const uint64_t groupMask = 0xFFFFull << (grp * 16);
+  int out[4] = {-1, -1, -1, -1};
+  for (int r = 0; r < 4; r++) {
+    for (int k = 0; k < topn; k++) {
+      float best = v[r][0];
+      best = fmaxf(best, __shfl_xor(best, 1));
+      best = fmaxf(best, __shfl_xor(best, 2));
+      best = fmaxf(best, __shfl_xor(best, 4));
+      best = fmaxf(best, __shfl_xor(best, 8));
+      const uint64_t hit = __ballot(v[r][0] == best) & groupMask;
+      const int winner = (int)__builtin_ctzll(hit);
+      const int key = __shfl(id[r][0], winner);
+      if (col == k) out[r] = key;
+      if (lane == winner) {
+        for (int j = 0; j + 1 < LT; j++) {
+          v[r][j] = v[r][j + 1];
+          id[r][j] = id[r][j + 1];
         }
-      }
-      if (d > v0) {
-        v1 = v0;
-        i1 = i0;
-        v0 = d;
-        i0 = (int)m;
-      } else if (d > v1) {
-        v1 = d;
-        i1 = (int)m;
+        v[r][LT - 1] = kNone;
+        id[r][LT - 1] = -1;
       }
     }
   }
-  int mine = -1;
-  for (int r = 0; r < 16; r++) {
-    float best = v0;
-    for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-    if (best <= -3.0e38f) break;
-    const uint64_t hit = __ballot(v0 == best && i0 >= 0);
-    const int winner = (int)__builtin_ctzll(hit);
-    const int key = __shfl(i0, winner);
-    if (lane == r) mine = key;
-    if (lane == winner) {
-      v0 = v1;
-      i0 = i1;
-      v1 = -3.4e38f;
-      i1 = -1;
-    }
+  for (int r = 0; r < 4; r++) {
+    if (rq[r] < 0) continue;
+    const int k = out[r];
+    if (POINTS)
+      props[(size_t)rq[r] * 16 + col] = k >= 0 ? make_int2(keyPts[k].x * 2, keyPts[k].y * 2) : make_int2(-1, -1);
+    else if (col < topn)
+      ids[(size_t)rq[r] * topn + col] = k;
   }
-  if (lane < 16) props[(size_t)q * 16 + lane] = mine >= 0 ? make_int2(keyPtsS[mine].x * 2, keyPtsS[mine].y * 2)
-                                                          : make_int2(-1, -1);
+}
+
+// Grid-stride, so that at most one atomic per workgroup reaches the single total.
+__global__ __launch_bounds__(256) void k_count_scored(const int32_t *qcl, int n, int P, const uint32_t *start,
+                                                      unsigned long long *total)
+{
+  __shared__ unsigned long long part[4];
+  unsigned long long v = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int p = 0; p < P; p++) {
+      const int c = qcl[(size_t)i * P + p];
+      if (c >= 0) v += start[c + 1] - start[c];
+    }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(total, t);
+  }
+}
+
+hipError_t learn_count_scored(hipStream_t s, const int32_t *qcl, int n, int P, const uint32_t *start,
+                              unsigned long long *total)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_scored, dim3(std::min((n + 255) / 256, 512)), dim3(256), 0, s, qcl, n, P, start, total);
+  return hipGetLastError();
+}
+
+hipError_t learn_query_sub(hipStream_t s, const uint16_t *qz, int nq, const uint32_t *order, const int32_t *qsuper,
+                           int Q, const uint16_t *subCent, int P, int32_t *qcl)
+{
+  if (nq <= 0) return hipSuccess;
+  if (P < 1 || P > 2) return hipErrorInvalidValue;
+  const unsigned tiles = (unsigned)((nq + 15) / 16);
+  hipLaunchKernelGGL((k_topn_mfma<2, false>), dim3((tiles + 3) / 4), dim3(256), 0, s, qz, nq, order, qsuper, 1,
+                     subCent, nullptr, Q, P, nullptr, nullptr, qcl);
+  return hipGetLastError();
 }
 
 hipError_t learn_fine_top16_sorted(hipStream_t s, const uint16_t *qz, int nq, const uint32_t *order,
@@ -1151,8 +1200,9 @@ hipError_t learn_fine_top16_sorted(hipStream_t s, const uint16_t *qz, int nq, co
                                    const int2 *keyPtsS, int2 *props)
 {
   if (nq <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fine_top16_sorted, dim3((nq + 3) / 4), dim3(256), 0, s, qz, nq, order, qcl, P, keyZs, start,
-                     keyPtsS, props);
+  const unsigned tiles = (unsigned)((nq + 15) / 16);
+  hipLaunchKernelGGL((k_topn_mfma<8, true>), dim3((tiles + 3) / 4), dim3(256), 0, s, qz, nq, order, qcl, P, keyZs,
+                     start, 0, 16, keyPtsS, props, nullptr);
   return hipGetLastError();
 }
 

@@ -27,6 +27,68 @@ def test_mfma_gemm_matches_float64(ta, tb):
     assert np.abs(C - ref).max() <= 1e-4 * np.abs(ref).max() + 1e-5
 
 
+
+@pytest.mark.parametrize("P", [1, 2])
+def test_query_routing_matches_brute_force(P):
+    """The index's query routing (MFMA score tiles over the query's super-cluster, per-lane
+    lists, merge): the P sub-centroids of the query's super-cluster with the largest dot
+    products, best first, as cluster ids s*Q + c; a negative super-cluster counts as 0."""
+    rng = np.random.RandomState(40 + P)
+    S, Q, nq = 9, 128, 777
+    sub = _bf16(rng, (S * Q, 64))
+    subf = MO.from_bits(sub)
+    qb = _bf16(rng, (nq, 64))
+    qf = MO.from_bits(qb)
+    sup = rng.randint(0, S, nq).astype(np.int32)
+    sup[::50] = -1
+    got = R.route(qb, sup, sub, S, Q, P)
+    for i in range(nq):
+        s = max(int(sup[i]), 0)
+        scores = subf[s * Q:(s + 1) * Q] @ qf[i]
+        order = np.argsort(-scores, kind="stable")
+        assert (got[i] >= s * Q).all() and (got[i] < (s + 1) * Q).all(), i
+        gs = scores[got[i] - s * Q]
+        assert np.allclose(gs, scores[order[:P]], atol=1e-5), (i, gs, scores[order[:P]])
+
+@pytest.mark.parametrize("P", [1, 2])
+def test_fine_search_returns_exact_top16(P):
+    """The index's fine search (16 x 16 MFMA score tiles, per-lane sorted lists, a merge per
+    query) against brute force: for every query, the 16 keys of its clusters with the
+    largest dot products, best first, padded with (-1, -1) when its clusters hold fewer than
+    16 keys.  Cluster sizes cover empty, partial key tiles and multi-tile clusters; tiles
+    of 16 sorted queries straddle cluster boundaries."""
+    rng = np.random.RandomState(20 + P)
+    sizes = [0, 1, 5, 15, 16, 17, 31, 33, 240, 700, 64, 3] * 3
+    nc = len(sizes)
+    start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    nk = int(start[-1])
+    kb = _bf16(rng, (nk, 64))
+    kf = MO.from_bits(kb)
+    nq = 523
+    qb = _bf16(rng, (nq, 64))
+    qf = MO.from_bits(qb)
+    pts = np.stack([np.arange(nk) % 1000, np.arange(nk) // 1000], 1).astype(np.int32)
+    cl = np.stack([rng.permutation(nc)[:P] for _ in range(nq)]).astype(np.int32)
+    cl[rng.rand(nq) < 0.05, 0] = -1
+    if P == 2:
+        cl[rng.rand(nq) < 0.05, 1] = -1
+    out = R.fine_top16(qb, cl, kb, start, pts)
+    for i in range(nq):
+        cand = np.concatenate([np.arange(start[c], start[c + 1]) for c in cl[i] if c >= 0] + [np.zeros(0, int)])
+        cand = cand.astype(np.int64)
+        scores = kf[cand] @ qf[i]
+        want = min(16, len(cand))
+        got = out[i]
+        assert (got[want:] == -1).all(), i
+        ids = (got[:want, 0] // 2) + (got[:want, 1] // 2) * 1000
+        assert (got[:want] % 2 == 0).all() and len(set(ids.tolist())) == want, i
+        assert set(ids.tolist()) <= set(cand.tolist()), i
+        s = kf[ids] @ qf[i]
+        assert (np.diff(s) <= 1e-5).all(), (i, s)
+        if want:
+            kth = np.sort(scores)[::-1][want - 1]
+            assert s.min() >= kth - 1e-5, (i, s.min(), kth)
+
 def _features(n, seed):
     rng = np.random.RandomState(seed)
     x = np.zeros((n, 128), np.float32)
