@@ -217,6 +217,10 @@ __device__ __forceinline__ unsigned scan_start(const Geometry &g, const uint8_t 
   return j < 1u ? 1u : j;
 }
 
+// Steps of the workgroup offset scan taken per round once the first step has not filled
+// the patch.
+constexpr unsigned kScanBatch = 4;
+
 // Neighbour patch of target (tx,ty) (lib/synthesize.h:189-241): neighbour 0 is the
 // target itself; then the sorted offsets are scanned 64 at a time, keeping points in
 // the target image (or wrapped into it when tiling) that have a value.
@@ -288,6 +292,7 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
 {
   const unsigned lane = lane_id();
   bool valid = false;
+  uint32_t lastProber = k;
   cx = 0;
   cy = 0;
   if (lane < count) {
@@ -296,7 +301,10 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
       cx = src.x - nbOff[lane].x;
       cy = src.y - nbOff[lane].y;
       if ((unsigned)cx < (unsigned)g.cw && (unsigned)cy < (unsigned)g.ch) {
-        Word q = cword[(size_t)cy * (unsigned)g.cw + (unsigned)cx];
+        // the candidate's word and its recent-prober entry, loaded together
+        const size_t ci = (size_t)cy * (unsigned)g.cw + (unsigned)cx;
+        Word q = cword[ci];
+        lastProber = prober[ci];
         valid = (q & 0xFEu) == 0xFEu;
       }
     }
@@ -310,8 +318,7 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
       if (j < lane && ((vmask >> j) & 1ull) && ox == cx && oy == cy) dup = true;
     }
   }
-  bool active = valid && !dup;
-  if (active) active = prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] != k;
+  const bool active = valid && !dup && lastProber != k;
   uint32_t F = 0, M = 0;
   bool alive = false;
   if (active)
@@ -398,7 +405,7 @@ struct VisitLDS {
   int2 off[64];
   Word pix[64];
   int2 src[64];
-  uint32_t waveCnt[WAVES];
+  uint32_t waveCnt[kScanBatch * WAVES];
   uint32_t candM[64 * WAVES], candF[64 * WAVES];
   int2 cand[64 * WAVES];
   uint8_t candOk[64 * WAVES];
@@ -417,10 +424,15 @@ struct VisitLDS {
 // Neighbour patch of target (tx,ty) (lib/synthesize.h:189-241) gathered by the whole
 // workgroup: neighbour 0 is the target itself; then the sorted offsets are scanned
 // 64*WAVES at a time, keeping in-image (or wrapped, when tiling) points that have a value.
+// A scan that the first step does not finish (pass-0 targets deep in a hole scan tens of
+// thousands of offsets) goes on kScanBatch steps per round: their loads are in flight
+// together, and the hits are placed in offset order by per-(step, wave) counts, so the
+// patch is the same.
 template <typename Word, int WAVES>
 __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, const int2 *sourceOf,
                                          const int2 *offsets, int tx, int ty, VisitLDS<Word, WAVES> &S)
 {
+  constexpr unsigned LANES = 64u * WAVES;
   const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   const size_t tp = (size_t)ty * (unsigned)g.tw + (unsigned)tx;
   if (tid == 0) {
@@ -432,48 +444,62 @@ __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, c
   const unsigned L = g.patchLimit;
   unsigned start = 1;
   if (g.skipEmpty && S.tileHas)
-    start = scan_start(g, S.tileHas, S.firstIdx, tx, ty, tid, 64u * WAVES,
+    start = scan_start(g, S.tileHas, S.firstIdx, tx, ty, tid, LANES,
                        [](bool v) { return __syncthreads_or(v) != 0; });
-  int2 nextOff = make_int2(0, 0);
-  if (start + tid < g.nO) nextOff = offsets[start + tid];
-  for (unsigned base = start; base < g.nO && count < L; base += 64u * WAVES) {
-    unsigned j = base + tid;
-    bool v = false;
-    int2 off = nextOff;
-    size_t p = 0;
-    Word w = 0;
-    // prefetch the next block of offsets while this one is gathered
-    if (j + 64u * WAVES < g.nO) nextOff = offsets[j + 64u * WAVES];
-    if (j < g.nO) {
-      int px = tx + off.x, py = ty + off.y;
-      bool in = true;
-      if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
-      else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
-      if (py < 0) { if (g.tileV) py += g.th; else in = false; }
-      else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
-      if (in) {
-        p = (size_t)py * (unsigned)g.tw + (unsigned)px;
-        w = tword[p];
-        v = (w & 1u) != 0;
+  unsigned steps = 1;
+  for (unsigned base = start; base < g.nO && count < L; base += steps * LANES, steps = kScanBatch) {
+    int2 off[kScanBatch];
+    Word w[kScanBatch];
+    size_t p[kScanBatch];
+    bool v[kScanBatch];
+#pragma unroll
+    for (unsigned st = 0; st < kScanBatch; st++) {
+      const unsigned j = base + st * LANES + tid;
+      v[st] = false;
+      p[st] = 0;
+      w[st] = 0;
+      off[st] = make_int2(0, 0);
+      if (st < steps && j < g.nO) {
+        off[st] = offsets[j];
+        int px = tx + off[st].x, py = ty + off[st].y;
+        bool in = true;
+        if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
+        else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
+        if (py < 0) { if (g.tileV) py += g.th; else in = false; }
+        else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
+        if (in) {
+          p[st] = (size_t)py * (unsigned)g.tw + (unsigned)px;
+          w[st] = tword[p[st]];
+          v[st] = (w[st] & 1u) != 0;
+        }
       }
     }
-    uint64_t m = __ballot(v);
-    if (lane == 0) S.waveCnt[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    unsigned before = 0, total = 0;
+    uint64_t m[kScanBatch];
 #pragma unroll
-    for (int wv = 0; wv < WAVES; wv++) {
-      unsigned c = S.waveCnt[wv];
-      before += (wv < (int)wave) ? c : 0;
-      total += c;
+    for (unsigned st = 0; st < kScanBatch; st++) {
+      m[st] = __ballot(v[st]);
+      if (st < steps && lane == 0) S.waveCnt[st * WAVES + wave] = (uint32_t)__popcll(m[st]);
     }
-    unsigned idx = count + before + (unsigned)__popcll(m & lanemask_lt());
-    if (v && idx < L) {
-      S.off[idx] = off;
-      S.pix[idx] = w;
-      S.src[idx] = sourceOf[p];
+    __syncthreads();
+    unsigned before = 0;
+#pragma unroll
+    for (unsigned st = 0; st < kScanBatch; st++) {
+      if (st >= steps) break;
+      unsigned mine = before;
+#pragma unroll
+      for (int wv = 0; wv < WAVES; wv++) {
+        const unsigned c = S.waveCnt[st * WAVES + wv];
+        mine += (wv < (int)wave) ? c : 0;
+        before += c;
+      }
+      const unsigned idx = count + mine + (unsigned)__popcll(m[st] & lanemask_lt());
+      if (v[st] && idx < L) {
+        S.off[idx] = off[st];
+        S.pix[idx] = w[st];
+        S.src[idx] = sourceOf[p[st]];
+      }
     }
-    count = (count + total < L) ? count + total : L;
+    count = (count + before < L) ? count + before : L;
     __syncthreads();
   }
   return count;
@@ -704,20 +730,25 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   const Word *cword = (const Word *)b.cword;
   const bool skip0 = g.nMaps == 0;
   const int2 t = b.targets[k];
+  // the target's learned proposals do not depend on its neighbourhood: their load overlaps
+  // the neighbour scan
+  int2 prop = make_int2(-1, -1);
+  if (b.props && lane < 16) prop = b.props[(size_t)(k - b.propsK0) * 16 + lane];
   const unsigned count =
       gather_neighbours<Word>(g, tword, b.sourceOf, b.offsets, t.x, t.y, sOff, sPix, sSrc, b.tileHas, b.firstIdx);
   uint32_t best = 0xFFFFFFFFu;
   int winX = 0, winY = 0;
-  bool perfect = false, scored = false, record = false;
-  int cx, cy;
+  bool perfect = false, scored = false;
   unsigned evals = 0;
+  bool record = false;
+  int cx, cy;
   coherence_phase<Word, NC>(g, cword, b.prober, k, count, sOff, sPix, sSrc, sT, sTm, best, winX, winY, perfect,
                             scored, record, cx, cy, evals);
   if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
-  // learned proposals: up to 16 corpus points, scored in rank order like probes
+  // learned proposals: up to 16 corpus points, scored in rank order like probes (groups of
+  // 8 loads would save round trips but take 8 more VGPRs, one wave per SIMD less)
   if (b.props && !perfect) {
-    int2 c = make_int2(-1, -1);
-    if (lane < 16) c = b.props[(size_t)(k - b.propsK0) * 16 + lane];
+    const int2 c = prop;
     const bool valid = c.x >= 0;
     uint32_t F = 0, M = 0;
     bool alive = false;

@@ -882,15 +882,27 @@ __device__ __forceinline__ void mlp_layer(const uint16_t (*in)[kLdsW], const uin
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c0 = wave * (N / 4);
   const int rr = lane & 15, kq = (lane >> 4) * 8;
+  constexpr int KS = K / 32;
   f32x4 acc[4][CB];
   for (int a = 0; a < 4; a++)
     for (int b = 0; b < CB; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 fb[CB];
-    for (int b = 0; b < CB; b++) fb[b] = *(const bf16x8 *)(wt + (size_t)(c0 + b * 16 + rr) * K + k0 + kq);
+  // All of this wave's weight fragments for the layer are requested before the first MFMA
+  // (KS*CB*4 VGPRs, at most 128), so the layer waits for one L2 round trip instead of one
+  // per k-step.
+  bf16x8 fb[KS][CB];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+    for (int b = 0; b < CB; b++) fb[ks][b] = *(const bf16x8 *)(wt + (size_t)(c0 + b * 16 + rr) * K + ks * 32 + kq);
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the MFMAs
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+#pragma unroll
     for (int a = 0; a < 4; a++) {
-      const bf16x8 fa = *(const bf16x8 *)&in[a * 16 + rr][k0 + kq];
-      for (int b = 0; b < CB; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[b], acc[a][b], 0, 0, 0);
+      const bf16x8 fa = *(const bf16x8 *)&in[a * 16 + rr][ks * 32 + kq];
+#pragma unroll
+      for (int b = 0; b < CB; b++)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[ks][b], acc[a][b], 0, 0, 0);
     }
   }
   for (int b = 0; b < CB; b++) {
