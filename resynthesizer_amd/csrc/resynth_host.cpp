@@ -658,7 +658,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
 
 // ------------------------------------------------------------------ learned proposals
 
-constexpr int kKeyBlock = 65536;   // corpus keys embedded per block
+constexpr int kKeyBlock = 1 << 20;   // corpus keys embedded per block (256 B of features each)
 
 // Inverted-file index of the corpus for learned proposals: keys at every corpus point of
 // even coordinates, their embeddings, centroids (every (nK/C)-th key) and the keys bucketed

@@ -846,11 +846,10 @@ namespace rs {
 
 // ------------------------------------------------------------------ fused embedding
 //
-// k_mlp_embed: 64 rows per workgroup through all three layers and the normalisation, the
-// activations kept in LDS (two ping-pong buffers) and the weights read as transposed bf16
-// copies ([out x in], k-contiguous) so that every MFMA B operand is one 16-byte load.
-// Four waves split each layer's output columns.
-constexpr int kRowsE = 64, kLdsW = 264;   // rows per workgroup, LDS row stride (bf16)
+// The fused embedding takes 64-row blocks through all three layers with the activations in
+// LDS; the weights come as transposed bf16 copies ([out x in], k-contiguous) so that every
+// MFMA B operand is one 16-byte load.  Four waves split each layer's output columns.
+constexpr int kRowsE = 64, kLdsW = 264;   // rows per block, LDS row stride (bf16)
 
 __global__ void k_transpose_w(const float *w, int in, int out, uint16_t *wt)
 {
@@ -872,98 +871,185 @@ hipError_t transpose_weights(hipStream_t s, const float *w, uint16_t *wt)
   return hipGetLastError();
 }
 
-// One layer: out[64 x N] = act(in[64 x K] W + b); this wave computes columns
-// [wave*N/4, (wave+1)*N/4).  CB = N/64 column blocks of 16 per wave.
-template <int K, int N, bool RELU>
-__device__ __forceinline__ void mlp_layer(const uint16_t (*in)[kLdsW], const uint16_t *wt, const float *bias,
-                                          uint16_t (*out)[kLdsW], float (*outF)[kEmb + 4])
+// Two floats rounded to bf16 (nearest even) and packed, low half first.
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
 {
-  constexpr int CB = N / 64;
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  bf16x2 v;
+  v.x = (__bf16)lo;
+  v.y = (__bf16)hi;
+  return __builtin_bit_cast(uint32_t, v);
+}
+
+// One layer on 64 rows: out[64 x N] = act(in[64 x K] W + b); this wave computes output
+// features [wave*N/4, (wave+1)*N/4), CB = N/64 blocks of 16, from its slice of the weights
+// held in registers (fw[k-step][feature block]).  The weights are the MFMA A operand and
+// the rows the B operand, so a lane ends up with 4 consecutive features of one row and
+// writes them with one 8-byte (16-byte for the fp32 last layer) LDS store.
+template <int K, int N, bool RELU>
+__device__ __forceinline__ void mlp_layer(const uint16_t (*in)[kLdsW], const bf16x8 (&fw)[K / 32][N / 64],
+                                          const float *bias, uint16_t (*out)[kLdsW], float (*outF)[kEmb + 4])
+{
+  constexpr int CB = N / 64, KS = K / 32;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c0 = wave * (N / 4);
-  const int rr = lane & 15, kq = (lane >> 4) * 8;
-  constexpr int KS = K / 32;
+  const int rr = lane & 15, kq = (lane >> 4) * 8, f4 = (lane >> 4) * 4;
   f32x4 acc[4][CB];
   for (int a = 0; a < 4; a++)
     for (int b = 0; b < CB; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // All of this wave's weight fragments for the layer are requested before the first MFMA
-  // (KS*CB*4 VGPRs, at most 128), so the layer waits for one L2 round trip instead of one
-  // per k-step.
-  bf16x8 fb[KS][CB];
-#pragma unroll
-  for (int ks = 0; ks < KS; ks++)
-#pragma unroll
-    for (int b = 0; b < CB; b++) fb[ks][b] = *(const bf16x8 *)(wt + (size_t)(c0 + b * 16 + rr) * K + ks * 32 + kq);
-  __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the MFMAs
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-      const bf16x8 fa = *(const bf16x8 *)&in[a * 16 + rr][ks * 32 + kq];
+      const bf16x8 fx = *(const bf16x8 *)&in[a * 16 + rr][ks * 32 + kq];
 #pragma unroll
       for (int b = 0; b < CB; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[ks][b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[ks][b], fx, acc[a][b], 0, 0, 0);
     }
   }
+  // acc[a][b][r]: feature c0 + b*16 + f4 + r of row a*16 + rr
+#pragma unroll
   for (int b = 0; b < CB; b++) {
-    const int col = c0 + b * 16 + (lane & 15);
-    const float bs = bias[col];
-    for (int a = 0; a < 4; a++)
-      for (int r = 0; r < 4; r++) {
-        const int row = a * 16 + (lane >> 4) * 4 + r;
-        float v = acc[a][b][r] + bs;
-        if (RELU) {
-          v = v > 0.f ? v : 0.f;
-          out[row][col] = f2bf(v);
-        } else {
-          outF[row][col] = v;
-        }
+    const int col = c0 + b * 16 + f4;
+    const float4 bb = *(const float4 *)(bias + col);
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int row = a * 16 + rr;
+      float v0 = acc[a][b][0] + bb.x, v1 = acc[a][b][1] + bb.y, v2 = acc[a][b][2] + bb.z, v3 = acc[a][b][3] + bb.w;
+      if (RELU) {
+        v0 = v0 > 0.f ? v0 : 0.f;
+        v1 = v1 > 0.f ? v1 : 0.f;
+        v2 = v2 > 0.f ? v2 : 0.f;
+        v3 = v3 > 0.f ? v3 : 0.f;
+        *(uint2 *)&out[row][col] = make_uint2(pack_bf16(v0, v1), pack_bf16(v2, v3));
+      } else {
+        *(float4 *)&outF[row][col] = make_float4(v0, v1, v2, v3);
       }
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_mlp_embed(const uint16_t *x, int n, const uint16_t *wt, const float *w,
-                                                   uint16_t *zb, float *z)
+// Loads this wave's slice of one layer's transposed weights (wt: [N x K] bf16) into fw.
+template <int K, int N>
+__device__ __forceinline__ void load_layer(const uint16_t *wt, bf16x8 (&fw)[K / 32][N / 64])
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int c0 = wave * (N / 4), rr = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+  for (int ks = 0; ks < K / 32; ks++)
+#pragma unroll
+    for (int cb = 0; cb < N / 64; cb++) fw[ks][cb] = *(const bf16x8 *)(wt + (size_t)(c0 + cb * 16 + rr) * K + ks * 32 + kq);
+}
+
+// k_mlp_embed: persistent, one 4-wave workgroup per compute unit.  Each wave loads its
+// column slices of all three layers' weights into registers once (224 VGPRs of bf16
+// B operands) and then takes 64-row blocks in turn: the rows go through the three layers
+// and the normalisation with the activations in LDS (two ping-pong buffers), and the next
+// block's input rows are fetched into registers while this one computes.  Re-reading the
+// 229 KB of weights from L2 for every 64 rows left the matrix cores idle 88% of the time.
+__global__ __launch_bounds__(256, 1) void k_mlp_embed(const uint16_t *x, int n, const uint16_t *wt, const float *w,
+                                                      uint16_t *zb, float *z)
 {
   __shared__ __attribute__((aligned(16))) uint16_t bufA[kRowsE][kLdsW];
   __shared__ __attribute__((aligned(16))) uint16_t bufB[kRowsE][kLdsW];
   const size_t OB1 = (size_t)kIn * kH1, OW2 = OB1 + kH1, OB2 = OW2 + (size_t)kH1 * kH2, OW3 = OB2 + kH2,
                OB3 = OW3 + (size_t)kH2 * kEmb;
   const uint16_t *wt1 = wt, *wt2 = wt + (size_t)kIn * kH1, *wt3 = wt2 + (size_t)kH1 * kH2;
-  const int r0 = blockIdx.x * kRowsE, tid = threadIdx.x;
-  // x rows -> bufA (128 bf16 each: 16 uint4)
-  for (int i = tid; i < kRowsE * 16; i += 256) {
-    const int r = i >> 4, c = (i & 15) * 8;
-    uint4 v = make_uint4(0, 0, 0, 0);
-    if (r0 + r < n) v = *(const uint4 *)(x + (size_t)(r0 + r) * kIn + c);
-    *(uint4 *)&bufA[r][c] = v;
-  }
-  __syncthreads();
-  mlp_layer<kIn, kH1, true>(bufA, wt1, w + OB1, bufB, nullptr);
-  __syncthreads();
-  mlp_layer<kH1, kH2, true>(bufB, wt2, w + OB2, bufA, nullptr);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  bf16x8 w1[kIn / 32][kH1 / 64], w2[kH1 / 32][kH2 / 64], w3[kH2 / 32][kEmb / 64];
+  load_layer<kIn, kH1>(wt1, w1);
+  load_layer<kH1, kH2>(wt2, w2);
+  load_layer<kH2, kEmb>(wt3, w3);
+  const int nBlocks = (n + kRowsE - 1) / kRowsE;
+  // input rows of a block: 64 x 128 bf16 = 1024 uint4, four per thread
+  uint4 xin[4];
+  auto fetch = [&](int blk) {
+    for (int u = 0; u < 4; u++) {
+      const int i = tid + u * 256, r = i >> 4, c = (i & 15) * 8;
+      const int row = blk * kRowsE + r;
+      xin[u] = (blk < nBlocks && row < n) ? *(const uint4 *)(x + (size_t)row * kIn + c) : make_uint4(0, 0, 0, 0);
+    }
+  };
+  auto stage = [&]() {
+    for (int u = 0; u < 4; u++) {
+      const int i = tid + u * 256, r = i >> 4, c = (i & 15) * 8;
+      *(uint4 *)&bufA[r][c] = xin[u];
+    }
+  };
+  int blk = blockIdx.x;
+  fetch(blk);
+  stage();
   __syncthreads();
   float (*eF)[kEmb + 4] = (float (*)[kEmb + 4])&bufB[0][0];   // 64 x 68 floats fit in bufB
-  mlp_layer<kH2, kEmb, false>(bufA, wt3, w + OB3, nullptr, eF);
-  __syncthreads();
-  // normalise: wave w takes rows w*16 .. w*16+15, lane = column
-  const int wave = tid >> 6, lane = tid & 63;
-  for (int r = wave * 16; r < wave * 16 + 16; r++) {
-    if (r0 + r >= n) break;
-    const float v = eF[r][lane];
-    float s2 = v * v;
-    for (int o = 32; o > 0; o >>= 1) s2 += __shfl_xor(s2, o);
-    const float zz = v / sqrtf(fmaxf(s2, 1e-12f));
-    if (zb) zb[(size_t)(r0 + r) * kEmb + lane] = f2bf(zz);
-    if (z) z[(size_t)(r0 + r) * kEmb + lane] = zz;
+  for (; blk < nBlocks; blk += gridDim.x) {
+    fetch(blk + (int)gridDim.x);
+    mlp_layer<kIn, kH1, true>(bufA, w1, w + OB1, bufB, nullptr);
+    __syncthreads();
+    mlp_layer<kH1, kH2, true>(bufB, w2, w + OB2, bufA, nullptr);
+    __syncthreads();
+    mlp_layer<kH2, kEmb, false>(bufA, w3, w + OB3, nullptr, eF);
+    __syncthreads();
+    stage();   // bufA is free: the next block's rows go in while eF is normalised
+    // normalise: wave w takes rows w*16 .. w*16+15 all at once, four lanes per row with 16
+    // columns each (a lane per column and a 64-lane reduction per row would serialise 16
+    // reductions at one wave per SIMD)
+    {
+      const int r = wave * 16 + (lane >> 2), seg = (lane & 3) * 16;
+      const int row = blk * kRowsE + r;
+      float v[16];
+      float s2 = 0.f;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const float4 f = *(const float4 *)&eF[r][seg + 4 * q];
+        v[4 * q] = f.x;
+        v[4 * q + 1] = f.y;
+        v[4 * q + 2] = f.z;
+        v[4 * q + 3] = f.w;
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++) s2 += v[c] * v[c];
+      s2 += __shfl_xor(s2, 1);
+      s2 += __shfl_xor(s2, 2);
+      const float nrm = sqrtf(fmaxf(s2, 1e-12f));
+      if (row < n) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) v[c] = v[c] / nrm;
+        if (zb) {
+          uint32_t pk[8];
+#pragma unroll
+          for (int c = 0; c < 8; c++) pk[c] = (uint32_t)f2bf(v[2 * c]) | ((uint32_t)f2bf(v[2 * c + 1]) << 16);
+          uint4 *dst = (uint4 *)(zb + (size_t)row * kEmb + seg);
+          dst[0] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+          dst[1] = make_uint4(pk[4], pk[5], pk[6], pk[7]);
+        }
+        if (z) {
+          float4 *dst = (float4 *)(z + (size_t)row * kEmb + seg);
+#pragma unroll
+          for (int q = 0; q < 4; q++) dst[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+        }
+      }
+    }
+    __syncthreads();
   }
+}
+
+static int compute_units()
+{
+  static const int cus = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0)
+      return 256;
+    return v;
+  }();
+  return cus;
 }
 
 hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, const uint16_t *x, int n, uint16_t *zb,
                            float *z)
 {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_embed, dim3((n + kRowsE - 1) / kRowsE), dim3(256), 0, s, x, n, wt, w, zb, z);
+  const int blocks = std::min((n + kRowsE - 1) / kRowsE, compute_units());
+  hipLaunchKernelGGL(k_mlp_embed, dim3(blocks), dim3(256), 0, s, x, n, wt, w, zb, z);
   return hipGetLastError();
 }
 
