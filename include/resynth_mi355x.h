@@ -335,6 +335,10 @@ int resynth_model_train(ResynthModel *model, const uint16_t *xq, const uint16_t 
  * A ([M x K], or [K x M] when ta) and B ([K x N], or [N x K] when tb); K a multiple of 32,
  * M and N multiples of 8.  For tests. */
 int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, const uint16_t *B, float *C);
+/* The sorted offsets the engine builds on the device (lib/engine.c:465-497) for offset
+ * ranges [-w+1, w-1] x [-h+1, h-1]: writes (2w-1)(2h-1) (x, y) pairs to out.  Returns 0, or
+ * -1 on invalid arguments.  For tests. */
+int resynth_sorted_offsets(int w, int h, int32_t *out);
 /* The learned index's query routing, on host arrays, through the same device sort and kernel
  * the engine uses.  queries: bf16 [nq x 64]; supers: [nq] super-cluster per query (a
  * negative one counts as 0); subCentroids: bf16 [S*Q x 64], super-cluster s owning rows

@@ -171,6 +171,7 @@ SIGNATURES = {
                                          ctypes.c_float, ctypes.c_uint, ctypes.c_void_p]),
     "resynth_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "resynth_sorted_offsets": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "resynth_learn_route": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_uint, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]),
     "resynth_learn_fine_top16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_int,
@@ -490,6 +491,14 @@ def gemm_bf16(a_bits, b_bits, ta=False, tb=False):
     if rc:
         raise ValueError("resynth_gemm_bf16 rejected its arguments")
     return C
+
+
+def sorted_offsets(w, h):
+    """The engine's device-built sorted offset table for a w x h offset range, (N, 2) int32."""
+    out = np.zeros(((2 * w - 1) * (2 * h - 1), 2), np.int32)
+    if lib().resynth_sorted_offsets(w, h, out.ctypes.data):
+        raise ValueError("resynth_sorted_offsets rejected its arguments")
+    return out
 
 
 def route(queries_bits, supers, sub_centroids_bits, S, Q, P):

@@ -529,7 +529,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   const size_t nO = (size_t)(2 * ow - 1) * (size_t)(2 * oh - 1);
   job.st.nOffsets = (unsigned)nO;
   int2 *offsets = (int2 *)W.offsets.get(nO * sizeof(int2));
-  size_t otb = offsets_temp_bytes(nO);
+  size_t otb = offsets_temp_bytes(ow, oh);
   HIP_TRY(launch_sorted_offsets(s, ow, oh, offsets, W.offTemp.get(otb), otb));
   Tables *hT = (Tables *)W.pinTables.get(sizeof(Tables));
   quantize_tables(P.sensitivityToOutliers, P.mapWeight, *hT);
@@ -1904,6 +1904,23 @@ int resynth_learn_fine_top16(const uint16_t *queries, unsigned int nq, const int
   HIP_TRY(learn_fine_top16_sorted(s, (const uint16_t *)dq.p, (int)n, v1, qcl, P, (const uint16_t *)dk.p,
                                   (const uint32_t *)dst.p, (const int2 *)dpts.p, (int2 *)dout.p));
   HIP_TRY(hipMemcpyAsync(props, dout.p, n * 16 * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int resynth_sorted_offsets(int w, int h, int32_t *out)
+{
+  if (w < 1 || h < 1 || !out || (size_t)w * (size_t)h > (1u << 28)) return -1;
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (!ensure_device()) return -1;
+  hipStream_t s = g_dev.stream;
+  const size_t nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
+  const size_t tb = offsets_temp_bytes(w, h);
+  TmpBuf dout, dtemp;
+  HIP_TRY(dout.alloc(nO * sizeof(int2)));
+  HIP_TRY(dtemp.alloc(tb));
+  HIP_TRY(launch_sorted_offsets(s, w, h, (int2 *)dout.p, dtemp.p, tb));
+  HIP_TRY(hipMemcpyAsync(out, dout.p, nO * sizeof(int2), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   return 0;
 }

@@ -126,7 +126,7 @@ hipError_t launch_adapt_maps(hipStream_t s, bool wide, const unsigned char *pix,
 size_t compact_temp_bytes(size_t n);
 hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int2 *out, unsigned *count,
                           void *temp);
-size_t offsets_temp_bytes(size_t nO);
+size_t offsets_temp_bytes(int w, int h);
 size_t sort_pairs_temp_bytes(size_t n);
 hipError_t launch_sort_pairs(hipStream_t s, const uint32_t *kIn, uint32_t *kOut, const uint32_t *vIn, uint32_t *vOut,
                              size_t n, unsigned bits, void *temp, size_t tempBytes);

@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace rs {
 
@@ -1606,23 +1607,77 @@ hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int
 // with a comparator that never returns 0 under GLib's merge sort, which orders equal
 // distances by descending row-major index.  The same order is a stable LSD radix sort
 // by x^2+y^2 of the offsets listed in descending row-major order.
-__global__ void k_offset_keys(int w, int h, uint32_t *keys, uint32_t *vals, size_t nO)
+// The sorted offsets (lib/engine.c:465-497): every (x, y) in [-w+1, w-1] x [-h+1, h-1]
+// by x^2 + y^2 ascending, ties in descending row-major index (y descending, then x
+// descending), the order GLib's merge sort gives the reference's never-equal comparator.
+// The table is symmetric under x -> -x and y -> -y, so only the quadrant x, y >= 0 is
+// sorted (a quarter of the entries); each quadrant entry (a, b) then writes its 1, 2 or 4
+// signed offsets straight to their places.  Within a ring of equal x^2 + y^2 the quadrant
+// entries come in descending b, and the ring's offsets are:
+//   y = +b rows in that order, each (a, b) then (-a, b);
+//   the y = 0 row, (a, 0) then (-a, 0);
+//   y = -b rows in the reverse order, each (a, -b) then (-a, -b).
+
+// Quadrant entry q = b*w + a, generated in descending q so that a stable sort by the key
+// leaves ties in descending q, i.e. descending b.
+__global__ void k_quad_keys(int w, uint32_t *keys, uint32_t *vals, size_t nQ)
 {
-  const unsigned W2 = 2u * (unsigned)w - 1u;
-  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nO; e += (size_t)gridDim.x * blockDim.x) {
-    uint32_t r = (uint32_t)(nO - 1 - e);
-    int x = (int)(r % W2) - (w - 1), y = (int)(r / W2) - (h - 1);
-    keys[e] = (uint32_t)(x * x + y * y);
-    vals[e] = r;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nQ; e += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t q = (uint32_t)(nQ - 1 - e);
+    const uint32_t a = q % (unsigned)w, b = q / (unsigned)w;
+    keys[e] = a * a + b * b;
+    vals[e] = q;
   }
 }
 
-__global__ void k_offset_decode(int w, int h, const uint32_t *vals, int2 *out, size_t nO)
+// Offsets a sorted quadrant entry stands for.
+__global__ void k_quad_members(int w, const uint32_t *vals, uint32_t *mem, size_t nQ)
 {
-  const unsigned W2 = 2u * (unsigned)w - 1u;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nO; i += (size_t)gridDim.x * blockDim.x) {
-    uint32_t r = vals[i];
-    out[i] = make_int2((int)(r % W2) - (w - 1), (int)(r / W2) - (h - 1));
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nQ; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t q = vals[i];
+    const uint32_t a = q % (unsigned)w, b = q / (unsigned)w;
+    mem[i] = (a > 0 ? 2u : 1u) * (b > 0 ? 2u : 1u);
+  }
+}
+
+// first: exclusive prefix sums of the member counts, so that first[lo] is where the ring
+// of entries lo.. begins in the full table.
+__global__ void k_quad_expand(int w, const uint32_t *keys, const uint32_t *vals, const uint32_t *first, size_t nQ,
+                              int2 *out)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nQ; i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t D = keys[i];
+    size_t lo = i, hi = i + 1;
+    while (lo > 0 && keys[lo - 1] == D) lo--;
+    while (hi < nQ && keys[hi] == D) hi++;
+    uint32_t before = 0, rowsAbove = 0, axis = 0;   // y > 0 offsets before entry i; all of them; the y = 0 ones
+    for (size_t j = lo; j < hi; j++) {
+      const uint32_t qj = vals[j];
+      const uint32_t m = (qj % (unsigned)w) > 0 ? 2u : 1u;
+      if (qj / (unsigned)w > 0) {
+        rowsAbove += m;
+        if (j < i) before += m;
+      } else {
+        axis = m;
+      }
+    }
+    const uint32_t q = vals[i];
+    const int a = (int)(q % (unsigned)w), b = (int)(q / (unsigned)w);
+    const uint32_t m = a > 0 ? 2u : 1u;
+    const size_t start = first[lo];
+    if (b > 0) {
+      const size_t up = start + before, down = start + rowsAbove + axis + (rowsAbove - before - m);
+      out[up] = make_int2(a, b);
+      out[down] = make_int2(a, -b);
+      if (a > 0) {
+        out[up + 1] = make_int2(-a, b);
+        out[down + 1] = make_int2(-a, -b);
+      }
+    } else {
+      const size_t mid = start + rowsAbove;
+      out[mid] = make_int2(a, 0);
+      if (a > 0) out[mid + 1] = make_int2(-a, 0);
+    }
   }
 }
 
@@ -1634,28 +1689,40 @@ static unsigned key_bits(int w, int h)
   return b;
 }
 
-size_t offsets_temp_bytes(size_t nO)
+static size_t quad_scan_bytes(size_t nQ)
 {
+  size_t scanBytes = 0;
+  (void)rocprim::exclusive_scan(nullptr, scanBytes, (uint32_t *)nullptr, (uint32_t *)nullptr, 0u, nQ,
+                                rocprim::plus<uint32_t>());
+  return scanBytes;
+}
+
+size_t offsets_temp_bytes(int w, int h)
+{
+  const size_t nQ = (size_t)w * (size_t)h;
   size_t sortBytes = 0;
   (void)rocprim::radix_sort_pairs<rocprim::default_config>(nullptr, sortBytes, (uint32_t *)nullptr,
                                                            (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                                           (uint32_t *)nullptr, nO, 0, 32);
-  return 4 * nO * sizeof(uint32_t) + sortBytes + 1024;
+                                                           (uint32_t *)nullptr, nQ, 0, 32);
+  return 5 * nQ * sizeof(uint32_t) + std::max(sortBytes, quad_scan_bytes(nQ)) + 1024;
 }
 
 hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes)
 {
-  size_t nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
-  uint32_t *k0 = (uint32_t *)temp, *v0 = k0 + nO, *k1 = v0 + nO, *v1 = k1 + nO;
-  char *sortTemp = (char *)(v1 + nO);
-  size_t sortBytes = tempBytes - 4 * nO * sizeof(uint32_t);
-  unsigned blocks = (unsigned)((nO + 255) / 256);
+  const size_t nQ = (size_t)w * (size_t)h;
+  uint32_t *k0 = (uint32_t *)temp, *v0 = k0 + nQ, *k1 = v0 + nQ, *v1 = k1 + nQ, *first = v1 + nQ;
+  char *work = (char *)(first + nQ);
+  size_t workBytes = tempBytes - 5 * nQ * sizeof(uint32_t);
+  unsigned blocks = (unsigned)((nQ + 255) / 256);
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_offset_keys, dim3(blocks), dim3(256), 0, s, w, h, k0, v0, nO);
-  hipError_t e = rocprim::radix_sort_pairs<rocprim::default_config>(sortTemp, sortBytes, k0, k1, v0, v1, nO, 0,
+  hipLaunchKernelGGL(k_quad_keys, dim3(blocks), dim3(256), 0, s, w, k0, v0, nQ);
+  hipError_t e = rocprim::radix_sort_pairs<rocprim::default_config>(work, workBytes, k0, k1, v0, v1, nQ, 0,
                                                                     key_bits(w, h), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_offset_decode, dim3(blocks), dim3(256), 0, s, w, h, v1, out, nO);
+  hipLaunchKernelGGL(k_quad_members, dim3(blocks), dim3(256), 0, s, w, v1, k0, nQ);
+  e = rocprim::exclusive_scan(work, workBytes, k0, first, 0u, nQ, rocprim::plus<uint32_t>(), s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_quad_expand, dim3(blocks), dim3(256), 0, s, w, k1, v1, first, nQ, out);
   return hipGetLastError();
 }
 

@@ -23,6 +23,35 @@ def test_device_is_gfx950():
     assert rc == 0 and arch.startswith("gfx950") and cus >= 200
 
 
+# ---------------------------------------------------------------- sorted offsets
+
+def _sha(a):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("wh", [(1, 1), (2, 1), (3, 3), (7, 5), (20, 13), (64, 64), (31, 47), (128, 96)])
+def test_device_sorted_offsets_match_g_array_sort(wh):
+    """The device-built offset table (quadrant sort + expansion) equals GLib's g_array_sort
+    order of the reference (lib/engine.c:465-497), pinned by the sha256 golden vectors."""
+    import json
+    import os
+    v = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "vectors.json")))["sort"]
+    got = R.sorted_offsets(*wh)
+    assert _sha(got) == v["offsets_%dx%d_sha256" % wh]
+
+
+@pytest.mark.parametrize("wh", [(1, 300), (300, 1), (257, 190), (190, 257), (1024, 768)])
+def test_device_sorted_offsets_closed_form(wh):
+    """Larger and degenerate offset ranges against the closed form of that order: x^2 + y^2
+    ascending, ties in descending row-major index."""
+    w, h = wh
+    ys, xs = np.mgrid[-h + 1:h, -w + 1:w]
+    pts = np.stack([xs.ravel(), ys.ravel()], 1).astype(np.int64)
+    order = np.lexsort((-np.arange(len(pts)), pts[:, 0] ** 2 + pts[:, 1] ** 2))
+    assert np.array_equal(R.sorted_offsets(w, h), pts[order])
+
+
 # ---------------------------------------------------------------- frozen-state scorer
 
 @pytest.mark.parametrize("layout", ["rgb", "gray", "rgb_maps3", "gray_map1", "rgba"])
