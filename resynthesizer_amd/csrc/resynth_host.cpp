@@ -485,6 +485,8 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   Source &S = job.src;
   const int tw = S.tw, th = S.th, cw = S.cw, ch = S.ch;
   const size_t nTpix = (size_t)tw * th, nCpix = (size_t)cw * ch;
+  // the kernels index pixels with 32-bit words (2^31 pixels is 8 GiB of words per image)
+  if (nTpix >= (1u << 31) || nCpix >= (1u << 31)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
   job.nColors = ix.img_match_bpp;
   job.nMaps = ix.map_match_bpp;
   job.wide = job.nColors + job.nMaps > 3;
@@ -869,7 +871,12 @@ int refine_engine(Job &job, Prepared &pr)
   // grows with the cap's share of the targets: about +1.5% at nT/64, +2% at nT/32 and
   // +6% at nT/8 over the serial engine on 512x512 heals).
   const unsigned cap = job.opt.batch ? job.opt.batch : std::min(65536u, std::max(64u, nT / 32u));
-  const unsigned minChunk = std::min(cap, std::max(1u, std::min(64u, nT / 2048u)));
+  // RESYNTH_MIN_CHUNK overrides the smallest pass-0 chunk (for tuning studies)
+  static const unsigned minChunkEnv = [] {
+    const char *e = std::getenv("RESYNTH_MIN_CHUNK");
+    return e ? (unsigned)std::max(1L, std::atol(e)) : 0u;
+  }();
+  const unsigned minChunk = std::min(cap, minChunkEnv ? minChunkEnv : std::max(1u, std::min(64u, nT / 2048u)));
   static const unsigned rampDefault = [] {
     const char *e = std::getenv("RESYNTH_RAMP_PERMILLE");
     long v = e ? std::atol(e) : 60;

@@ -61,16 +61,14 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
   bool in[G];
 #pragma unroll
   for (int u = 0; u < G; u++) {
-    unsigned i = base + u;
-    in[u] = false;
-    q[u] = 0;
-    if (i < count) {
-      int qx = cx + nbOff[i].x, qy = cy + nbOff[i].y;
-      if ((unsigned)qx < (unsigned)g.cw && (unsigned)qy < (unsigned)g.ch) {
-        q[u] = cword[(size_t)qy * (unsigned)g.cw + (unsigned)qx];
-        in[u] = true;
-      }
-    }
+    const unsigned i = base + u;
+    const bool live = i < count;
+    const int2 o = nbOff[live ? i : 0u];
+    const int qx = cx + o.x, qy = cy + o.y;
+    in[u] = live && (unsigned)qx < (unsigned)g.cw && (unsigned)qy < (unsigned)g.ch;
+    // unconditional load, word 0 standing in for an outside neighbour (see score_group_rgb)
+    const unsigned idx = in[u] ? __umul24((unsigned)qy, (unsigned)g.cw) + (unsigned)qx : 0u;
+    q[u] = cword[idx];
   }
 #pragma unroll
   for (int u = 0; u < G; u++) {
@@ -104,7 +102,11 @@ __device__ __forceinline__ void score_group_rgb(const Geometry &g, const Word *_
     const int2 o = nbOff[live ? i : 0u];
     const int qx = cx + o.x, qy = cy + o.y;
     in[u] = live && (unsigned)qx < (unsigned)g.cw && (unsigned)qy < (unsigned)g.ch;
-    q[u] = in[u] ? cword[(size_t)qy * (unsigned)g.cw + (unsigned)qx] : (Word)0;
+    // Every lane loads (word 0 stands in for an outside neighbour, whose term is the
+    // penalty), so the loads need no branch around them; the index is 32-bit (corpus
+    // sides below 2^24 pixels, fewer than 2^31 words).
+    const unsigned idx = in[u] ? __umul24((unsigned)qy, (unsigned)g.cw) + (unsigned)qx : 0u;
+    q[u] = cword[idx];
   }
 #pragma unroll
   for (int u = 0; u < G; u++) {
