@@ -245,25 +245,19 @@ __device__ unsigned gather_neighbours(const Geometry &g, const Word *tword, cons
   if (g.skipEmpty && tileHas)
     start = scan_start(g, tileHas, firstIdx, tx, ty, lane, 64u, [](bool v) { return __ballot(v) != 0ull; });
   for (unsigned base = start; base < g.nO && count < L; base += 64) {
-    unsigned j = base + lane;
-    bool v = false;
-    int2 off = make_int2(0, 0);
-    size_t p = 0;
-    Word w = 0;
-    if (j < g.nO) {
-      off = offsets[j];
-      int px = tx + off.x, py = ty + off.y;
-      bool in = true;
-      if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
-      else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
-      if (py < 0) { if (g.tileV) py += g.th; else in = false; }
-      else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
-      if (in) {
-        p = (size_t)py * (unsigned)g.tw + (unsigned)px;
-        w = tword[p];
-        v = (w & 1u) != 0;
-      }
-    }
+    // loads without branches: lanes past the table or outside the image read entry 0 / the
+    // target's own word and do not count
+    const unsigned j = base + lane;
+    const int2 off = offsets[j < g.nO ? j : 0u];
+    int px = tx + off.x, py = ty + off.y;
+    bool in = j < g.nO;
+    if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
+    else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
+    if (py < 0) { if (g.tileV) py += g.th; else in = false; }
+    else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
+    const size_t p = in ? (size_t)(__umul24((unsigned)py, (unsigned)g.tw) + (unsigned)px) : tp;
+    const Word w = tword[p];
+    const bool v = in && (w & 1u) != 0;
     uint64_t m = __ballot(v);
     unsigned rank = __popcll(m & lanemask_lt());
     if (v && count + rank < L) {
@@ -298,18 +292,22 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
   uint32_t lastProber = k;
   cx = 0;
   cy = 0;
-  if (lane < count) {
-    int2 src = nbSrc[lane];
-    if (src.x != -1) {
-      cx = src.x - nbOff[lane].x;
-      cy = src.y - nbOff[lane].y;
-      if ((unsigned)cx < (unsigned)g.cw && (unsigned)cy < (unsigned)g.ch) {
-        // the candidate's word and its recent-prober entry, loaded together
-        const size_t ci = (size_t)cy * (unsigned)g.cw + (unsigned)cx;
-        Word q = cword[ci];
-        lastProber = prober[ci];
-        valid = (q & 0xFEu) == 0xFEu;
-      }
+  {
+    // the candidate's word and its recent-prober entry, loaded together and without a
+    // branch (lanes without a candidate read entry 0 and stay invalid)
+    const int2 src = nbSrc[lane < count ? lane : 0u];
+    const int2 o = nbOff[lane < count ? lane : 0u];
+    cx = src.x - o.x;
+    cy = src.y - o.y;
+    const bool in = lane < count && src.x != -1 && (unsigned)cx < (unsigned)g.cw && (unsigned)cy < (unsigned)g.ch;
+    const unsigned ci = in ? __umul24((unsigned)cy, (unsigned)g.cw) + (unsigned)cx : 0u;
+    const Word q = cword[ci];
+    const uint32_t pr = prober[ci];
+    valid = in && (q & 0xFEu) == 0xFEu;
+    lastProber = in ? pr : k;
+    if (!in) {
+      cx = 0;
+      cy = 0;
     }
   }
   // earlier lane with the same candidate
@@ -459,22 +457,21 @@ __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, c
     for (unsigned st = 0; st < kScanBatch; st++) {
       const unsigned j = base + st * LANES + tid;
       v[st] = false;
-      p[st] = 0;
+      p[st] = tp;
       w[st] = 0;
       off[st] = make_int2(0, 0);
-      if (st < steps && j < g.nO) {
-        off[st] = offsets[j];
+      if (st < steps) {
+        // loads without branches, as in the wave scan
+        off[st] = offsets[j < g.nO ? j : 0u];
         int px = tx + off[st].x, py = ty + off[st].y;
-        bool in = true;
+        bool in = j < g.nO;
         if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
         else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
         if (py < 0) { if (g.tileV) py += g.th; else in = false; }
         else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
-        if (in) {
-          p[st] = (size_t)py * (unsigned)g.tw + (unsigned)px;
-          w[st] = tword[p[st]];
-          v[st] = (w[st] & 1u) != 0;
-        }
+        p[st] = in ? (size_t)(__umul24((unsigned)py, (unsigned)g.tw) + (unsigned)px) : tp;
+        w[st] = tword[p[st]];
+        v[st] = in && (w[st] & 1u) != 0;
       }
     }
     uint64_t m[kScanBatch];
