@@ -322,7 +322,7 @@ struct PinnedBuf {
 // Device and pinned buffers of one engine run (grow-only, reused across calls).
 struct Workspace {
   Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
-      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, firstIdx, wctl,
+      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, ctxRings, ringTemp, firstIdx, wctl,
       bandStart;
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
   Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, centIdx, keyCl, bCount, bStart, bCursor,
@@ -552,6 +552,10 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   uint8_t *tileHas = (uint8_t *)W.tileHas.get((nTiles + 3) / 4 * 4);
   HIP_TRY(launch_fill_u32(s, (uint32_t *)tileHas, 0u, (nTiles + 3) / 4));
   HIP_TRY(launch_tile_init(s, job.wide, tword, tw, th, tileHas));
+  // empty rings of the context-only occupancy, before any visit updates tileHas
+  int32_t *ctxRings = (int32_t *)W.ctxRings.get(nTiles * sizeof(int32_t) + 64);
+  HIP_TRY(launch_ctx_rings(s, tileHas, tilesW, tilesH, (int32_t *)W.ringTemp.get(nTiles * sizeof(int32_t) + 64),
+                           ctxRings));
   const unsigned rhoMax = (unsigned)std::ceil(std::sqrt((double)(ow - 1) * (ow - 1) + (double)(oh - 1) * (oh - 1))) + 1;
   uint32_t *firstIdx = (uint32_t *)W.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
   HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
@@ -634,6 +638,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   b.tables = dT;
   b.counters = cnt;
   b.tileHas = tileHas;
+  b.ctxRings = ctxRings;
   b.firstIdx = firstIdx;
   b.ctl = (uint32_t *)W.wctl.get(WCTL_WORDS * sizeof(uint32_t));
   b.bandStart = bandStart;
@@ -996,6 +1001,7 @@ int refine_engine(Job &job, Prepared &pr)
           windowLaunches++;
         } else {
           Geometry gl = g;
+          gl.nVisited = g.skipEmpty ? k : 0u;   // pass 0: targets before the chunk hold values
           SynthBuffers bl = b;
           const bool learnPass = !job.opt.learnedPasses || pass < job.opt.learnedPasses;
           if (learned.ok && job.opt.learnedRefresh && propsAll) {

@@ -183,8 +183,8 @@ __device__ __forceinline__ void resolve(bool cand, uint32_t M, uint32_t F, uint3
 // pixels away, so the offsets with x^2+y^2 < (8R+1)^2 cannot hold one.  `lanes` threads
 // (a wave or a workgroup) share the ring walk; `anyTile` reduces over them.
 template <typename AnyFn>
-__device__ __forceinline__ unsigned scan_start(const Geometry &g, const uint8_t *tileHas, const uint32_t *firstIdx,
-                                               int tx, int ty, unsigned tid, unsigned lanes, AnyFn anyTile)
+__device__ __forceinline__ int empty_rings(const Geometry &g, const uint8_t *tileHas, int tx, int ty, unsigned tid,
+                                           unsigned lanes, AnyFn anyTile)
 {
   const int TX = tx / kTile, TY = ty / kTile;
   const int maxR = max(max(TX, g.tilesW - 1 - TX), max(TY, g.tilesH - 1 - TY));
@@ -213,16 +213,78 @@ __device__ __forceinline__ unsigned scan_start(const Geometry &g, const uint8_t 
     if (hit) break;
     R = r;
   }
-  if (R < 0) return 1u;
-  unsigned rho = (unsigned)(kTile * R + 1);
+  return R;
+}
+
+// Index of the first sorted offset with x^2 + y^2 >= rho^2 (at least 1: offset 0 is the
+// target itself), g.nO past the table.
+__device__ __forceinline__ unsigned offset_index(const Geometry &g, const uint32_t *firstIdx, unsigned rho)
+{
   if (rho > g.rhoMax) return g.nO;
-  unsigned j = firstIdx[rho];
+  const unsigned j = firstIdx[rho];
   return j < 1u ? 1u : j;
+}
+
+// Index of the first sorted offset worth scanning (see empty_rings).
+template <typename AnyFn>
+__device__ __forceinline__ unsigned scan_start(const Geometry &g, const uint8_t *tileHas, const uint32_t *firstIdx,
+                                               int tx, int ty, unsigned tid, unsigned lanes, AnyFn anyTile)
+{
+  const int R = empty_rings(g, tileHas, tx, ty, tid, lanes, anyTile);
+  return R < 0 ? 1u : offset_index(g, firstIdx, (unsigned)(kTile * R + 1));
+}
+
+// Empty Chebyshev rings of tiles around every tile of a static occupancy map (the
+// context-only snapshot): rings[i] = d - 1 for d the Chebyshev distance in tiles from
+// tile i to the nearest marked tile (-1 when tile i is marked), or the tile's largest
+// ring when nothing is marked -- what empty_rings walks to, in one load.  Rows first:
+// h = horizontal distance to the nearest marked tile of the row.
+__global__ void k_ring_rows(const uint8_t *mark, int W, int H, int32_t *h)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  constexpr int kFar = 1 << 30;
+  const int x = i % W;
+  const uint8_t *row = mark + (size_t)(i / W) * W;
+  int d = kFar;
+  for (int dist = 0; x - dist >= 0 || x + dist < W; dist++)
+    if ((x - dist >= 0 && row[x - dist]) || (x + dist < W && row[x + dist])) {
+      d = dist;
+      break;
+    }
+  h[i] = d;
+}
+
+// Then columns: d(x, y) = min over y' of max(h(x, y'), |y - y'|), searched outward from y
+// and stopped once |y - y'| reaches the best found.
+__global__ void k_ring_cols(const int32_t *h, int W, int H, int32_t *rings)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  constexpr int kFar = 1 << 30;
+  const int x = i % W, y = i / W;
+  int best = kFar;
+  for (int dist = 0; dist < best && (y - dist >= 0 || y + dist < H); dist++) {
+    if (y - dist >= 0) best = min(best, max(h[(size_t)(y - dist) * W + x], dist));
+    if (y + dist < H) best = min(best, max(h[(size_t)(y + dist) * W + x], dist));
+  }
+  const int maxR = max(max(x, W - 1 - x), max(y, H - 1 - y));
+  rings[i] = best >= kFar ? maxR : best - 1;
+}
+
+hipError_t launch_ctx_rings(hipStream_t s, const uint8_t *mark, int W, int H, int32_t *scratch, int32_t *rings)
+{
+  hipLaunchKernelGGL(k_ring_rows, dim3((W * H + 255) / 256), dim3(256), 0, s, mark, W, H, scratch);
+  hipLaunchKernelGGL(k_ring_cols, dim3((W * H + 255) / 256), dim3(256), 0, s, scratch, W, H, rings);
+  return hipGetLastError();
 }
 
 // Steps of the workgroup offset scan taken per round once the first step has not filled
 // the patch.
 constexpr unsigned kScanBatch = 4;
+// Pass-0 workgroup scans while at most kNearTargets targets are visited: the near part of
+// the patch comes from those targets directly (see gather_neighbours_wg).
+constexpr unsigned kNearTargets = 4096, kNearMax = 64;
 
 // Neighbour patch of target (tx,ty) (lib/synthesize.h:189-241): neighbour 0 is the
 // target itself; then the sorted offsets are scanned 64 at a time, keeping points in
@@ -407,6 +469,10 @@ struct VisitLDS {
   Word pix[64];
   int2 src[64];
   uint32_t waveCnt[kScanBatch * WAVES];
+  // visited targets nearer than any context pixel (pass 0, early chunks)
+  uint32_t nearCnt;
+  uint32_t nearD2[kNearMax], nearIdx[kNearMax];
+  int2 nearPt[kNearMax];
   uint32_t candM[64 * WAVES], candF[64 * WAVES];
   int2 cand[64 * WAVES];
   uint8_t candOk[64 * WAVES];
@@ -419,6 +485,7 @@ struct VisitLDS {
   uint32_t streamPos;
   uint32_t lastUsed;
   const uint8_t *tileHas;
+  const int32_t *ctxRings;
   const uint32_t *firstIdx;
 };
 
@@ -429,9 +496,20 @@ struct VisitLDS {
 // thousands of offsets) goes on kScanBatch steps per round: their loads are in flight
 // together, and the hits are placed in offset order by per-(step, wave) counts, so the
 // patch is the same.
+//
+// Early in pass 0 (nVisited = targets[0, nVisited) visited, at most kNearTargets), a
+// target deep in a hole would scan everything between a few scattered visited pixels and
+// the hole's border.  Instead the context-only tile map (a snapshot taken before any
+// visit) gives a radius rho inside which no context pixel lies; inside it the only
+// valued pixels are visited targets, which are found directly and ordered like the
+// sorted offsets (x^2 + y^2 ascending, row-major offset index descending); the scan then
+// resumes at rho.  When more of them lie inside rho than the patch holds, the plain scan
+// runs instead.  The patch is the same either way (targets of the current chunk that
+// land early are the only difference a batched visit may see).
 template <typename Word, int WAVES>
 __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, const int2 *sourceOf,
-                                         const int2 *offsets, int tx, int ty, VisitLDS<Word, WAVES> &S)
+                                         const int2 *offsets, int tx, int ty, VisitLDS<Word, WAVES> &S,
+                                         const int2 *targets, unsigned nVisited)
 {
   constexpr unsigned LANES = 64u * WAVES;
   const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
@@ -440,13 +518,54 @@ __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, c
     S.off[0] = make_int2(0, 0);
     S.pix[0] = tword[tp];
     S.src[0] = sourceOf[tp];
+    S.nearCnt = 0;
   }
   unsigned count = 1;
   const unsigned L = g.patchLimit;
   unsigned start = 1;
-  if (g.skipEmpty && S.tileHas)
-    start = scan_start(g, S.tileHas, S.firstIdx, tx, ty, tid, LANES,
-                       [](bool v) { return __syncthreads_or(v) != 0; });
+  auto anyTile = [](bool v) { return __syncthreads_or(v) != 0; };
+  bool near = false;
+  if (g.skipEmpty && S.ctxRings && nVisited > 0 && nVisited <= kNearTargets) {
+    const int R = S.ctxRings[(size_t)(ty / kTile) * g.tilesW + tx / kTile];
+    const unsigned rho = R < 0 ? 0u : (unsigned)(kTile * R + 1);
+    const uint64_t rho2 = (uint64_t)rho * rho;
+    const int ow = min(g.cw, g.tw), oh = min(g.ch, g.th);
+    for (unsigned j = tid; j < nVisited; j += LANES) {
+      const int2 q = targets[j];
+      const int dx = q.x - tx, dy = q.y - ty;
+      const uint64_t d2 = (uint64_t)((int64_t)dx * dx) + (uint64_t)((int64_t)dy * dy);
+      if (d2 > 0 && d2 < rho2 && dx > -ow && dx < ow && dy > -oh && dy < oh) {
+        const unsigned slot = atomicAdd(&S.nearCnt, 1u);
+        if (slot < kNearMax) {
+          S.nearD2[slot] = (uint32_t)d2;
+          S.nearIdx[slot] = (uint32_t)(dy + oh - 1) * (uint32_t)(2 * ow - 1) + (uint32_t)(dx + ow - 1);
+          S.nearPt[slot] = q;
+        }
+      }
+    }
+    __syncthreads();
+    const unsigned nNear = S.nearCnt;
+    if (nNear + 1 <= L) {
+      near = true;
+      if (tid < nNear) {
+        const uint32_t d2 = S.nearD2[tid], id = S.nearIdx[tid];
+        unsigned rank = 0;
+        for (unsigned s = 0; s < nNear; s++) {
+          const uint32_t e2 = S.nearD2[s], ie = S.nearIdx[s];
+          rank += (e2 < d2 || (e2 == d2 && ie > id)) ? 1u : 0u;
+        }
+        const int2 q = S.nearPt[tid];
+        const size_t pq = (size_t)q.y * (unsigned)g.tw + (unsigned)q.x;
+        S.off[1 + rank] = make_int2(q.x - tx, q.y - ty);
+        S.pix[1 + rank] = tword[pq];
+        S.src[1 + rank] = sourceOf[pq];
+      }
+      count = 1 + nNear;
+      start = rho == 0 ? 1u : offset_index(g, S.firstIdx, rho);
+      __syncthreads();
+    }
+  }
+  if (g.skipEmpty && S.tileHas && !near) start = scan_start(g, S.tileHas, S.firstIdx, tx, ty, tid, LANES, anyTile);
   unsigned steps = 1;
   for (unsigned base = start; base < g.nO && count < L; base += steps * LANES, steps = kScanBatch) {
     int2 off[kScanBatch];
@@ -548,17 +667,19 @@ __device__ void resolve_slots_wave0(VisitLDS<Word, WAVES> &S, unsigned nSlots, i
 // probes from the exact PRNG stream (prngKind, stream, streamLen, S.streamPos) and aborts
 // without side effects when the stream runs dry; otherwise probes come from a per-target
 // counter-based stream.  Global writes happen only in the commit at the end.
+// nVisited: targets[0, nVisited) are known to hold values (0: unknown).
 template <typename Word, int WAVES, bool SERIAL, int NC = 0>
 __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word, WAVES> &S, unsigned k,
                          unsigned seed, unsigned pass, int prngKind, const uint32_t *stream, uint32_t streamLen,
                          uint32_t maxvalue, unsigned long long &evalTot, unsigned long long &termTot,
-                         unsigned &betTot)
+                         unsigned &betTot, unsigned nVisited)
 {
   const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
   Word *tword = (Word *)b.tword;
   const Word *cword = (const Word *)b.cword;
   const int2 t = b.targets[k];
-  const unsigned count = gather_neighbours_wg<Word, WAVES>(g, tword, b.sourceOf, b.offsets, t.x, t.y, S);
+  const unsigned count =
+      gather_neighbours_wg<Word, WAVES>(g, tword, b.sourceOf, b.offsets, t.x, t.y, S, b.targets, nVisited);
 
   // heuristic 1 on wave 0
   bool record = false;
@@ -679,6 +800,7 @@ __device__ __forceinline__ void load_tables(const SynthBuffers &b, VisitLDS<Word
   }
   if (threadIdx.x == 0) {
     S.tileHas = b.tileHas;
+    S.ctxRings = b.ctxRings;
     S.firstIdx = b.firstIdx;
   }
   __syncthreads();
@@ -694,7 +816,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_batched(Geometry g, SynthB
   unsigned long long ev = 0, tm = 0;
   unsigned bt = 0;
   for (unsigned k = k0 + blockIdx.x; k < k1; k += gridDim.x) {
-    visit_wg<Word, WAVES, false, NC>(g, b, S, k, seed, pass, 0, nullptr, 0, 0, ev, tm, bt);
+    visit_wg<Word, WAVES, false, NC>(g, b, S, k, seed, pass, 0, nullptr, 0, 0, ev, tm, bt, g.nVisited);
     __syncthreads();
   }
   if (threadIdx.x == 0) {
@@ -1345,7 +1467,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_synth_serial(Geometry g, SynthBu
   for (; k < k1; k++) {
     const uint32_t posBefore = S.streamPos;
     __syncthreads();
-    visit_wg<Word, WAVES, true, NC>(g, b, S, k, 0, 0, prngKind, stream, streamLen, maxvalue, ev, tm, bt);
+    visit_wg<Word, WAVES, true, NC>(g, b, S, k, 0, 0, prngKind, stream, streamLen, maxvalue, ev, tm, bt,
+                                    g.skipEmpty ? k : 0u);
     __syncthreads();
     if (S.aborted) {
       __syncthreads();
