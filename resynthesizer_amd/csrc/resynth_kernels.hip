@@ -1819,6 +1819,11 @@ static size_t quad_scan_bytes(size_t nQ)
   return scanBytes;
 }
 
+// The five u32 arrays and rocPRIM's work area each start on a 256-byte boundary of the
+// (256-aligned) temporary buffer: the multi-block sort and scan keep 64-bit look-back
+// state in the work area and need it aligned, whatever w*h is.
+static size_t quad_array_bytes(size_t nQ) { return (nQ * sizeof(uint32_t) + 255) & ~(size_t)255; }
+
 size_t offsets_temp_bytes(int w, int h)
 {
   const size_t nQ = (size_t)w * (size_t)h;
@@ -1826,15 +1831,17 @@ size_t offsets_temp_bytes(int w, int h)
   (void)rocprim::radix_sort_pairs<rocprim::default_config>(nullptr, sortBytes, (uint32_t *)nullptr,
                                                            (uint32_t *)nullptr, (uint32_t *)nullptr,
                                                            (uint32_t *)nullptr, nQ, 0, 32);
-  return 5 * nQ * sizeof(uint32_t) + std::max(sortBytes, quad_scan_bytes(nQ)) + 1024;
+  return 5 * quad_array_bytes(nQ) + std::max(sortBytes, quad_scan_bytes(nQ)) + 1024;
 }
 
 hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes)
 {
-  const size_t nQ = (size_t)w * (size_t)h;
-  uint32_t *k0 = (uint32_t *)temp, *v0 = k0 + nQ, *k1 = v0 + nQ, *v1 = k1 + nQ, *first = v1 + nQ;
-  char *work = (char *)(first + nQ);
-  size_t workBytes = tempBytes - 5 * nQ * sizeof(uint32_t);
+  const size_t nQ = (size_t)w * (size_t)h, A = quad_array_bytes(nQ);
+  char *base = (char *)temp;
+  uint32_t *k0 = (uint32_t *)base, *v0 = (uint32_t *)(base + A), *k1 = (uint32_t *)(base + 2 * A),
+           *v1 = (uint32_t *)(base + 3 * A), *first = (uint32_t *)(base + 4 * A);
+  char *work = base + 5 * A;
+  size_t workBytes = tempBytes - 5 * A;
   unsigned blocks = (unsigned)((nQ + 255) / 256);
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_quad_keys, dim3(blocks), dim3(256), 0, s, w, k0, v0, nQ);

@@ -41,7 +41,7 @@ def test_device_sorted_offsets_match_g_array_sort(wh):
     assert _sha(got) == v["offsets_%dx%d_sha256" % wh]
 
 
-@pytest.mark.parametrize("wh", [(1, 300), (300, 1), (257, 190), (190, 257), (1024, 768)])
+@pytest.mark.parametrize("wh", [(1, 300), (300, 1), (257, 190), (190, 257), (1024, 768), (1023, 1001)])
 def test_device_sorted_offsets_closed_form(wh):
     """Larger and degenerate offset ranges against the closed form of that order: x^2 + y^2
     ascending, ties in descending row-major index."""
@@ -537,6 +537,29 @@ def test_batch_of_uneven_requests():
         ratios.append(r["meanError"] / single)
     print("batch vs single mean error ratios", np.round(ratios, 3))
     assert np.mean(ratios) <= 1.05, ratios
+
+
+def test_batch_of_rgb_requests_with_odd_sizes():
+    """An all-RGB batch (the multi-request kernel's three-colour scorer) whose images have odd
+    pixel counts large enough for the multi-block sorts of the offset tables: every request
+    succeeds, context pixels stay untouched and targets take corpus colours."""
+    sizes = [(1001, 769), (517, 1023), (403, 401)]
+    imgs, masks = [], []
+    for k, (h, w) in enumerate(sizes):
+        img = W.grating_noise(h, w, 3, 700 + k)
+        m = W.ellipse_mask(h, w, 2, 700 + k, axes=(0.08 * min(w, h), 0.2 * min(w, h)))
+        imgs.append(img)
+        masks.append((m * 255).astype(np.uint8))
+    params = [R.default_params(patchSize=p) for p in (10, 30, 50)]
+    print("odd RGB batch", sizes, flush=True)
+    rc, res, st, outs = R.image_synth_batch(imgs, masks, [R.T_RGB] * 3, params)
+    assert rc == 0
+    for img, m, r, out in zip(imgs, masks, res, outs):
+        assert r["error"] == 0 and r["nTargets"] == int((m != 0).sum())
+        sel = m != 0
+        assert np.array_equal(out[~sel], img[~sel])
+        colours = {tuple(c) for c in img[~sel].reshape(-1, 3)}
+        assert all(tuple(c) in colours for c in out[sel].reshape(-1, 3)[::11])
 
 
 def test_batch_device_buffers_and_autotuner():
