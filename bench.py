@@ -126,10 +126,10 @@ def bench_config5(args, np, torch, R, W):
     the plain method's time at the learned error."""
     t = time.time()
     model = R.Model(seed=1)
-    for i, (img, mask) in enumerate(W.training_textures(200, 512)):
+    for i, (img, mask) in enumerate(W.training_textures(args.train_textures, 512)):
         err, _ = model.log_pairs(img, mask, R.T_RGB, R.default_params(), max_pairs=4096, seed=i + 1)
         assert err == 0, err
-    losses = model.fit(1500, batch=1024, lr=1e-3, tau=0.1, seed=3)
+    losses = model.fit(args.train_steps, batch=1024, lr=1e-3, tau=0.1, seed=3)
     train_s = time.time() - t
     img, mask = W.config5(size=args.size or 4096)
     h, w, c = img.shape
@@ -166,13 +166,22 @@ def bench_config5(args, np, torch, R, W):
     # the plain method with more probes, until its error reaches the learned error or the
     # probe budget ends: its time there bounds the plain time at equal error from below
     ladder = []
-    for probes in (1000, 4000):
+    for probes in (() if args.no_ladder else (1000, 4000)):
         sec, err = timed(probes, False, 1, 0)
         ladder.append({"probes": probes, "ms": round(sec * 1e3, 1), "mean_match_error": round(err, 1)})
         if err <= err_l:
             break
+    if not ladder:   # profiling runs: the plain 200-probe run stands in
+        ladder.append({"probes": 200, "ms": round(sec_p * 1e3, 1), "mean_match_error": round(err_p, 1)})
     reached = ladder[-1]["mean_match_error"] <= err_l
     kern_ms = rl.waveKernelMs
+    # HBM bytes per launch of the learned call's k_synth_wave, from FETCH_SIZE/WRITE_SIZE
+    # counter runs of this workload (profiles/synth_wave_config5_traffic.md), when present
+    traffic5 = None
+    tfile5 = os.path.join(ROOT, "profiles", "synth_wave_config5_traffic.json")
+    if os.path.exists(tfile5):
+        with open(tfile5) as f:
+            traffic5 = json.load(f).get("hbm_bytes_per_launch")
     alg = rl.wavePatchTerms * rl.pixelWordBytes
     achieved = alg / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
     out = {
@@ -199,7 +208,7 @@ def bench_config5(args, np, torch, R, W):
                    "synth_kernels_ms": round(rl.synthKernelMs, 2), "passes": rl.passes,
                    "plain_synth_kernels_ms": round(rp.synthKernelMs, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_synth_wave",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic5, "kernel": "k_synth_wave",
                      "launches": rl.waveLaunches,
                      "avg_launch_ms": round(kern_ms / max(rl.waveLaunches, 1), 4)},
     }
@@ -272,6 +281,9 @@ def main():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--train-textures", type=int, default=200, help="config5: training textures (profiling runs)")
+    ap.add_argument("--train-steps", type=int, default=1500, help="config5: Adam steps (profiling runs)")
+    ap.add_argument("--no-ladder", action="store_true", help="config5: skip the plain probe ladder")
     ap.add_argument("--workload", default="config5", choices=["config2", "config3", "config4", "config5"],
                     help="config5 (default): learned proposals vs the plain method on a 4096x4096 heal; "
                          "config2: the plain method on a 2048x2048 heal; config3: 96 uneven heal "
