@@ -1741,32 +1741,34 @@ hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int
 //   y = -b rows in the reverse order, each (a, -b) then (-a, -b).
 
 // Quadrant entry q = b*w + a, generated in descending q so that a stable sort by the key
-// leaves ties in descending q, i.e. descending b.
-__global__ void k_quad_keys(int w, uint32_t *keys, uint32_t *vals, size_t nQ)
+// leaves ties in descending q, i.e. descending b.  The sorted payload is (b << shift) | a
+// (shift = bits of w - 1; it fits, as w*h < 2^31), so the later kernels need no division.
+__global__ void k_quad_keys(int w, unsigned shift, uint32_t *keys, uint32_t *vals, size_t nQ)
 {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nQ; e += (size_t)gridDim.x * blockDim.x) {
     const uint32_t q = (uint32_t)(nQ - 1 - e);
     const uint32_t a = q % (unsigned)w, b = q / (unsigned)w;
     keys[e] = a * a + b * b;
-    vals[e] = q;
+    vals[e] = (b << shift) | a;
   }
 }
 
 // Offsets a sorted quadrant entry stands for.
-__global__ void k_quad_members(int w, const uint32_t *vals, uint32_t *mem, size_t nQ)
+__global__ void k_quad_members(unsigned shift, const uint32_t *vals, uint32_t *mem, size_t nQ)
 {
+  const uint32_t amask = (1u << shift) - 1u;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nQ; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t q = vals[i];
-    const uint32_t a = q % (unsigned)w, b = q / (unsigned)w;
-    mem[i] = (a > 0 ? 2u : 1u) * (b > 0 ? 2u : 1u);
+    const uint32_t v = vals[i];
+    mem[i] = ((v & amask) > 0 ? 2u : 1u) * ((v >> shift) > 0 ? 2u : 1u);
   }
 }
 
 // first: exclusive prefix sums of the member counts, so that first[lo] is where the ring
 // of entries lo.. begins in the full table.
-__global__ void k_quad_expand(int w, const uint32_t *keys, const uint32_t *vals, const uint32_t *first, size_t nQ,
-                              int2 *out)
+__global__ void k_quad_expand(unsigned shift, const uint32_t *keys, const uint32_t *vals, const uint32_t *first,
+                              size_t nQ, int2 *out)
 {
+  const uint32_t amask = (1u << shift) - 1u;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nQ; i += (size_t)gridDim.x * blockDim.x) {
     const uint32_t D = keys[i];
     size_t lo = i, hi = i + 1;
@@ -1774,17 +1776,17 @@ __global__ void k_quad_expand(int w, const uint32_t *keys, const uint32_t *vals,
     while (hi < nQ && keys[hi] == D) hi++;
     uint32_t before = 0, rowsAbove = 0, axis = 0;   // y > 0 offsets before entry i; all of them; the y = 0 ones
     for (size_t j = lo; j < hi; j++) {
-      const uint32_t qj = vals[j];
-      const uint32_t m = (qj % (unsigned)w) > 0 ? 2u : 1u;
-      if (qj / (unsigned)w > 0) {
+      const uint32_t vj = vals[j];
+      const uint32_t m = (vj & amask) > 0 ? 2u : 1u;
+      if ((vj >> shift) > 0) {
         rowsAbove += m;
         if (j < i) before += m;
       } else {
         axis = m;
       }
     }
-    const uint32_t q = vals[i];
-    const int a = (int)(q % (unsigned)w), b = (int)(q / (unsigned)w);
+    const uint32_t v = vals[i];
+    const int a = (int)(v & amask), b = (int)(v >> shift);
     const uint32_t m = a > 0 ? 2u : 1u;
     const size_t start = first[lo];
     if (b > 0) {
@@ -1844,14 +1846,16 @@ hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *t
   size_t workBytes = tempBytes - 5 * A;
   unsigned blocks = (unsigned)((nQ + 255) / 256);
   if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_quad_keys, dim3(blocks), dim3(256), 0, s, w, k0, v0, nQ);
+  unsigned shift = 1;
+  while (shift < 31 && ((unsigned)(w - 1) >> shift)) shift++;   // bits of w - 1 (at least 1)
+  hipLaunchKernelGGL(k_quad_keys, dim3(blocks), dim3(256), 0, s, w, shift, k0, v0, nQ);
   hipError_t e = rocprim::radix_sort_pairs<rocprim::default_config>(work, workBytes, k0, k1, v0, v1, nQ, 0,
                                                                     key_bits(w, h), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_quad_members, dim3(blocks), dim3(256), 0, s, w, v1, k0, nQ);
+  hipLaunchKernelGGL(k_quad_members, dim3(blocks), dim3(256), 0, s, shift, v1, k0, nQ);
   e = rocprim::exclusive_scan(work, workBytes, k0, first, 0u, nQ, rocprim::plus<uint32_t>(), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_quad_expand, dim3(blocks), dim3(256), 0, s, w, k1, v1, first, nQ, out);
+  hipLaunchKernelGGL(k_quad_expand, dim3(blocks), dim3(256), 0, s, shift, k1, v1, first, nQ, out);
   return hipGetLastError();
 }
 
