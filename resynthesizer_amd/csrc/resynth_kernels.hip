@@ -50,8 +50,9 @@ __device__ __forceinline__ uint32_t neighbour_term(Word t, Word q, bool first, i
 }
 
 // Accumulates neighbours [base, base+G) of candidate (cx,cy): the G corpus words are
-// loaded first, then summed in neighbour order with the early out below.
-template <typename Word, int G>
+// loaded first, then summed in neighbour order with the early out below.  FIRST is false
+// for groups that start past neighbour 0, which then need no test for it.
+template <typename Word, int G, bool FIRST>
 __device__ __forceinline__ void score_group(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
                                             unsigned base, unsigned count, const int2 *nbOff, const Word *nbPix,
                                             const uint16_t *T, const uint32_t *Tm, uint32_t B0, uint32_t &acc,
@@ -75,7 +76,7 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
     unsigned i = base + u;
     if (i < count && !dead) {
       uint32_t term = (in[u] && (q[u] & 0xFEu) == 0xFEu)
-                          ? neighbour_term<Word>(nbPix[i], q[u], i == 0, g.nColors, g.nMaps, T, Tm)
+                          ? neighbour_term<Word>(nbPix[i], q[u], FIRST && i == 0, g.nColors, g.nMaps, T, Tm)
                           : g.penalty;
       acc += term;
       mx = acc > mx ? acc : mx;
@@ -87,7 +88,7 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
 // The same accumulation without branches, for NC colour channels and no map channels:
 // every term is computed (table indices stay in [1, 511] whatever the loaded word holds)
 // and selected, so the wave does not diverge per neighbour.  The sums are identical.
-template <typename Word, int G, int NC>
+template <typename Word, int G, int NC, bool FIRST>
 __device__ __forceinline__ void score_group_rgb(const Geometry &g, const Word *__restrict__ cword, int cx, int cy,
                                                 unsigned base, unsigned count, const int2 *nbOff, const Word *nbPix,
                                                 const uint16_t *T, uint32_t B0, uint32_t &acc, uint32_t &mx,
@@ -115,7 +116,7 @@ __device__ __forceinline__ void score_group_rgb(const Geometry &g, const Word *_
     uint32_t cs = 0;
 #pragma unroll
     for (int c = 0; c < NC; c++) cs += T[256u + chan(t, c) - chan(q[u], c)];
-    cs = (i == 0) ? 0u : cs;
+    if constexpr (FIRST) cs = (i == 0) ? 0u : cs;
     const bool valid = in[u] && (q[u] & 0xFEu) == 0xFEu;
     const uint32_t term = valid ? cs : g.penalty;
     const bool act = i < count && !dead;
@@ -145,13 +146,13 @@ __device__ __forceinline__ void score_candidate(const Geometry &g, const Word *_
   bool dead = false;
   unsigned base = (skip0 && B0 > 0) ? 1u : 0u;
   if constexpr (NC > 0) {
-    score_group_rgb<Word, G0, NC>(g, cword, cx, cy, base, count, nbOff, nbPix, T, B0, acc, mx, dead);
+    score_group_rgb<Word, G0, NC, true>(g, cword, cx, cy, base, count, nbOff, nbPix, T, B0, acc, mx, dead);
     for (base += G0; base < count && !dead; base += GL)
-      score_group_rgb<Word, GL, NC>(g, cword, cx, cy, base, count, nbOff, nbPix, T, B0, acc, mx, dead);
+      score_group_rgb<Word, GL, NC, false>(g, cword, cx, cy, base, count, nbOff, nbPix, T, B0, acc, mx, dead);
   } else {
-    score_group<Word, G0>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+    score_group<Word, G0, true>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
     for (base += G0; base < count && !dead; base += GL)
-      score_group<Word, GL>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
+      score_group<Word, GL, false>(g, cword, cx, cy, base, count, nbOff, nbPix, T, Tm, B0, acc, mx, dead);
   }
   F = acc;
   M = mx;
@@ -997,9 +998,11 @@ __device__ __forceinline__ void flush_counters(const SynthBuffers &b, unsigned l
 
 // Batched visits, one wavefront per target (large chunks: every wave stays busy on its
 // own target instead of idling through another wave's gather and heuristic-1 phases).
+// The kernel is bound by VALU issue and latency, so the small-group shapes ask for 7 waves
+// per SIMD (at most 72 VGPRs; they fit without spilling); the larger groups would spill.
 template <typename Word, int WAVES, int G0, int GL, int NC = 0>
-__global__ __launch_bounds__(64 * WAVES) void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1,
-                                                           unsigned seed, unsigned pass)
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu((GL == 4 && G0 <= 4) ? 7 : 1)))
+void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned seed, unsigned pass)
 {
   __shared__ uint16_t sT[512];
   __shared__ uint32_t sTm[512];
