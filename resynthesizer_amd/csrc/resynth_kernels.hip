@@ -1227,9 +1227,12 @@ __device__ __forceinline__ unsigned long long first_lane64(unsigned long long v)
 // Tickets a wave takes at once: one counter update per group instead of per visit.
 constexpr unsigned kTicketGroup = 4;
 
+// 32-bit words with groups of 2 then 4 ask for 6 waves per SIMD (at most 80 VGPRs), which
+// they reach without spilling; the resident grid below follows from the occupancy query.
 template <typename Word, int WAVES, int G0, int GL, int NC = 0>
-__global__ __launch_bounds__(64 * WAVES) void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls,
-                                                           unsigned nReq, uint32_t *global)
+__global__ __launch_bounds__(64 * WAVES)
+__attribute__((amdgpu_waves_per_eu((sizeof(Word) == 4 && G0 == 2 && GL == 4) ? 6 : 1)))
+void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
 {
   __shared__ uint16_t sT[WAVES][512];
   __shared__ uint32_t sTm[WAVES][512];
