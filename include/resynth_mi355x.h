@@ -128,6 +128,28 @@ void interleave_mask(Map *pixmap, Map *mask);
 
 /* ======================================================================= Part 2 */
 
+/* Return codes beyond TImageSynthError.  No counterpart in the reference, whose engine
+ * runs on the host and has no device to fail.  Every entry point returns one of these
+ * instead of terminating the caller's process:
+ *   RESYNTH_ERROR_INVALID_ARGUMENT: a Part 2 argument is invalid (null pointer, a row
+ *     pitch shorter than the row, a kernel shape that does not exist, ...);
+ *   RESYNTH_ERROR_DEVICE: no usable HIP device, or a HIP call failed (out of device
+ *     memory, a failed launch); the caller's image is left as it was unless the failure
+ *     came during the final write-back;
+ *   RESYNTH_ERROR_TOO_LARGE: an image exceeds the library's size limits (fewer than 2^31
+ *     pixels per image, sides below 2^24, fewer than 2^32 sorted offsets), or the
+ *     allocation-free entry's maximum. */
+#define RESYNTH_ERROR_INVALID_ARGUMENT (-1)
+#define RESYNTH_ERROR_DEVICE 100
+#define RESYNTH_ERROR_TOO_LARGE 101
+
+/* Caps the device memory the library's grow-only engine buffers may hold (bytes; 0 = no
+ * cap, the default).  An engine call whose buffers would exceed it fails with
+ * RESYNTH_ERROR_DEVICE, as it would when hipMalloc runs out of memory.  For callers that
+ * share a GPU with other work, and for tests of the out-of-memory path.  A cap below what
+ * the library holds releases its idle engine buffers. */
+void resynth_set_device_memory_limit(size_t bytes);
+
 /* Synthesis modes.  No counterpart in the reference (which has one serial build and
  * one nondeterministic threaded build, lib/buildSwitches.h:79).
  *   RESYNTH_MODE_SERIAL:  the reference's serial visiting order and PRNG stream,
@@ -136,7 +158,8 @@ void interleave_mask(Map *pixmap, Map *mask);
  *                         target, per-target counter-based random probes; judged by
  *                         mean best-match error, like the reference's threaded build.
  * The default for Part 1 entry points is RESYNTH_MODE_BATCHED unless the environment
- * variable RESYNTH_MODE is "serial", or resynth_set_default_mode() was called. */
+ * variable RESYNTH_MODE is "serial", or resynth_set_default_mode() was called.  Their PRNG
+ * is GRand unless the environment variable RESYNTH_PRNG is "libc". */
 enum { RESYNTH_MODE_SERIAL = 0, RESYNTH_MODE_BATCHED = 1 };
 
 /* PRNG of the serial mode: GLib GRand (the GIMP build) or the C library's rand()
@@ -265,8 +288,12 @@ typedef struct {
 /* Runs n requests.  deviceBuffers != 0: image.data and mask.data are device pointers.
  * options may be NULL (batched defaults; the mode field is ignored: always batched);
  * config may be NULL (the autotuned shape if resynth_autotune_batch ran, else 4/2/4).
- * Returns 0, or -1 on invalid arguments or no device; per-request results are in
- * requests[i].error and the other out fields. */
+ * Returns 0, RESYNTH_ERROR_INVALID_ARGUMENT, or RESYNTH_ERROR_DEVICE; per-request results
+ * are in requests[i].error (RESYNTH_ERROR_INVALID_ARGUMENT for a request whose rowBytes is
+ * shorter than width * bytes per pixel, or whose mask rowBytes is shorter than width) and
+ * the other out fields.  Of the options, seed, batch, maxPasses and rampPermille apply per
+ * request; mode, timing, model and the learned* fields are ignored (always the plain
+ * batched method). */
 int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int deviceBuffers,
                               const ResynthOptions *options, const ResynthKernelConfig *config,
                               ResynthBatchStats *stats);
@@ -275,7 +302,8 @@ int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int devi
  * firstGroup x laterGroup, 12 shapes) on copies of the sample requests, keeps the fastest
  * for later resynth_image_synth_batch calls and returns it in *best.  msPerConfig (may be
  * NULL) receives the 12 timings in the order waves {2,4,8} x first {2,4} x later {4,8}.
- * The sample's images are left unchanged.  Returns 0, or -1 on invalid arguments. */
+ * The sample's images are left unchanged.  Returns 0, RESYNTH_ERROR_INVALID_ARGUMENT or
+ * RESYNTH_ERROR_DEVICE. */
 int resynth_autotune_batch(const ResynthRequest *sample, unsigned int n, int deviceBuffers,
                            ResynthKernelConfig *best, double *msPerConfig);
 

@@ -23,6 +23,10 @@ IMAGE_SYNTH_ERROR_PATCH_SIZE_EXCEEDED = 3
 IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE = 4
 IMAGE_SYNTH_ERROR_EMPTY_TARGET = 5
 IMAGE_SYNTH_ERROR_EMPTY_CORPUS = 6
+# Return codes beyond TImageSynthError (include/resynth_mi355x.h)
+RESYNTH_ERROR_INVALID_ARGUMENT = -1
+RESYNTH_ERROR_DEVICE = 100
+RESYNTH_ERROR_TOO_LARGE = 101
 
 MODE_SERIAL, MODE_BATCHED = 0, 1
 PRNG_GRAND, PRNG_LIBC = 0, 1
@@ -135,6 +139,7 @@ SIGNATURES = {
     "invert_bytemap": (None, [_P(Map)]),
     "interleave_mask": (None, [_P(Map), _P(Map)]),
     "resynth_set_default_mode": (None, [ctypes.c_int]),
+    "resynth_set_device_memory_limit": (None, [ctypes.c_size_t]),
     "resynth_default_options": (None, [_P(ResynthOptions)]),
     "imageSynthEx": (ctypes.c_int, [_P(ImageBuffer), _P(ImageBuffer), _P(ImageBuffer), ctypes.c_int,
                                     _P(TImageSynthParameters), PROGRESS, ctypes.c_void_p, _P(ctypes.c_int),
@@ -589,6 +594,11 @@ def order_targets(match_type, points, prng=PRNG_GRAND):
     pts = np.ascontiguousarray(points, dtype=np.int32).copy()
     err = lib().resynth_order_targets(match_type, prng, pts.ctypes.data, len(pts))
     return err, pts
+
+
+def set_device_memory_limit(nbytes):
+    """Caps the device memory of the library's engine buffers (0 = no cap)."""
+    lib().resynth_set_device_memory_limit(int(nbytes))
 
 
 def device_info():

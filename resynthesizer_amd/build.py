@@ -18,7 +18,8 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-ffp-contract=off", "-I" + os.path.join(HERE, "..", "include")]
 
-SOURCES = ["resynth_kernels.hip", "resynth_host.cpp", "resynth_learn.hip", "resynth_learn_host.cpp"]
+SOURCES = ["resynth_kernels.hip", "resynth_host.cpp", "resynth_learn.hip", "resynth_learn_host.cpp",
+           "resynth_order.hip"]
 
 
 def _compile(src):

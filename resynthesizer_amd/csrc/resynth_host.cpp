@@ -18,11 +18,18 @@
 #include <functional>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <thread>
 #include <vector>
 
 using namespace rs;
+
+void rs::hip_fail(hipError_t e, const char *expr, const char *file, int line)
+{
+  std::fprintf(stderr, "resynth_mi355x: %s failed: %s (%s:%d)\n", expr, hipGetErrorString(e), file, line);
+  throw HipFailure{e};
+}
 
 namespace {
 
@@ -35,14 +42,12 @@ double now_ms()
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// A failed HIP call unwinds to the C entry point (see `guarded`), which returns
+// RESYNTH_ERROR_DEVICE to the caller.
 #define HIP_TRY(expr)                                                                                 \
   do {                                                                                                \
     hipError_t e_ = (expr);                                                                           \
-    if (e_ != hipSuccess) {                                                                           \
-      std::fprintf(stderr, "resynth_mi355x: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_),  \
-                   __FILE__, __LINE__);                                                               \
-      std::abort();                                                                                   \
-    }                                                                                                 \
+    if (e_ != hipSuccess) hip_fail(e_, #expr, __FILE__, __LINE__);                                    \
   } while (0)
 
 // ------------------------------------------------------------------ PRNG streams
@@ -126,18 +131,21 @@ struct Prng {
       return lc.next() / (2147483647u / n + 1u);
     }
     if (n == 0) return 0;
-    uint32_t maxvalue;
-    if (n <= 0x80000000u) {
-      uint32_t leftover = (0x80000000u % n) * 2u;
-      if (leftover >= n) leftover -= n;
-      maxvalue = 0xFFFFFFFFu - leftover;
-    } else {
-      maxvalue = n - 1u;
+    if (n != rangeN) {   // the rejection bound of the last n (the band shuffle repeats n)
+      rangeN = n;
+      if (n <= 0x80000000u) {
+        uint32_t leftover = (0x80000000u % n) * 2u;
+        if (leftover >= n) leftover -= n;
+        rangeMax = 0xFFFFFFFFu - leftover;
+      } else {
+        rangeMax = n - 1u;
+      }
     }
     uint32_t v;
-    do v = mt.next(); while (v > maxvalue);
+    do v = mt.next(); while (v > rangeMax);
     return v % n;
   }
+  uint32_t rangeN = 0, rangeMax = 0;
   // The random target order of types 0 and 1 (lib/orderTarget.h:36-47): for i in 0..n-1,
   // swap(perm[i], perm[int_range(0, n)]), perm starting as the identity.  Every draw has the
   // same n, so the rejection bound and the reduction are set up once (v % n by Lemire's
@@ -196,126 +204,159 @@ void quantize_tables(double sensitivity, double mapWeight, Tables &t)
 
 // ------------------------------------------------------------------ ordering
 
-// Top-down merge sort taking the left element iff cmp(left, right) <= 0: the sort behind
-// g_array_sort, whose order matters here because the engine's comparators never return 0.
-template <typename T, typename Cmp>
-void merge_sort(std::vector<T> &v, Cmp cmp)
+// The banded shuffle of lib/orderTarget.h:58-79 (halfBand = (gint)(n * 0.1)) applied to
+// the identity: out[i] is the sorted position that ends at position i.  It consumes the
+// engine's PRNG exactly as the reference's loop does; the device sorts the targets
+// meanwhile (types 2..8, resynth_order.hip) and composes the two.
+void band_permutation(uint32_t *perm, uint32_t n, Prng &prng)
 {
-  std::vector<T> tmp(v.size());
-  struct Rec {
-    static void run(T *b, size_t n, T *t, Cmp &cmp)
-    {
-      if (n <= 1) return;
-      size_t n1 = n / 2, n2 = n - n1;
-      run(b, n1, t, cmp);
-      run(b + n1, n2, t, cmp);
-      T *l = b, *r = b + n1, *o = t;
-      while (n1 && n2) {
-        if (cmp(*l, *r) <= 0) { *o++ = *l++; n1--; }
-        else { *o++ = *r++; n2--; }
-      }
-      while (n1) { *o++ = *l++; n1--; }
-      std::copy(t, o, b);
-    }
-  };
-  if (!v.empty()) Rec::run(v.data(), v.size(), tmp.data(), cmp);
-}
-
-struct Pt {
-  int32_t x, y;
-};
-struct SortPt {
-  int32_t x, y;
-  float prop;
-};
-
-// lib/orderTarget.h:58-79
-void randomize_bands(std::vector<Pt> &pts, Prng &prng)
-{
-  const int n = (int)pts.size();
-  const int last = n - 1;
+  for (uint32_t i = 0; i < n; i++) perm[i] = i;
+  if (n == 0) return;
+  const int last = (int)n - 1;
   const int half = (int)(n * 0.1);
   for (int i = 0; i <= last; i++) {
-    int start = std::max(i - half, 0);
-    int end = std::min(i + half, last);
-    int j = start + (int)prng.range((uint32_t)(end - start));
-    std::swap(pts[i], pts[j]);
+    const int start = std::max(i - half, 0);
+    const int end = std::min(i + half, last);
+    const int j = start + (int)prng.range((uint32_t)(end - start));
+    std::swap(perm[i], perm[j]);
   }
 }
 
-// lib/orderTarget.h:269-343 for types 2..8 (types 0/1 are a plain index permutation,
-// see order_random).  Returns the error of an out-of-range type.
-int order_directional(int type, std::vector<Pt> &pts, Prng &prng)
+// grad (lib/brushfire.h:63-67) of the nine directions where the ray index falls exactly on
+// a boundary, in the order the device expects: centre, +x, -x, +y, -y, (+,+), (-,+),
+// (-,-), (+,-).  Computed with this host's C library, as the reference computes them.
+void axis_rays(uint32_t out[9])
 {
-  int32_t ulx = INT32_MAX, uly = INT32_MAX, lrx = 0, lry = 0;   // lib/engineTypes.h:208-226
-  for (auto &p : pts) {
-    ulx = std::min(ulx, p.x); uly = std::min(uly, p.y);
-    lrx = std::max(lrx, p.x); lry = std::max(lry, p.y);
-  }
-  const int32_t cx = (lrx - ulx) / 2 + ulx, cy = (lry - uly) / 2 + uly;
-  for (auto &p : pts) { p.x -= cx; p.y -= cy; }
-  auto lessOf = [](bool less) { return less ? -1 : 1; };
-  auto invert = [](int r) { return r == 1 ? -1 : 1; };
-  if (type == 2 || type == 5 || type == 8) {
-    // lib/brushfire.h:63-148: proportion of the distance to the farthest point on the ray
-    auto grad = [](int32_t x, int32_t y) {
-      return (unsigned)(std::atan2((double)(float)y, (double)(float)x) * 200 / 3.14159265358979323846 + 200);
-    };
-    unsigned maxc[401] = {0};
-    for (auto &p : pts) {
-      unsigned c = (unsigned)(p.x * p.x + p.y * p.y);
-      unsigned r = grad(p.x, p.y);
-      if (c > maxc[r]) maxc[r] = c;
-    }
-    std::vector<SortPt> s(pts.size());
-    for (size_t i = 0; i < pts.size(); i++)
-      s[i] = {pts[i].x, pts[i].y, (float)(pts[i].y * pts[i].y + pts[i].x * pts[i].x) / (float)maxc[grad(pts[i].x, pts[i].y)]};
-    if (type == 5) merge_sort(s, [&](const SortPt &a, const SortPt &b) { return lessOf(a.prop < b.prop); });
-    else merge_sort(s, [&](const SortPt &a, const SortPt &b) { return invert(lessOf(a.prop < b.prop)); });
-    // type 8 (lib/orderTarget.h:209-262) overwrites its interleaving with this same order
-    for (size_t i = 0; i < pts.size(); i++) pts[i] = {s[i].x, s[i].y};
-  } else {
-    bool horiz = (type == 3 || type == 6);
-    bool more = (type == 3 || type == 4);
-    merge_sort(pts, [&](const Pt &a, const Pt &b) {
-      int r = horiz ? lessOf(a.x * a.x < b.x * b.x) : lessOf(a.y * a.y < b.y * b.y);
-      return more ? invert(r) : r;
-    });
-  }
-  for (auto &p : pts) { p.x += cx; p.y += cy; }
-  randomize_bands(pts, prng);
-  return 0;
+  static const int d[9][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {-1, -1}, {1, -1}};
+  for (int k = 0; k < 9; k++)
+    out[k] = (uint32_t)(std::atan2((double)(float)d[k][1], (double)(float)d[k][0]) * 200 / 3.14159265358979323846 +
+                        200);
 }
 
 // ------------------------------------------------------------------ device state
 
+// Bump allocator over a fixed region (the allocation-free entry's scratch): carve() hands
+// out 256-byte aligned pieces until the region is spent, then fails like hipMalloc.
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+  void *carve(size_t bytes)
+  {
+    const size_t at = (used + 255) & ~(size_t)255;
+    if (!base || at > size || bytes > size - at) hip_fail(hipErrorOutOfMemory, "scratch carve", __FILE__, __LINE__);
+    used = at + bytes;
+    return base + at;
+  }
+};
+
+// While set (by the allocation-free entry, under the library lock), Buf and PinnedBuf carve
+// from these regions instead of allocating.
+Arena *g_devArena = nullptr, *g_hostArena = nullptr;
+
+// Device bytes held by grow-only buffers, and the cap set by resynth_set_device_memory_limit.
+size_t g_devBytes = 0, g_devLimit = 0;
+
 struct Buf {
   void *p = nullptr;
   size_t cap = 0;
+  bool carved = false;
+  Buf() = default;
+  Buf(const Buf &) = delete;
+  Buf &operator=(const Buf &) = delete;
+  ~Buf() { release(); }
   void *get(size_t bytes)
   {
     if (bytes > cap) {
-      if (p) HIP_TRY(hipFree(p));
-      size_t c = std::max(bytes, cap + cap / 2);
+      if (g_devArena) {
+        release();
+        p = g_devArena->carve(bytes);
+        cap = bytes;
+        carved = true;
+        return p;
+      }
+      const size_t c = std::max(bytes, cap + cap / 2);
+      release();
+      if (g_devLimit && g_devBytes + c > g_devLimit)
+        hip_fail(hipErrorOutOfMemory, "device memory limit", __FILE__, __LINE__);
       HIP_TRY(hipMalloc(&p, c));
       cap = c;
+      g_devBytes += c;
     }
     return p;
+  }
+  void release()
+  {
+    if (p && !carved) {
+      (void)hipFree(p);
+      g_devBytes -= cap;
+    }
+    p = nullptr;
+    cap = 0;
+    carved = false;
   }
 };
 
 struct PinnedBuf {
   void *p = nullptr;
   size_t cap = 0;
+  bool carved = false;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf()
+  {
+    if (p && !carved) (void)hipHostFree(p);
+  }
   void *get(size_t bytes)
   {
     if (bytes > cap) {
-      if (p) HIP_TRY(hipHostFree(p));
-      size_t c = std::max(bytes, cap + cap / 2);
+      if (p && !carved) (void)hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      carved = false;
+      if (g_hostArena) {
+        p = g_hostArena->carve(bytes);
+        cap = bytes;
+        carved = true;
+        return p;
+      }
+      const size_t c = std::max(bytes, cap + cap / 2);
       HIP_TRY(hipHostMalloc(&p, c, hipHostMallocDefault));
       cap = c;
     }
     return p;
+  }
+  // Grows to `bytes` keeping the first `keep` bytes of the contents.
+  void *grow_keep(size_t bytes, size_t keep)
+  {
+    if (bytes <= cap) return p;
+    void *old = p;
+    const bool oldCarved = carved;
+    void *q = nullptr;
+    if (g_hostArena) {
+      q = g_hostArena->carve(bytes);
+    } else {
+      HIP_TRY(hipHostMalloc(&q, bytes, hipHostMallocDefault));
+    }
+    if (old && keep) std::memcpy(q, old, std::min(keep, cap));
+    if (old && !oldCarved) (void)hipHostFree(old);
+    p = q;
+    cap = bytes;
+    carved = g_hostArena != nullptr;
+    return p;
+  }
+};
+
+// Device memory owned by one call (test hooks, the autotuner's sample copies).
+struct TmpBuf {
+  void *p = nullptr;
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes < 256 ? 256 : bytes); }
+  TmpBuf() = default;
+  TmpBuf(const TmpBuf &) = delete;
+  TmpBuf &operator=(const TmpBuf &) = delete;
+  ~TmpBuf()
+  {
+    if (p) (void)hipFree(p);
   }
 };
 
@@ -329,7 +370,11 @@ struct Workspace {
       bMembers, qX, qH1, qH2, qE, qZ, qCl, props, propsAll, pairPts, pairSrc, superCent, superId, sCount, sStart,
       sCursor, sMembers, subCent, tileStart, qSuper, sortK0, sortV0, sortK1, sortV1, sortTemp, keyZs, keyPtsS, qK0,
       qV0, qK1, qV1, qSortTemp;
-  PinnedBuf pinCounts, pinTables, pinPerm;
+  // serial mode: raw PRNG outputs and the stream state
+  Buf streamBuf, streamState;
+  // target ordering of types 2..8
+  Buf orderRays, orderTemp;
+  PinnedBuf pinCounts, pinTables, pinPerm, pinStream;
 };
 
 struct DeviceState {
@@ -339,12 +384,14 @@ struct DeviceState {
   hipStream_t stream = nullptr;
   Workspace ws;                                   // single-request entries
   std::vector<std::unique_ptr<Workspace>> pool;   // one per request of a multi-request call
-  Buf streamBuf, streamState, manyDesc, manyCtl, manyGlobal;
-  PinnedBuf pinStream, pinDesc, pinCtl;
+  Buf manyDesc, manyCtl, manyGlobal;
+  PinnedBuf pinDesc, pinCtl;
+  hipEvent_t countsReady = nullptr;               // prepare_engine: point counts on the host
 };
 
 std::mutex g_mutex;
-DeviceState g_dev;
+// Never destroyed: its buffers must not be freed after the HIP runtime has shut down.
+DeviceState &g_dev = *new DeviceState;
 int g_defaultMode = -1;
 
 bool ensure_device()
@@ -356,10 +403,39 @@ bool ensure_device()
     std::fprintf(stderr, "resynth_mi355x: no HIP device is available\n");
     return false;
   }
-  HIP_TRY(hipGetDevice(&g_dev.device));
-  HIP_TRY(hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking));
+  if (hipGetDevice(&g_dev.device) != hipSuccess ||
+      hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&g_dev.countsReady, hipEventDisableTiming) != hipSuccess) {
+    std::fprintf(stderr, "resynth_mi355x: the HIP device could not be set up\n");
+    return false;
+  }
   g_dev.ok = true;
   return true;
+}
+
+// Clears what a failed HIP call may have left: work still queued on the library's stream
+// and a non-sticky error code.
+void recover_after_failure()
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  if (g_dev.ok) (void)hipStreamSynchronize(g_dev.stream);
+  (void)hipGetLastError();
+}
+
+// Runs the body of a C entry point; a HIP failure (hip_fail) or host memory exhaustion
+// inside becomes RESYNTH_ERROR_DEVICE.
+template <typename F>
+int guarded(F &&body)
+{
+  try {
+    return body();
+  } catch (const HipFailure &) {
+    recover_after_failure();
+    return RESYNTH_ERROR_DEVICE;
+  } catch (const std::bad_alloc &) {
+    recover_after_failure();
+    return RESYNTH_ERROR_DEVICE;
+  }
 }
 
 int default_mode()
@@ -445,7 +521,7 @@ struct Prepared {
   unsigned sizes[6] = {};
   unsigned nT = 0;
   size_t wb = 4;
-  std::unique_ptr<Prng> prng;   // the engine's PRNG after the ordering draws
+  Prng prng{RESYNTH_PRNG_GRAND, kEngineSeed};   // the engine's PRNG after the ordering draws
   // Types 0 and 1 with a deferred order: the shuffle runs on a host thread (it needs only
   // the target count) while the device works; finish_order joins it and gathers the
   // row-major targets in that order.
@@ -453,6 +529,7 @@ struct Prepared {
   bool orderPending = false;
   uint32_t *hPerm = nullptr, *dPerm = nullptr;
   const int2 *targetsRM = nullptr;
+  const uint32_t *sortedIdx = nullptr;   // types 2..8: row-major indices in sorted order (device)
   ~Prepared()
   {
     if (orderThread.joinable()) orderThread.join();
@@ -462,11 +539,14 @@ struct Prepared {
 void finish_order(Prepared &pr)
 {
   if (!pr.orderPending) return;
-  pr.orderThread.join();
+  if (pr.orderThread.joinable()) pr.orderThread.join();
   pr.orderPending = false;
   hipStream_t s = g_dev.stream;
   HIP_TRY(hipMemcpyAsync(pr.dPerm, pr.hPerm, (size_t)pr.nT * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  HIP_TRY(launch_gather_targets(s, pr.targetsRM, pr.dPerm, const_cast<int2 *>(pr.b.targets), pr.nT));
+  if (pr.sortedIdx)
+    HIP_TRY(launch_order_compose(s, pr.targetsRM, pr.sortedIdx, pr.dPerm, const_cast<int2 *>(pr.b.targets), pr.nT));
+  else
+    HIP_TRY(launch_gather_targets(s, pr.targetsRM, pr.dPerm, const_cast<int2 *>(pr.b.targets), pr.nT));
 }
 
 // Preparation of an engine run (lib/engine.c:539-660) on the job's workspace.  Returns a
@@ -485,8 +565,18 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   Source &S = job.src;
   const int tw = S.tw, th = S.th, cw = S.cw, ch = S.ch;
   const size_t nTpix = (size_t)tw * th, nCpix = (size_t)cw * ch;
-  // the kernels index pixels with 32-bit words (2^31 pixels is 8 GiB of words per image)
-  if (nTpix >= (1u << 31) || nCpix >= (1u << 31)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+  // The kernels index pixels with 32-bit words and multiply rows by __umul24 (sides below
+  // 2^24), and count the sorted offsets in 32 bits.
+  {
+    const uint64_t ow = (uint64_t)std::min(cw, tw), oh = (uint64_t)std::min(ch, th);
+    if (nTpix >= (1u << 31) || nCpix >= (1u << 31) || tw >= (1 << 24) || th >= (1 << 24) || cw >= (1 << 24) ||
+        ch >= (1 << 24) || (2 * ow - 1) * (2 * oh - 1) >= (1ull << 32))
+      return RESYNTH_ERROR_TOO_LARGE;
+    // the inward/outward orders admit one NaN ratio (the centre pixel) while offsets from
+    // the target's centre stay below 65536
+    const int t = job.P.matchContextType;
+    if ((t == 2 || t == 5 || t == 8) && (tw > 65536 || th > 65536)) return RESYNTH_ERROR_TOO_LARGE;
+  }
   job.nColors = ix.img_match_bpp;
   job.nMaps = ix.map_match_bpp;
   job.wide = job.nColors + job.nMaps > 3;
@@ -520,8 +610,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   HIP_TRY(launch_band_starts(s, corpusPts, dCounts + 1, ch, bandStart));
   unsigned *hCounts = (unsigned *)W.pinCounts.get(64);
   HIP_TRY(hipMemcpyAsync(hCounts, dCounts, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
-  hipEvent_t countsReady;
-  HIP_TRY(hipEventCreateWithFlags(&countsReady, hipEventDisableTiming));
+  hipEvent_t countsReady = g_dev.countsReady;
   HIP_TRY(hipEventRecord(countsReady, s));
 
   // Work that does not depend on the counts goes to the device now and overlaps the
@@ -561,7 +650,6 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
 
   HIP_TRY(hipEventSynchronize(countsReady));
-  HIP_TRY(hipEventDestroy(countsReady));
   const unsigned nT = hCounts[0], nC = hCounts[1];
   job.st.nTargets = nT;
   job.st.nCorpus = nC;
@@ -574,30 +662,39 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   // target order (lib/orderTarget.h:269-343) with the engine's PRNG
   // the engine's seed (lib/engine.c:643); a nonzero option seed gives the serial mode
   // another stream, for statistics over seeds
-  pr.prng.reset(new Prng(job.opt.prng, (serialMode && job.opt.seed) ? job.opt.seed : kEngineSeed));
-  Prng &prng = *pr.prng;
+  pr.prng = Prng(job.opt.prng, (serialMode && job.opt.seed) ? job.opt.seed : kEngineSeed);
+  Prng &prng = pr.prng;
   int2 *targets = (int2 *)W.targets.get((size_t)nT * sizeof(int2) + 64);
   const int type = P.matchContextType;
-  if (type == 0 || type == 1) {
-    pr.hPerm = (uint32_t *)W.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
-    pr.dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
-    pr.targetsRM = targetsRM;
-    Prng *pp = &prng;
-    uint32_t *hPerm = pr.hPerm;
-    pr.orderThread = std::thread([pp, hPerm, nT] { pp->shuffle(hPerm, nT); });
-    pr.orderPending = true;
-  } else if (type >= 2 && type <= 8) {
-    std::vector<Pt> pts(nT);
-    HIP_TRY(hipMemcpyAsync(pts.data(), targetsRM, (size_t)nT * sizeof(int2), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    order_directional(type, pts, prng);
-    Pt *hp = (Pt *)W.pinPerm.get((size_t)nT * sizeof(Pt) + 64);
-    std::copy(pts.begin(), pts.end(), hp);
-    HIP_TRY(hipMemcpyAsync(targets, hp, (size_t)nT * sizeof(int2), hipMemcpyHostToDevice, s));
-  } else {
+  if (type < 0 || type > 8) {
     HIP_TRY(hipStreamSynchronize(s));
     return IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE;
   }
+  // Types 0/1 are a shuffle of the row-major list; types 2..8 a device sort by direction
+  // followed by a banded shuffle.  The shuffles need only nT and the PRNG: a host thread
+  // draws them (as an index permutation) while the device works, and finish_order applies
+  // them.  The allocation-free entry draws them inline.
+  pr.hPerm = (uint32_t *)W.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
+  pr.dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
+  pr.targetsRM = targetsRM;
+  if (type >= 2) {
+    static uint32_t hRays[9];
+    static std::once_flag raysOnce;
+    std::call_once(raysOnce, [] { axis_rays(hRays); });
+    uint32_t *dRays = (uint32_t *)W.orderRays.get(64);
+    HIP_TRY(hipMemcpyAsync(dRays, hRays, sizeof hRays, hipMemcpyHostToDevice, s));
+    const size_t otb = order_temp_bytes(nT);
+    HIP_TRY(launch_order_sort(s, targetsRM, nT, type, dRays, W.orderTemp.get(otb), otb, &pr.sortedIdx));
+  }
+  Prng *pp = &prng;
+  uint32_t *hPerm = pr.hPerm;
+  auto draw = [pp, hPerm, nT, type] {
+    if (type <= 1) pp->shuffle(hPerm, nT);
+    else band_permutation(hPerm, nT, *pp);
+  };
+  if (g_devArena) draw();
+  else pr.orderThread = std::thread(draw);
+  pr.orderPending = true;
 
   Geometry g = {};
   g.tw = tw; g.th = th; g.cw = cw; g.ch = ch;
@@ -827,7 +924,7 @@ int refine_engine(Job &job, Prepared &pr)
   const bool serial = job.opt.mode == RESYNTH_MODE_SERIAL;
   // without a learned index to build first, nothing else can overlap the host shuffle
   if (serial || !job.opt.model) finish_order(pr);
-  Prng &prng = *pr.prng;
+  Prng &prng = pr.prng;
   Counters *cnt = b.counters;
   const uint32_t *lastBest = b.lastBest;
   const int2 *targets = b.targets;
@@ -839,7 +936,7 @@ int refine_engine(Job &job, Prepared &pr)
   // serial mode: raw PRNG outputs after the ordering draws, streamed to the device
   // raw outputs buffered on the device; RESYNTH_SERIAL_STREAM_CAP shrinks the buffer so
   // that tests exercise the refill path
-  uint32_t streamCap = 1u << 22;
+  uint32_t streamCap = g_devArena ? (1u << 20) : (1u << 22);   // the arena's share is 4 MB
   if (const char *capEnv = std::getenv("RESYNTH_SERIAL_STREAM_CAP")) {
     long v = std::atol(capEnv);
     if (v >= 1024) streamCap = (uint32_t)std::min<long>(v, 1L << 30);
@@ -848,10 +945,10 @@ int refine_engine(Job &job, Prepared &pr)
   StreamState *dState = nullptr, *hState = nullptr;
   uint32_t streamLen = 0, streamPos = 0;
   if (serial) {
-    dStream = (uint32_t *)g_dev.streamBuf.get((size_t)streamCap * 4);
-    hStream = (uint32_t *)g_dev.pinStream.get((size_t)streamCap * 4 + sizeof(StreamState) * 2);
+    dStream = (uint32_t *)W.streamBuf.get((size_t)streamCap * 4);
+    hStream = (uint32_t *)W.pinStream.get((size_t)streamCap * 4 + sizeof(StreamState) * 2);
     hState = (StreamState *)(hStream + streamCap);
-    dState = (StreamState *)g_dev.streamState.get(sizeof(StreamState));
+    dState = (StreamState *)W.streamState.get(sizeof(StreamState));
   }
   auto refill = [&]() {
     // keep the unconsumed tail, append fresh outputs, upload
@@ -976,20 +1073,21 @@ int refine_engine(Job &job, Prepared &pr)
             HIP_TRY(hipMemcpyAsync(hState, dState, sizeof(StreamState), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             streamPos = hState->pos;
-            if (hState->kDone == kk && streamPos == 0) {
+            // read before a grow below moves hState to the new (unwritten) pinned buffer
+            const uint32_t kDone = hState->kDone;
+            if (kDone == kk && streamPos == 0) {
               // one visit needs more draws than the stream holds: grow it
               if (streamCap >= (1u << 30)) {
                 std::fprintf(stderr, "resynth_mi355x: maxProbeCount too large for the serial stream\n");
-                std::abort();
+                return RESYNTH_ERROR_TOO_LARGE;
               }
-              std::vector<uint32_t> keep(hStream, hStream + streamLen);
               streamCap *= 2;
-              dStream = (uint32_t *)g_dev.streamBuf.get((size_t)streamCap * 4);
-              hStream = (uint32_t *)g_dev.pinStream.get((size_t)streamCap * 4 + sizeof(StreamState) * 2);
+              dStream = (uint32_t *)W.streamBuf.get((size_t)streamCap * 4);
+              hStream = (uint32_t *)W.pinStream.grow_keep((size_t)streamCap * 4 + sizeof(StreamState) * 2,
+                                                          (size_t)streamLen * 4);
               hState = (StreamState *)(hStream + streamCap);
-              std::copy(keep.begin(), keep.end(), hStream);
             }
-            kk = hState->kDone;
+            kk = kDone;
             if (kk < k1) refill();   // stopped for want of PRNG outputs
           }
         } else if (windowLen) {
@@ -1148,19 +1246,24 @@ bool remap_mask2(const ImageBuffer *image, const ImageBuffer *mask2, std::vector
 
 // `after` (may be null) runs under the library lock once the engine has succeeded, with the
 // run's device state still in the workspace.
-int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat format,
+int image_synth_impl(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat format,
                      TImageSynthParameters *params, void (*cb)(int, void *), void *ctx, int *cancelFlag,
                      const ResynthOptions *opt, ResynthStats *stats,
-                     const std::function<void(Job &, unsigned)> *after = nullptr)
+                     const std::function<void(Job &, unsigned)> *after)
 {
   // lib/imageSynth.c:62-137 / 140-217
   if (image->width != mask->width || image->height != mask->height) return IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
   TFormatIndices ix;
   if (prepareImageFormatIndicesFromFormatType(&ix, format)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+  // rows shorter than their pixels cannot be what the sizes say
+  if ((size_t)image->width * countPixelelsPerPixelForFormat(format) > image->rowBytes ||
+      (size_t)mask->width > mask->rowBytes || (image->width && image->height && (!image->data || !mask->data)))
+    return RESYNTH_ERROR_INVALID_ARGUMENT;
+  if (mask2 && (!mask2->data || (size_t)mask2->width > mask2->rowBytes)) return RESYNTH_ERROR_INVALID_ARGUMENT;
   std::vector<unsigned char> m2;
   if (mask2 && !remap_mask2(image, mask2, m2)) return IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
   std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
+  if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
   const double t0 = now_ms();
   hipStream_t s = g_dev.stream;
   const unsigned w = image->width, h = image->height;
@@ -1206,12 +1309,12 @@ int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, 
   return err;
 }
 
-int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+int engine_impl(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
                 void (*cb)(int, void *), void *ctx, int *cancelFlag, const ResynthOptions *opt, ResynthStats *stats)
 {
   if (parameters.patchSize > kMaxNeighbors) return IMAGE_SYNTH_ERROR_PATCH_SIZE_EXCEEDED;
   std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
+  if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
   const double t0 = now_ms();
   hipStream_t s = g_dev.stream;
   Job job;
@@ -1245,6 +1348,64 @@ int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *
   return err;
 }
 
+int image_synth_host(ImageBuffer *image, ImageBuffer *mask, ImageBuffer *mask2, TImageFormat format,
+                     TImageSynthParameters *params, void (*cb)(int, void *), void *ctx, int *cancelFlag,
+                     const ResynthOptions *opt, ResynthStats *stats,
+                     const std::function<void(Job &, unsigned)> *after = nullptr)
+{
+  if (!image || !mask) return RESYNTH_ERROR_INVALID_ARGUMENT;
+  return guarded([&] { return image_synth_impl(image, mask, mask2, format, params, cb, ctx, cancelFlag, opt, stats, after); });
+}
+
+int engine_host(TImageSynthParameters parameters, TFormatIndices *indices, Map *targetMap, Map *corpusMap,
+                void (*cb)(int, void *), void *ctx, int *cancelFlag, const ResynthOptions *opt, ResynthStats *stats)
+{
+  if (!indices || !targetMap || !corpusMap) return RESYNTH_ERROR_INVALID_ARGUMENT;
+  return guarded(
+      [&] { return engine_impl(parameters, indices, targetMap, corpusMap, cb, ctx, cancelFlag, opt, stats); });
+}
+
+// imageSynth on device buffers with workspace W (the library lock held, device ready).
+int image_synth_device_impl(Workspace &W, unsigned char *d_image, unsigned int width, unsigned int height,
+                            size_t rowBytes, const unsigned char *d_mask, size_t maskRowBytes,
+                            const unsigned char *d_mask2, size_t mask2RowBytes, TImageFormat imageFormat,
+                            const TImageSynthParameters *parameters, void (*progressCallback)(int, void *),
+                            void *contextInfo, int *cancelFlag, const ResynthOptions *options, ResynthStats *stats)
+{
+  TFormatIndices ix;
+  if (prepareImageFormatIndicesFromFormatType(&ix, imageFormat)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
+  const unsigned bpp = countPixelelsPerPixelForFormat(imageFormat);
+  if ((size_t)width * height && (!d_image || !d_mask || rowBytes < (size_t)width * bpp || maskRowBytes < width ||
+                                 (d_mask2 && mask2RowBytes < width)))
+    return RESYNTH_ERROR_INVALID_ARGUMENT;
+  const double t0 = now_ms();
+  Job job;
+  init_job(job, parameters, options, progressCallback, contextInfo, cancelFlag);
+  job.W = &W;
+  job.ix = ix;
+  Source &S = job.src;
+  S.simple = true;
+  S.tw = S.cw = (int)width;
+  S.th = S.ch = (int)height;
+  S.bpp = bpp;
+  S.image = d_image;
+  S.rowBytes = rowBytes;
+  S.mask = d_mask;
+  S.maskRowBytes = maskRowBytes;
+  S.mask2 = d_mask2;
+  S.mask2RowBytes = mask2RowBytes;
+  S.alphaIdx = (imageFormat == T_RGBA) ? 3 : (imageFormat == T_GrayA) ? 1 : -1;
+  unsigned nT = 0;
+  int err = ((size_t)width * height) ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
+  if (!err && !(cancelFlag && *cancelFlag))
+    HIP_TRY(launch_writeback_simple(g_dev.stream, job.wide, W.tword.p, (const int2 *)W.targets.p, nT, d_image,
+                                    rowBytes, S.bpp, job.nColors, (int)width));
+  HIP_TRY(hipStreamSynchronize(g_dev.stream));
+  job.st.totalMs = now_ms() - t0;
+  if (stats) *stats = job.st;
+  return err;
+}
+
 // ------------------------------------------------------------------ many requests
 
 ManyConfig g_tuned[2] = {{0, 0, 0}, {0, 0, 0}};   // per pixel-word width, from the autotuner
@@ -1258,21 +1419,33 @@ unsigned env_permille(const char *name, long dflt)
 }
 
 // Sliding-window parameters of a request: the single-request schedule's chunk sizes
-// halved (see refine_engine).
+// halved (see refine_engine), with the options' pass-0 ramp.
 void request_window(const ResynthOptions &o, unsigned nT, unsigned &minW, unsigned &ramp, unsigned &capW)
 {
   const unsigned cap = o.batch ? o.batch : std::min(65536u, std::max(64u, nT / 32u));
   const unsigned minChunk = std::min(cap, std::max(1u, std::min(64u, nT / 2048u)));
   minW = std::max(1u, minChunk / 2);
-  ramp = std::max(1u, env_permille("RESYNTH_RAMP_PERMILLE", 60) / 2);
+  const unsigned rampPermille =
+      o.rampPermille ? std::min(1000u, o.rampPermille) : env_permille("RESYNTH_RAMP_PERMILLE", 60);
+  ramp = std::max(1u, rampPermille / 2);
   capW = std::max(1u, cap / 2);
+}
+
+// A request whose buffers cannot be what its sizes say: null data, or a row pitch shorter
+// than a row of pixels (image) or of mask bytes.
+bool request_invalid(const ResynthRequest &q)
+{
+  const unsigned bpp = countPixelelsPerPixelForFormat(q.format);
+  if (!bpp || !q.image.width || !q.image.height) return false;   // reported by the usual checks
+  return !q.image.data || !q.mask.data || q.image.rowBytes < (size_t)q.image.width * bpp ||
+         (q.mask.width == q.image.width && q.mask.height == q.image.height && q.mask.rowBytes < q.mask.width);
 }
 
 int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOptions *opt, const ManyConfig *force,
               ResynthBatchStats *st)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
+  if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
   const double t0 = now_ms();
   hipStream_t s = g_dev.stream;
   while (g_dev.pool.size() < n) g_dev.pool.emplace_back(new Workspace());
@@ -1300,6 +1473,10 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
     const ImageBuffer *image = &q.image, *mask = &q.mask;
     if (image->width != mask->width || image->height != mask->height) {
       q.error = IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH;
+      continue;
+    }
+    if (request_invalid(q)) {
+      q.error = RESYNTH_ERROR_INVALID_ARGUMENT;
       continue;
     }
     if (prepareImageFormatIndicesFromFormatType(&job.ix, q.format)) {
@@ -1361,6 +1538,7 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
       std::copy(pr.sizes, pr.sizes + 6, d.sizes);
       d.seed = job.opt.seed ? job.opt.seed : kEngineSeed;
       request_window(job.opt, pr.nT, d.minW, d.ramp, d.capW);
+      d.maxPasses = job.opt.maxPasses;
     }
     ReqDesc *dD = (ReqDesc *)g_dev.manyDesc.get((size_t)m * sizeof(ReqDesc));
     ReqCtl *dC = (ReqCtl *)g_dev.manyCtl.get((size_t)m * sizeof(ReqCtl));
@@ -1451,33 +1629,38 @@ int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, Resynt
 {
   {
     std::lock_guard<std::mutex> lock(g_mutex);
-    if (!ensure_device()) return -1;
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
   }
   hipStream_t s = g_dev.stream;
   std::vector<ResynthRequest> reqs(sample, sample + n);
-  std::vector<void *> pristine(n, nullptr), work(n, nullptr), masks(n, nullptr);
+  std::vector<TmpBuf> pristine(n), work(n), masks(n);
   std::vector<size_t> bytes(n, 0), mbytes(n, 0);
   bool wideSeen[2] = {false, false};
   for (unsigned i = 0; i < n; i++) {
     const ResynthRequest &q = sample[i];
+    if (request_invalid(q)) {
+      // run_batch reports it; no pointer of the caller's is handed to the device
+      reqs[i].image.data = nullptr;
+      reqs[i].mask.data = nullptr;
+      continue;
+    }
     bytes[i] = q.image.rowBytes * q.image.height;
     mbytes[i] = q.mask.rowBytes * q.mask.height;
-    if (!bytes[i] || !mbytes[i]) continue;
-    HIP_TRY(hipMalloc(&pristine[i], bytes[i]));
-    HIP_TRY(hipMalloc(&work[i], bytes[i]));
-    HIP_TRY(hipMalloc(&masks[i], mbytes[i]));
+    HIP_TRY(pristine[i].alloc(bytes[i]));
+    HIP_TRY(work[i].alloc(bytes[i]));
+    HIP_TRY(masks[i].alloc(mbytes[i]));
     const hipMemcpyKind k = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    HIP_TRY(hipMemcpyAsync(pristine[i], q.image.data, bytes[i], k, s));
-    HIP_TRY(hipMemcpyAsync(masks[i], q.mask.data, mbytes[i], k, s));
-    reqs[i].image.data = (unsigned char *)work[i];
-    reqs[i].mask.data = (unsigned char *)masks[i];
+    HIP_TRY(hipMemcpyAsync(pristine[i].p, q.image.data, bytes[i], k, s));
+    HIP_TRY(hipMemcpyAsync(masks[i].p, q.mask.data, mbytes[i], k, s));
+    reqs[i].image.data = (unsigned char *)work[i].p;
+    reqs[i].mask.data = (unsigned char *)masks[i].p;
     TFormatIndices ix;
     if (!prepareImageFormatIndicesFromFormatType(&ix, q.format))
       wideSeen[(ix.img_match_bpp + ix.map_match_bpp) > 3 ? 1 : 0] = true;
   }
   auto restore = [&]() {
     for (unsigned i = 0; i < n; i++)
-      if (work[i]) HIP_TRY(hipMemcpyAsync(work[i], pristine[i], bytes[i], hipMemcpyDeviceToDevice, s));
+      if (work[i].p) HIP_TRY(hipMemcpyAsync(work[i].p, pristine[i].p, bytes[i], hipMemcpyDeviceToDevice, s));
   };
   double bestMs = 1e30;
   ManyConfig bestCfg = kDefaultMany;
@@ -1501,11 +1684,6 @@ int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, Resynt
         c++;
       }
   HIP_TRY(hipStreamSynchronize(s));
-  for (unsigned i = 0; i < n; i++) {
-    if (pristine[i]) HIP_TRY(hipFree(pristine[i]));
-    if (work[i]) HIP_TRY(hipFree(work[i]));
-    if (masks[i]) HIP_TRY(hipFree(masks[i]));
-  }
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> lock(g_mutex);
@@ -1645,11 +1823,30 @@ void interleave_mask(Map *pixmap, Map *mask)
 // ---- engine entries ----
 void resynth_set_default_mode(int mode) { g_defaultMode = mode; }
 
+void resynth_set_device_memory_limit(size_t bytes)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_devLimit = bytes;
+  if (bytes && g_devBytes > bytes) {
+    // release the idle engine buffers, so that later calls grow under the new cap
+    if (g_dev.ok) (void)hipStreamSynchronize(g_dev.stream);
+    g_dev.ws.~Workspace();
+    new (&g_dev.ws) Workspace();
+    g_dev.pool.clear();
+    g_dev.manyDesc.release();
+    g_dev.manyCtl.release();
+    g_dev.manyGlobal.release();
+  }
+}
+
 void resynth_default_options(ResynthOptions *opt)
 {
   std::memset(opt, 0, sizeof *opt);
   opt->mode = default_mode();
-  opt->prng = RESYNTH_PRNG_GRAND;
+  // the GIMP build's GRand, or with RESYNTH_PRNG=libc the stand-alone build's rand()
+  // (lib/glibProxy.c:30-49)
+  const char *p = std::getenv("RESYNTH_PRNG");
+  opt->prng = (p && std::strcmp(p, "libc") == 0) ? RESYNTH_PRNG_LIBC : RESYNTH_PRNG_GRAND;
 }
 
 int imageSynth(ImageBuffer *imageBuffer, ImageBuffer *mask, TImageFormat imageFormat,
@@ -1698,35 +1895,13 @@ int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsig
                                void *contextInfo, int *cancelFlag, const ResynthOptions *options,
                                ResynthStats *stats)
 {
-  TFormatIndices ix;
-  if (prepareImageFormatIndicesFromFormatType(&ix, imageFormat)) return IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT;
-  std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
-  const double t0 = now_ms();
-  Job job;
-  init_job(job, parameters, options, progressCallback, contextInfo, cancelFlag);
-  job.ix = ix;
-  Source &S = job.src;
-  S.simple = true;
-  S.tw = S.cw = (int)width;
-  S.th = S.ch = (int)height;
-  S.bpp = countPixelelsPerPixelForFormat(imageFormat);
-  S.image = d_image;
-  S.rowBytes = rowBytes;
-  S.mask = d_mask;
-  S.maskRowBytes = maskRowBytes;
-  S.mask2 = d_mask2;
-  S.mask2RowBytes = mask2RowBytes;
-  S.alphaIdx = (imageFormat == T_RGBA) ? 3 : (imageFormat == T_GrayA) ? 1 : -1;
-  unsigned nT = 0;
-  int err = ((size_t)width * height) ? run_engine(job, nT) : IMAGE_SYNTH_ERROR_EMPTY_TARGET;
-  if (!err && !(cancelFlag && *cancelFlag))
-    HIP_TRY(launch_writeback_simple(g_dev.stream, job.wide, g_dev.ws.tword.p, (const int2 *)g_dev.ws.targets.p, nT,
-                                    d_image, rowBytes, S.bpp, job.nColors, (int)width));
-  HIP_TRY(hipStreamSynchronize(g_dev.stream));
-  job.st.totalMs = now_ms() - t0;
-  if (stats) *stats = job.st;
-  return err;
+  return guarded([&]() -> int {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    return image_synth_device_impl(g_dev.ws, d_image, width, height, rowBytes, d_mask, maskRowBytes, d_mask2,
+                                   mask2RowBytes, imageFormat, parameters, progressCallback, contextInfo, cancelFlag,
+                                   options, stats);
+  });
 }
 
 int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int deviceBuffers,
@@ -1741,7 +1916,9 @@ int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int devi
     if (!many_config_valid(cfg)) return -1;
     force = &cfg;
   }
-  return run_batch(requests, n, deviceBuffers != 0, options, force, stats);
+  return guarded([&]() -> int {
+    return run_batch(requests, n, deviceBuffers != 0, options, force, stats);
+  });
 }
 
 int resynth_model_log_pairs(ResynthModel *model, ImageBuffer *image, ImageBuffer *mask, TImageFormat format,
@@ -1794,7 +1971,9 @@ int resynth_autotune_batch(const ResynthRequest *sample, unsigned int n, int dev
                            ResynthKernelConfig *best, double *msPerConfig)
 {
   if (!sample || !n) return -1;
-  return autotune_batch(sample, n, deviceBuffers != 0, best, msPerConfig);
+  return guarded([&]() -> int {
+    return autotune_batch(sample, n, deviceBuffers != 0, best, msPerConfig);
+  });
 }
 
 int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned int ch,
@@ -1808,70 +1987,60 @@ int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned 
     return -1;
   for (unsigned i = 0; i < nProblems; i++)
     if (nbCount[i] > 64 || candCount[i] > maxCands) return -1;
-  std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
-  hipStream_t s = g_dev.stream;
-  const int nColors = indices->img_match_bpp, nMaps = indices->map_match_bpp;
-  const bool wide = nColors + nMaps > 3;
-  const size_t wb = wide ? 8 : 4;
-  const int depth = indices->total_bpp;
-  const size_t ncp = (size_t)cw * ch, nnb = (size_t)nProblems * 64;
-  // corpus pixmap and neighbour pixels -> words on the device
-  unsigned char *dPix = (unsigned char *)g_dev.ws.cpix.get(ncp * depth + nnb * 8 + 64);
-  unsigned char *dNbPix = dPix + ncp * depth;
-  HIP_TRY(hipMemcpyAsync(dPix, corpus, ncp * depth, hipMemcpyHostToDevice, s));
-  // neighbour pixels arrive as 8-byte records; repack them to `depth` bytes each
-  std::vector<unsigned char> nb(nnb * depth);
-  for (size_t i = 0; i < nnb; i++) std::memcpy(&nb[i * depth], nbPixel + i * 8, depth);
-  HIP_TRY(hipMemcpyAsync(dNbPix, nb.data(), nb.size(), hipMemcpyHostToDevice, s));
-  void *dC = g_dev.ws.cword.get(ncp * wb + 64);
-  void *dNbW = g_dev.ws.tword.get(nnb * wb + 64);
-  HIP_TRY(launch_pack_words(s, wide, dPix, (int)ncp, depth, indices->colorEndBip, indices->map_start_bip,
-                            indices->map_end_bip, 1, dC));
-  HIP_TRY(launch_pack_words(s, wide, dNbPix, (int)nnb, depth, indices->colorEndBip, indices->map_start_bip,
-                            indices->map_end_bip, 0, dNbW));
-  Tables ht;
-  std::memcpy(ht.T, T, sizeof ht.T);
-  std::memcpy(ht.Tm, Tm, sizeof ht.Tm);
-  Tables *dT = (Tables *)g_dev.ws.tables.get(sizeof(Tables));
-  HIP_TRY(hipMemcpyAsync(dT, &ht, sizeof ht, hipMemcpyHostToDevice, s));
-  size_t szCnt = nProblems * sizeof(unsigned), szOff = nnb * sizeof(int2);
-  size_t szCand = (size_t)nProblems * maxCands * sizeof(int2), szOut = nProblems * 4;
-  char *dArgs = (char *)g_dev.ws.compactTemp.get(2 * szCnt + szOff + szCand + 3 * szOut + 512);
-  unsigned *dNbCount = (unsigned *)dArgs;
-  unsigned *dCandCount = (unsigned *)(dArgs + szCnt);
-  int2 *dOff = (int2 *)(dArgs + 2 * szCnt);
-  int2 *dCands = (int2 *)((char *)dOff + szOff);
-  int32_t *dWin = (int32_t *)((char *)dCands + szCand);
-  uint32_t *dBest = (uint32_t *)((char *)dWin + szOut);
-  int32_t *dStop = (int32_t *)((char *)dBest + szOut);
-  HIP_TRY(hipMemcpyAsync(dNbCount, nbCount, szCnt, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dCandCount, candCount, szCnt, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dOff, nbOffset, szOff, hipMemcpyHostToDevice, s));
-  if (szCand) HIP_TRY(hipMemcpyAsync(dCands, cands, szCand, hipMemcpyHostToDevice, s));
-  const uint32_t penalty = 65535u * (uint32_t)nColors + Tm[0] * (uint32_t)nMaps;
-  HIP_TRY(launch_score_frozen(s, wide, (int)cw, (int)ch, nColors, nMaps, penalty, dC, dT, nProblems, dNbCount, dOff,
-                              dNbW, dCandCount, maxCands, dCands, dWin, dBest, dStop));
-  HIP_TRY(hipMemcpyAsync(winner, dWin, szOut, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(bestSum, dBest, szOut, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(stopped, dStop, szOut, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return 0;
+  return guarded([&]() -> int {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    hipStream_t s = g_dev.stream;
+    const int nColors = indices->img_match_bpp, nMaps = indices->map_match_bpp;
+    const bool wide = nColors + nMaps > 3;
+    const size_t wb = wide ? 8 : 4;
+    const int depth = indices->total_bpp;
+    const size_t ncp = (size_t)cw * ch, nnb = (size_t)nProblems * 64;
+    // corpus pixmap and neighbour pixels -> words on the device
+    unsigned char *dPix = (unsigned char *)g_dev.ws.cpix.get(ncp * depth + nnb * 8 + 64);
+    unsigned char *dNbPix = dPix + ncp * depth;
+    HIP_TRY(hipMemcpyAsync(dPix, corpus, ncp * depth, hipMemcpyHostToDevice, s));
+    // neighbour pixels arrive as 8-byte records; repack them to `depth` bytes each
+    std::vector<unsigned char> nb(nnb * depth);
+    for (size_t i = 0; i < nnb; i++) std::memcpy(&nb[i * depth], nbPixel + i * 8, depth);
+    HIP_TRY(hipMemcpyAsync(dNbPix, nb.data(), nb.size(), hipMemcpyHostToDevice, s));
+    void *dC = g_dev.ws.cword.get(ncp * wb + 64);
+    void *dNbW = g_dev.ws.tword.get(nnb * wb + 64);
+    HIP_TRY(launch_pack_words(s, wide, dPix, (int)ncp, depth, indices->colorEndBip, indices->map_start_bip,
+                              indices->map_end_bip, 1, dC));
+    HIP_TRY(launch_pack_words(s, wide, dNbPix, (int)nnb, depth, indices->colorEndBip, indices->map_start_bip,
+                              indices->map_end_bip, 0, dNbW));
+    Tables ht;
+    std::memcpy(ht.T, T, sizeof ht.T);
+    std::memcpy(ht.Tm, Tm, sizeof ht.Tm);
+    Tables *dT = (Tables *)g_dev.ws.tables.get(sizeof(Tables));
+    HIP_TRY(hipMemcpyAsync(dT, &ht, sizeof ht, hipMemcpyHostToDevice, s));
+    size_t szCnt = nProblems * sizeof(unsigned), szOff = nnb * sizeof(int2);
+    size_t szCand = (size_t)nProblems * maxCands * sizeof(int2), szOut = nProblems * 4;
+    char *dArgs = (char *)g_dev.ws.compactTemp.get(2 * szCnt + szOff + szCand + 3 * szOut + 512);
+    unsigned *dNbCount = (unsigned *)dArgs;
+    unsigned *dCandCount = (unsigned *)(dArgs + szCnt);
+    int2 *dOff = (int2 *)(dArgs + 2 * szCnt);
+    int2 *dCands = (int2 *)((char *)dOff + szOff);
+    int32_t *dWin = (int32_t *)((char *)dCands + szCand);
+    uint32_t *dBest = (uint32_t *)((char *)dWin + szOut);
+    int32_t *dStop = (int32_t *)((char *)dBest + szOut);
+    HIP_TRY(hipMemcpyAsync(dNbCount, nbCount, szCnt, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dCandCount, candCount, szCnt, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dOff, nbOffset, szOff, hipMemcpyHostToDevice, s));
+    if (szCand) HIP_TRY(hipMemcpyAsync(dCands, cands, szCand, hipMemcpyHostToDevice, s));
+    const uint32_t penalty = 65535u * (uint32_t)nColors + Tm[0] * (uint32_t)nMaps;
+    HIP_TRY(launch_score_frozen(s, wide, (int)cw, (int)ch, nColors, nMaps, penalty, dC, dT, nProblems, dNbCount, dOff,
+                                dNbW, dCandCount, maxCands, dCands, dWin, dBest, dStop));
+    HIP_TRY(hipMemcpyAsync(winner, dWin, szOut, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(bestSum, dBest, szOut, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(stopped, dStop, szOut, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 }  // extern "C"
-
-namespace {
-// Device memory owned by one test-hook call.
-struct TmpBuf {
-  void *p = nullptr;
-  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes < 256 ? 256 : bytes); }
-  ~TmpBuf()
-  {
-    if (p) (void)hipFree(p);
-  }
-};
-}  // namespace
 
 extern "C" {
 
@@ -1886,56 +2055,60 @@ int resynth_learn_fine_top16(const uint16_t *queries, unsigned int nq, const int
     if (start[c] > start[c + 1]) return -1;
   for (size_t i = 0; i < (size_t)nq * P; i++)
     if (clusters[i] >= (int32_t)nClusters) return -1;
-  std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
-  hipStream_t s = g_dev.stream;
-  const size_t n = nq;
-  const size_t tb = sort_pairs_temp_bytes(n);
-  TmpBuf dq, dcl, dk, dst, dpts, dout, dk0, dv0, dk1, dv1, dtemp;
-  HIP_TRY(dq.alloc(n * kEmb * 2));
-  HIP_TRY(dcl.alloc(n * P * 4));
-  HIP_TRY(dk.alloc((size_t)std::max(1u, nKeys) * kEmb * 2));
-  HIP_TRY(dst.alloc(((size_t)nClusters + 1) * 4));
-  HIP_TRY(dpts.alloc((size_t)std::max(1u, nKeys) * 8));
-  HIP_TRY(dout.alloc(n * 16 * 8));
-  HIP_TRY(dk0.alloc(n * 4));
-  HIP_TRY(dv0.alloc(n * 4));
-  HIP_TRY(dk1.alloc(n * 4));
-  HIP_TRY(dv1.alloc(n * 4));
-  HIP_TRY(dtemp.alloc(tb));
-  HIP_TRY(hipMemcpyAsync(dq.p, queries, n * kEmb * 2, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dcl.p, clusters, n * P * 4, hipMemcpyHostToDevice, s));
-  if (nKeys) {
-    HIP_TRY(hipMemcpyAsync(dk.p, keys, (size_t)nKeys * kEmb * 2, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dpts.p, keyPts, (size_t)nKeys * 8, hipMemcpyHostToDevice, s));
-  }
-  HIP_TRY(hipMemcpyAsync(dst.p, start, ((size_t)nClusters + 1) * 4, hipMemcpyHostToDevice, s));
-  const int32_t *qcl = (const int32_t *)dcl.p;
-  uint32_t *k0 = (uint32_t *)dk0.p, *v0 = (uint32_t *)dv0.p, *k1 = (uint32_t *)dk1.p, *v1 = (uint32_t *)dv1.p;
-  HIP_TRY(learn_sort_keys(s, qcl, P, (int)n, k0, v0));
-  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, n, 32, dtemp.p, tb));
-  HIP_TRY(learn_fine_top16_sorted(s, (const uint16_t *)dq.p, (int)n, v1, qcl, P, (const uint16_t *)dk.p,
-                                  (const uint32_t *)dst.p, (const int2 *)dpts.p, (int2 *)dout.p));
-  HIP_TRY(hipMemcpyAsync(props, dout.p, n * 16 * 8, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return 0;
+  return guarded([&]() -> int {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    hipStream_t s = g_dev.stream;
+    const size_t n = nq;
+    const size_t tb = sort_pairs_temp_bytes(n);
+    TmpBuf dq, dcl, dk, dst, dpts, dout, dk0, dv0, dk1, dv1, dtemp;
+    HIP_TRY(dq.alloc(n * kEmb * 2));
+    HIP_TRY(dcl.alloc(n * P * 4));
+    HIP_TRY(dk.alloc((size_t)std::max(1u, nKeys) * kEmb * 2));
+    HIP_TRY(dst.alloc(((size_t)nClusters + 1) * 4));
+    HIP_TRY(dpts.alloc((size_t)std::max(1u, nKeys) * 8));
+    HIP_TRY(dout.alloc(n * 16 * 8));
+    HIP_TRY(dk0.alloc(n * 4));
+    HIP_TRY(dv0.alloc(n * 4));
+    HIP_TRY(dk1.alloc(n * 4));
+    HIP_TRY(dv1.alloc(n * 4));
+    HIP_TRY(dtemp.alloc(tb));
+    HIP_TRY(hipMemcpyAsync(dq.p, queries, n * kEmb * 2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dcl.p, clusters, n * P * 4, hipMemcpyHostToDevice, s));
+    if (nKeys) {
+      HIP_TRY(hipMemcpyAsync(dk.p, keys, (size_t)nKeys * kEmb * 2, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(dpts.p, keyPts, (size_t)nKeys * 8, hipMemcpyHostToDevice, s));
+    }
+    HIP_TRY(hipMemcpyAsync(dst.p, start, ((size_t)nClusters + 1) * 4, hipMemcpyHostToDevice, s));
+    const int32_t *qcl = (const int32_t *)dcl.p;
+    uint32_t *k0 = (uint32_t *)dk0.p, *v0 = (uint32_t *)dv0.p, *k1 = (uint32_t *)dk1.p, *v1 = (uint32_t *)dv1.p;
+    HIP_TRY(learn_sort_keys(s, qcl, P, (int)n, k0, v0));
+    HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, n, 32, dtemp.p, tb));
+    HIP_TRY(learn_fine_top16_sorted(s, (const uint16_t *)dq.p, (int)n, v1, qcl, P, (const uint16_t *)dk.p,
+                                    (const uint32_t *)dst.p, (const int2 *)dpts.p, (int2 *)dout.p));
+    HIP_TRY(hipMemcpyAsync(props, dout.p, n * 16 * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 int resynth_sorted_offsets(int w, int h, int32_t *out)
 {
   if (w < 1 || h < 1 || !out || (size_t)w * (size_t)h > (1u << 28)) return -1;
-  std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
-  hipStream_t s = g_dev.stream;
-  const size_t nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
-  const size_t tb = offsets_temp_bytes(w, h);
-  TmpBuf dout, dtemp;
-  HIP_TRY(dout.alloc(nO * sizeof(int2)));
-  HIP_TRY(dtemp.alloc(tb));
-  HIP_TRY(launch_sorted_offsets(s, w, h, (int2 *)dout.p, dtemp.p, tb));
-  HIP_TRY(hipMemcpyAsync(out, dout.p, nO * sizeof(int2), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return 0;
+  return guarded([&]() -> int {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    hipStream_t s = g_dev.stream;
+    const size_t nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
+    const size_t tb = offsets_temp_bytes(w, h);
+    TmpBuf dout, dtemp;
+    HIP_TRY(dout.alloc(nO * sizeof(int2)));
+    HIP_TRY(dtemp.alloc(tb));
+    HIP_TRY(launch_sorted_offsets(s, w, h, (int2 *)dout.p, dtemp.p, tb));
+    HIP_TRY(hipMemcpyAsync(out, dout.p, nO * sizeof(int2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 int resynth_learn_route(const uint16_t *queries, unsigned int nq, const int32_t *supers,
@@ -1945,39 +2118,41 @@ int resynth_learn_route(const uint16_t *queries, unsigned int nq, const int32_t 
     return -1;
   for (unsigned i = 0; i < nq; i++)
     if (supers[i] >= (int32_t)S) return -1;
-  std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
-  hipStream_t s = g_dev.stream;
-  const size_t n = nq;
-  const size_t tb = sort_pairs_temp_bytes(n);
-  TmpBuf dq, dsup, dsub, dout, dk0, dv0, dk1, dv1, dtemp;
-  HIP_TRY(dq.alloc(n * kEmb * 2));
-  HIP_TRY(dsup.alloc(n * 4));
-  HIP_TRY(dsub.alloc((size_t)S * Q * kEmb * 2));
-  HIP_TRY(dout.alloc(n * P * 4));
-  HIP_TRY(dk0.alloc(n * 4));
-  HIP_TRY(dv0.alloc(n * 4));
-  HIP_TRY(dk1.alloc(n * 4));
-  HIP_TRY(dv1.alloc(n * 4));
-  HIP_TRY(dtemp.alloc(tb));
-  HIP_TRY(hipMemcpyAsync(dq.p, queries, n * kEmb * 2, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dsup.p, supers, n * 4, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(dsub.p, subCentroids, (size_t)S * Q * kEmb * 2, hipMemcpyHostToDevice, s));
-  const int32_t *qs = (const int32_t *)dsup.p;
-  uint32_t *k0 = (uint32_t *)dk0.p, *v0 = (uint32_t *)dv0.p, *k1 = (uint32_t *)dk1.p, *v1 = (uint32_t *)dv1.p;
-  HIP_TRY(learn_sort_keys(s, qs, 1, (int)n, k0, v0));
-  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, n, bits_for(S), dtemp.p, tb));
-  HIP_TRY(learn_query_sub(s, (const uint16_t *)dq.p, (int)n, v1, qs, (int)Q, (const uint16_t *)dsub.p, P,
-                          (int32_t *)dout.p));
-  HIP_TRY(hipMemcpyAsync(clusters, dout.p, n * P * 4, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  return 0;
+  return guarded([&]() -> int {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    hipStream_t s = g_dev.stream;
+    const size_t n = nq;
+    const size_t tb = sort_pairs_temp_bytes(n);
+    TmpBuf dq, dsup, dsub, dout, dk0, dv0, dk1, dv1, dtemp;
+    HIP_TRY(dq.alloc(n * kEmb * 2));
+    HIP_TRY(dsup.alloc(n * 4));
+    HIP_TRY(dsub.alloc((size_t)S * Q * kEmb * 2));
+    HIP_TRY(dout.alloc(n * P * 4));
+    HIP_TRY(dk0.alloc(n * 4));
+    HIP_TRY(dv0.alloc(n * 4));
+    HIP_TRY(dk1.alloc(n * 4));
+    HIP_TRY(dv1.alloc(n * 4));
+    HIP_TRY(dtemp.alloc(tb));
+    HIP_TRY(hipMemcpyAsync(dq.p, queries, n * kEmb * 2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dsup.p, supers, n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dsub.p, subCentroids, (size_t)S * Q * kEmb * 2, hipMemcpyHostToDevice, s));
+    const int32_t *qs = (const int32_t *)dsup.p;
+    uint32_t *k0 = (uint32_t *)dk0.p, *v0 = (uint32_t *)dv0.p, *k1 = (uint32_t *)dk1.p, *v1 = (uint32_t *)dv1.p;
+    HIP_TRY(learn_sort_keys(s, qs, 1, (int)n, k0, v0));
+    HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, n, bits_for(S), dtemp.p, tb));
+    HIP_TRY(learn_query_sub(s, (const uint16_t *)dq.p, (int)n, v1, qs, (int)Q, (const uint16_t *)dsub.p, P,
+                            (int32_t *)dout.p));
+    HIP_TRY(hipMemcpyAsync(clusters, dout.p, n * P * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 int resynth_device_info(char *name, size_t nameLen, int *computeUnits)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
-  if (!ensure_device()) return -1;
+  if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
   hipDeviceProp_t p;
   if (hipGetDeviceProperties(&p, g_dev.device) != hipSuccess) return -1;
   if (name && nameLen) {
@@ -2012,25 +2187,46 @@ void resynth_metric_tables(double sensitivity, double mapWeight, uint16_t *T, ui
 
 int resynth_order_targets(int matchContextType, int prng, int32_t *points, unsigned int n)
 {
-  Prng p(prng, kEngineSeed);
-  std::vector<Pt> pts(n);
-  for (unsigned i = 0; i < n; i++) pts[i] = {points[2 * i], points[2 * i + 1]};
-  if (matchContextType == 0 || matchContextType == 1) {
-    std::vector<uint32_t> perm(n);
-    p.shuffle(perm.data(), n);
-    for (unsigned i = 0; i < n; i++) {
-      points[2 * i] = pts[perm[i]].x;
-      points[2 * i + 1] = pts[perm[i]].y;
-    }
-    return 0;
-  }
   if (matchContextType < 0 || matchContextType > 8) return IMAGE_SYNTH_ERROR_MATCH_CONTEXT_TYPE_RANGE;
-  order_directional(matchContextType, pts, p);
-  for (unsigned i = 0; i < n; i++) {
-    points[2 * i] = pts[i].x;
-    points[2 * i + 1] = pts[i].y;
-  }
-  return 0;
+  if (!points && n) return RESYNTH_ERROR_INVALID_ARGUMENT;
+  return guarded([&]() -> int {
+    Prng p(prng, kEngineSeed);
+    std::vector<uint32_t> perm(n);
+    std::vector<int32_t> rm(points, points + 2 * (size_t)n);
+    if (matchContextType <= 1) {
+      // the host shuffle the engine runs for types 0 and 1
+      p.shuffle(perm.data(), n);
+      for (unsigned i = 0; i < n; i++) {
+        points[2 * i] = rm[2 * (size_t)perm[i]];
+        points[2 * i + 1] = rm[2 * (size_t)perm[i] + 1];
+      }
+      return 0;
+    }
+    // types 2..8: the engine's device sort, the host's banded shuffle, the device composition
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    hipStream_t s = g_dev.stream;
+    uint32_t hRays[9];
+    axis_rays(hRays);
+    const size_t otb = order_temp_bytes(n);
+    TmpBuf drm, dtemp, dperm, dout, drays;
+    HIP_TRY(drm.alloc((size_t)n * 8));
+    HIP_TRY(dtemp.alloc(otb));
+    HIP_TRY(dperm.alloc((size_t)n * 4));
+    HIP_TRY(dout.alloc((size_t)n * 8));
+    HIP_TRY(drays.alloc(sizeof hRays));
+    HIP_TRY(hipMemcpyAsync(drm.p, rm.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(drays.p, hRays, sizeof hRays, hipMemcpyHostToDevice, s));
+    const uint32_t *sorted = nullptr;
+    HIP_TRY(launch_order_sort(s, (const int2 *)drm.p, n, matchContextType, (const uint32_t *)drays.p, dtemp.p, otb,
+                              &sorted));
+    band_permutation(perm.data(), n, p);
+    HIP_TRY(hipMemcpyAsync(dperm.p, perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_order_compose(s, (const int2 *)drm.p, sorted, (const uint32_t *)dperm.p, (int2 *)dout.p, n));
+    HIP_TRY(hipMemcpyAsync(points, dout.p, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 }  // extern "C"

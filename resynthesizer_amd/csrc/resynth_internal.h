@@ -7,6 +7,14 @@
 
 namespace rs {
 
+// A HIP call that failed inside an entry point.  It is thrown by hip_fail and caught at
+// the C boundary, which returns RESYNTH_ERROR_DEVICE instead of ending the caller's
+// process.
+struct HipFailure {
+  hipError_t err;
+};
+[[noreturn]] void hip_fail(hipError_t e, const char *expr, const char *file, int line);
+
 // Device pixel word.  Byte 0 is a flag byte; bytes 1.. hold the matched channels,
 // colours first, then map channels, each one byte (the engine's Pixelel).
 //   target word: flag bit 0 = hasValue (lib/engine.c:152-168)
@@ -95,6 +103,7 @@ struct ReqDesc {
   unsigned minW;          // pass-0 window floor (>= 1)
   unsigned ramp;          // pass-0 window growth, permille of the targets visited in the pass
   unsigned capW;          // window cap
+  unsigned maxPasses;     // stop after this many passes (0: the refiner's rule alone)
 };
 
 // Scheduling state of one request, each hot word on its own 128-byte line.
@@ -140,6 +149,12 @@ hipError_t launch_ctx_rings(hipStream_t s, const uint8_t *mark, int W, int H, in
 hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas);
 hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx);
 hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);
+// target ordering of match-context types 2..8 (resynth_order.hip)
+size_t order_temp_bytes(unsigned n);
+hipError_t launch_order_sort(hipStream_t s, const int2 *rm, unsigned n, int type, const uint32_t *axisRays,
+                             void *temp, size_t tempBytes, const uint32_t **order);
+hipError_t launch_order_compose(hipStream_t s, const int2 *rm, const uint32_t *order, const uint32_t *perm, int2 *out,
+                                unsigned n);
 hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                 unsigned k1, unsigned seed, unsigned pass);
 unsigned synth_wave_min();   // chunks of at least this many targets use k_synth_wave

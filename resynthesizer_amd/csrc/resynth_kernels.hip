@@ -1324,12 +1324,14 @@ void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq
       if ((unsigned)old + nGot == d.sizes[pass]) {
         // the pass is complete: lib/refiner.h:111-115 decides whether another follows
         const unsigned betters = (unsigned)(old >> 32) + bet;
-        const bool more = pass < 5 && d.sizes[pass + 1] > 0 && !((double)((float)betters / (float)d.g.nT) < 0.1);
+        const bool capped = d.maxPasses && pass + 1 >= d.maxPasses;
+        const bool more = pass < 5 && d.sizes[pass + 1] > 0 && !capped &&
+                          !((double)((float)betters / (float)d.g.nT) < 0.1);
         if (more) {
           atomicExch(&c.ctr, (unsigned long long)(pass + 1) << 32);
         } else {
           // an empty next pass counts as run, as in the host loop
-          c.passesRun = pass + 1 + ((pass < 5 && d.sizes[pass + 1] == 0) ? 1 : 0);
+          c.passesRun = pass + 1 + ((pass < 5 && d.sizes[pass + 1] == 0 && !capped) ? 1 : 0);
           atomicExch(&c.ctr, 6ull << 32);
           atomicSub(active, 1u);
         }

@@ -1,6 +1,7 @@
 // Host side of the learned proposals: the model object (FP32 master weights, BF16 copies,
 // Adam state) and its C ABI (include/resynth_mi355x.h), training loops and test entries.
 #include "../../include/resynth_mi355x.h"
+#include "resynth_internal.h"
 #include "resynth_learn.h"
 
 #include <hip/hip_runtime.h>
@@ -16,14 +17,12 @@
 
 using namespace rs;
 
+// A failed HIP call unwinds to the C entry point, which returns RESYNTH_ERROR_DEVICE (or
+// NULL) instead of ending the caller's process.
 #define LHIP(expr)                                                                                    \
   do {                                                                                                \
     hipError_t e_ = (expr);                                                                           \
-    if (e_ != hipSuccess) {                                                                           \
-      std::fprintf(stderr, "resynth_mi355x: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_),  \
-                   __FILE__, __LINE__);                                                               \
-      std::abort();                                                                                   \
-    }                                                                                                 \
+    if (e_ != hipSuccess) hip_fail(e_, #expr, __FILE__, __LINE__);                                    \
   } while (0)
 
 struct ResynthModel {
@@ -54,6 +53,39 @@ struct ResynthModel {
 };
 
 namespace {
+
+// Device memory owned by one call.
+struct DevTmp {
+  void *p = nullptr;
+  void *alloc(size_t bytes)
+  {
+    LHIP(hipMalloc(&p, bytes < 256 ? 256 : bytes));
+    return p;
+  }
+  DevTmp() = default;
+  DevTmp(const DevTmp &) = delete;
+  DevTmp &operator=(const DevTmp &) = delete;
+  ~DevTmp()
+  {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Runs a model entry; a HIP failure becomes RESYNTH_ERROR_DEVICE after the model's stream
+// has drained.
+template <typename F>
+int model_guarded(ResynthModel *M, F &&body)
+{
+  try {
+    return body();
+  } catch (const HipFailure &) {
+    if (M && M->s) (void)hipStreamSynchronize(M->s);
+    (void)hipGetLastError();
+    return RESYNTH_ERROR_DEVICE;
+  } catch (const std::bad_alloc &) {
+    return RESYNTH_ERROR_DEVICE;
+  }
+}
 
 void alloc_scratch(ResynthModel *M, LearnScratch &t, int n, bool query)
 {
@@ -144,9 +176,12 @@ void model_reserve_pairs(ResynthModel *M, unsigned n, uint16_t **pq, uint16_t **
 {
   if (M->nPairs + n > M->capPairs) {
     const unsigned cap = std::max(M->nPairs + n, M->capPairs * 2);
-    uint16_t *q, *k;
+    uint16_t *q = nullptr, *k = nullptr;
     LHIP(hipMalloc(&q, (size_t)cap * kIn * 2));
-    LHIP(hipMalloc(&k, (size_t)cap * kIn * 2));
+    if (hipMalloc(&k, (size_t)cap * kIn * 2) != hipSuccess) {
+      (void)hipFree(q);
+      hip_fail(hipErrorOutOfMemory, "hipMalloc (pair store)", __FILE__, __LINE__);
+    }
     if (M->nPairs) {
       LHIP(hipMemcpyAsync(q, M->pq, (size_t)M->nPairs * kIn * 2, hipMemcpyDeviceToDevice, M->s));
       LHIP(hipMemcpyAsync(k, M->pk, (size_t)M->nPairs * kIn * 2, hipMemcpyDeviceToDevice, M->s));
@@ -191,55 +226,65 @@ ResynthModel *resynth_model_create(unsigned int seed)
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return nullptr;
   ResynthModel *M = new ResynthModel();
-  LHIP(hipStreamCreateWithFlags(&M->s, hipStreamNonBlocking));
-  M->w = (float *)M->alloc(kParamCount * 4);
-  M->m = (float *)M->alloc(kParamCount * 4);
-  M->v = (float *)M->alloc(kParamCount * 4);
-  M->grad = (float *)M->alloc(kParamCount * 4);
-  M->loss = (float *)M->alloc(1024 * 4);
-  M->wb = (uint16_t *)M->alloc(kParamCount * 2);
-  M->wt = (uint16_t *)M->alloc(kParamCount * 2);
-  std::vector<float> p;
-  model_init_params(seed, p);
-  LHIP(hipMemcpyAsync(M->w, p.data(), kParamCount * 4, hipMemcpyHostToDevice, M->s));
-  LHIP(hipMemsetAsync(M->m, 0, kParamCount * 4, M->s));
-  LHIP(hipMemsetAsync(M->v, 0, kParamCount * 4, M->s));
-  LHIP(params_to_bf16(M->s, M->w, M->wb));
-  LHIP(hipStreamSynchronize(M->s));
+  try {
+    LHIP(hipStreamCreateWithFlags(&M->s, hipStreamNonBlocking));
+    M->w = (float *)M->alloc(kParamCount * 4);
+    M->m = (float *)M->alloc(kParamCount * 4);
+    M->v = (float *)M->alloc(kParamCount * 4);
+    M->grad = (float *)M->alloc(kParamCount * 4);
+    M->loss = (float *)M->alloc(1024 * 4);
+    M->wb = (uint16_t *)M->alloc(kParamCount * 2);
+    M->wt = (uint16_t *)M->alloc(kParamCount * 2);
+    std::vector<float> p;
+    model_init_params(seed, p);
+    LHIP(hipMemcpyAsync(M->w, p.data(), kParamCount * 4, hipMemcpyHostToDevice, M->s));
+    LHIP(hipMemsetAsync(M->m, 0, kParamCount * 4, M->s));
+    LHIP(hipMemsetAsync(M->v, 0, kParamCount * 4, M->s));
+    LHIP(params_to_bf16(M->s, M->w, M->wb));
+    LHIP(hipStreamSynchronize(M->s));
+  } catch (const HipFailure &) {
+    resynth_model_free(M);
+    return nullptr;
+  }
   return M;
 }
 
 void resynth_model_free(ResynthModel *M)
 {
   if (!M) return;
-  LHIP(hipStreamSynchronize(M->s));
-  if (M->pq) LHIP(hipFree(M->pq));
-  if (M->pk) LHIP(hipFree(M->pk));
-  for (void *p : M->owned) LHIP(hipFree(p));
-  LHIP(hipStreamDestroy(M->s));
+  if (M->s) (void)hipStreamSynchronize(M->s);
+  if (M->pq) (void)hipFree(M->pq);
+  if (M->pk) (void)hipFree(M->pk);
+  for (void *p : M->owned) (void)hipFree(p);
+  if (M->s) (void)hipStreamDestroy(M->s);
   delete M;
 }
 
-int resynth_model_get_params(const ResynthModel *M, float *out)
+int resynth_model_get_params(const ResynthModel *Mc, float *out)
 {
+  ResynthModel *M = const_cast<ResynthModel *>(Mc);
   if (!M || !out) return -1;
-  LHIP(hipMemcpyAsync(out, M->w, kParamCount * 4, hipMemcpyDeviceToHost, M->s));
-  LHIP(hipStreamSynchronize(M->s));
-  return 0;
+  return model_guarded(M, [&]() -> int {
+    LHIP(hipMemcpyAsync(out, M->w, kParamCount * 4, hipMemcpyDeviceToHost, M->s));
+    LHIP(hipStreamSynchronize(M->s));
+    return 0;
+  });
 }
 
 int resynth_model_set_params(ResynthModel *M, const float *in)
 {
   if (!M || !in) return -1;
   std::lock_guard<std::mutex> lock(M->mu);
-  LHIP(hipMemcpyAsync(M->w, in, kParamCount * 4, hipMemcpyHostToDevice, M->s));
-  LHIP(params_to_bf16(M->s, M->w, M->wb));
-  LHIP(hipMemsetAsync(M->m, 0, kParamCount * 4, M->s));
-  LHIP(hipMemsetAsync(M->v, 0, kParamCount * 4, M->s));
-  M->step = 0;
-  M->version++;
-  LHIP(hipStreamSynchronize(M->s));
-  return 0;
+  return model_guarded(M, [&]() -> int {
+    LHIP(hipMemcpyAsync(M->w, in, kParamCount * 4, hipMemcpyHostToDevice, M->s));
+    LHIP(params_to_bf16(M->s, M->w, M->wb));
+    LHIP(hipMemsetAsync(M->m, 0, kParamCount * 4, M->s));
+    LHIP(hipMemsetAsync(M->v, 0, kParamCount * 4, M->s));
+    M->step = 0;
+    M->version++;
+    LHIP(hipStreamSynchronize(M->s));
+    return 0;
+  });
 }
 
 int resynth_model_embed(const ResynthModel *Mc, const uint16_t *x, unsigned int n, float *out)
@@ -248,25 +293,18 @@ int resynth_model_embed(const ResynthModel *Mc, const uint16_t *x, unsigned int 
   if (!M || !x || !out) return -1;
   if (!n) return 0;
   std::lock_guard<std::mutex> lock(M->mu);
-  hipStream_t s = M->s;
-  uint16_t *dx;
-  float *dz;
-  std::vector<void *> tmp;
-  auto A = [&](size_t b) {
-    void *p;
-    LHIP(hipMalloc(&p, b < 256 ? 256 : b));
-    tmp.push_back(p);
-    return p;
-  };
-  dx = (uint16_t *)A((size_t)n * kIn * 2);
-  dz = (float *)A((size_t)n * kEmb * 4);
-  LHIP(hipMemcpyAsync(dx, x, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
-  // the fused kernel the engine embeds with
-  LHIP(mlp_embed_fused(s, M->w, model_weights_t(M, s), dx, (int)n, nullptr, dz));
-  LHIP(hipMemcpyAsync(out, dz, (size_t)n * kEmb * 4, hipMemcpyDeviceToHost, s));
-  LHIP(hipStreamSynchronize(s));
-  for (void *p : tmp) LHIP(hipFree(p));
-  return 0;
+  return model_guarded(M, [&]() -> int {
+    hipStream_t s = M->s;
+    DevTmp tx, tz;
+    uint16_t *dx = (uint16_t *)tx.alloc((size_t)n * kIn * 2);
+    float *dz = (float *)tz.alloc((size_t)n * kEmb * 4);
+    LHIP(hipMemcpyAsync(dx, x, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
+    // the fused kernel the engine embeds with
+    LHIP(mlp_embed_fused(s, M->w, model_weights_t(M, s), dx, (int)n, nullptr, dz));
+    LHIP(hipMemcpyAsync(out, dz, (size_t)n * kEmb * 4, hipMemcpyDeviceToHost, s));
+    LHIP(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 int resynth_model_infonce(ResynthModel *M, const uint16_t *xq, const uint16_t *xk, unsigned int n, float tau,
@@ -274,15 +312,17 @@ int resynth_model_infonce(ResynthModel *M, const uint16_t *xq, const uint16_t *x
 {
   if (!M || !xq || !xk || !n || n % 64) return -1;
   std::lock_guard<std::mutex> lock(M->mu);
-  ensure_scratch(M, (int)n);
-  hipStream_t s = M->s;
-  LHIP(hipMemcpyAsync(M->bq, xq, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
-  LHIP(hipMemcpyAsync(M->bk, xk, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
-  LHIP(mlp_infonce(s, M->w, M->wb, M->bq, M->bk, (int)n, tau, M->q, M->k, M->grad, M->loss));
-  if (loss) LHIP(hipMemcpyAsync(loss, M->loss, 4, hipMemcpyDeviceToHost, s));
-  if (grad) LHIP(hipMemcpyAsync(grad, M->grad, kParamCount * 4, hipMemcpyDeviceToHost, s));
-  LHIP(hipStreamSynchronize(s));
-  return 0;
+  return model_guarded(M, [&]() -> int {
+    ensure_scratch(M, (int)n);
+    hipStream_t s = M->s;
+    LHIP(hipMemcpyAsync(M->bq, xq, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
+    LHIP(hipMemcpyAsync(M->bk, xk, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
+    LHIP(mlp_infonce(s, M->w, M->wb, M->bq, M->bk, (int)n, tau, M->q, M->k, M->grad, M->loss));
+    if (loss) LHIP(hipMemcpyAsync(loss, M->loss, 4, hipMemcpyDeviceToHost, s));
+    if (grad) LHIP(hipMemcpyAsync(grad, M->grad, kParamCount * 4, hipMemcpyDeviceToHost, s));
+    LHIP(hipStreamSynchronize(s));
+    return 0;
+  });
 }
 
 int resynth_model_train(ResynthModel *M, const uint16_t *xq, const uint16_t *xk, unsigned int n, unsigned int steps,
@@ -290,17 +330,17 @@ int resynth_model_train(ResynthModel *M, const uint16_t *xq, const uint16_t *xk,
 {
   if (!M || !xq || !xk || !n || !batch || batch % 64) return -1;
   std::lock_guard<std::mutex> lock(M->mu);
-  hipStream_t s = M->s;
-  uint16_t *dq, *dk;
-  LHIP(hipMalloc(&dq, (size_t)n * kIn * 2));
-  LHIP(hipMalloc(&dk, (size_t)n * kIn * 2));
-  LHIP(hipMemcpyAsync(dq, xq, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
-  LHIP(hipMemcpyAsync(dk, xk, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
-  const int rc = model_train_device(M, dq, dk, n, steps, batch, lr, tau, seed, losses);
-  LHIP(hipStreamSynchronize(s));
-  LHIP(hipFree(dq));
-  LHIP(hipFree(dk));
-  return rc;
+  return model_guarded(M, [&]() -> int {
+    hipStream_t s = M->s;
+    DevTmp tq, tk;
+    uint16_t *dq = (uint16_t *)tq.alloc((size_t)n * kIn * 2);
+    uint16_t *dk = (uint16_t *)tk.alloc((size_t)n * kIn * 2);
+    LHIP(hipMemcpyAsync(dq, xq, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
+    LHIP(hipMemcpyAsync(dk, xk, (size_t)n * kIn * 2, hipMemcpyHostToDevice, s));
+    const int rc = model_train_device(M, dq, dk, n, steps, batch, lr, tau, seed, losses);
+    LHIP(hipStreamSynchronize(s));
+    return rc;
+  });
 }
 
 unsigned int resynth_model_pair_count(const ResynthModel *M) { return M ? M->nPairs : 0u; }
@@ -315,28 +355,28 @@ int resynth_model_fit(ResynthModel *M, unsigned int steps, unsigned int batch, f
 {
   if (!M || !M->nPairs) return -1;
   std::lock_guard<std::mutex> lock(M->mu);
-  return model_train_device(M, M->pq, M->pk, M->nPairs, steps, batch, lr, tau, seed, losses);
+  return model_guarded(M, [&]() -> int {
+    return model_train_device(M, M->pq, M->pk, M->nPairs, steps, batch, lr, tau, seed, losses);
+  });
 }
 
 int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, const uint16_t *B, float *C)
 {
   if (M <= 0 || N <= 0 || K <= 0 || K % 32 || M % 8 || N % 8 || !A || !B || !C) return -1;
   int nd = 0;
-  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return -1;
-  uint16_t *dA, *dB;
-  float *dC;
-  LHIP(hipMalloc(&dA, (size_t)M * K * 2));
-  LHIP(hipMalloc(&dB, (size_t)K * N * 2));
-  LHIP(hipMalloc(&dC, (size_t)M * N * 4));
-  LHIP(hipMemcpy(dA, A, (size_t)M * K * 2, hipMemcpyHostToDevice));
-  LHIP(hipMemcpy(dB, B, (size_t)K * N * 2, hipMemcpyHostToDevice));
-  const int lda = ta ? M : K, ldb = tb ? K : N;
-  LHIP(learn_gemm(nullptr, ta, tb, M, N, K, dA, lda, dB, ldb, dC, N));
-  LHIP(hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost));
-  LHIP(hipFree(dA));
-  LHIP(hipFree(dB));
-  LHIP(hipFree(dC));
-  return 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) return RESYNTH_ERROR_DEVICE;
+  return model_guarded(nullptr, [&]() -> int {
+    DevTmp ta_, tb_, tc_;
+    uint16_t *dA = (uint16_t *)ta_.alloc((size_t)M * K * 2);
+    uint16_t *dB = (uint16_t *)tb_.alloc((size_t)K * N * 2);
+    float *dC = (float *)tc_.alloc((size_t)M * N * 4);
+    LHIP(hipMemcpy(dA, A, (size_t)M * K * 2, hipMemcpyHostToDevice));
+    LHIP(hipMemcpy(dB, B, (size_t)K * N * 2, hipMemcpyHostToDevice));
+    const int lda = ta ? M : K, ldb = tb ? K : N;
+    LHIP(learn_gemm(nullptr, ta, tb, M, N, K, dA, lda, dB, ldb, dC, N));
+    LHIP(hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost));
+    return 0;
+  });
 }
 
 }  // extern "C"

@@ -134,7 +134,8 @@ def test_serial_testsynth_known_answers():
     assert r.err == R.IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT
 
 
-@pytest.mark.parametrize("name", ["resynthfull-zap-texture", "resynthfull-ufo-input", "resynth-ufo-input",
+@pytest.mark.parametrize("name", ["resynthfull-zap-texture", "resynthfull-ufo-input", "resynthfull-brick",
+                                  "resynth-ufo-input",
                                   "resynthtwoimages-ufo-input", "resynthtileable-ufo-input",
                                   "resynthfull-donkey_original", "mapstyleangel-angel_target"])
 def test_serial_reproduces_reference_regression_image(name):
@@ -241,6 +242,21 @@ def test_serial_stream_refill_path(monkeypatch):
     rc, oc = O.image_synth(img, mask, 0, _oparams(p))
     assert r1.launches > r0.launches
     assert np.array_equal(o1, oc) and np.array_equal(o0, oc)
+
+
+def test_serial_stream_grow_path(monkeypatch):
+    """A visit that needs more raw PRNG outputs than the whole device buffer holds (1 500
+    probes against a 1 024-word buffer) grows the buffer and resumes at the same visit;
+    the output must still equal the oracle's byte for byte."""
+    img, mask = _case_inputs(R.T_RGB, (36, 44), 9)
+    p = R.default_params(patchSize=12, maxProbeCount=1500)
+    monkeypatch.setenv("RESYNTH_SERIAL_STREAM_CAP", "1024")
+    r1, o1 = R.image_synth(img, mask, R.T_RGB, p, mode=R.MODE_SERIAL)
+    monkeypatch.delenv("RESYNTH_SERIAL_STREAM_CAP")
+    rc, oc = O.image_synth(img, mask, 0, _oparams(p))
+    assert r1.err == rc.err == 0
+    assert r1.passes == rc.passes and r1.betters == rc.betters and r1.visits == rc.visits
+    assert np.array_equal(o1, oc)
 
 
 def test_serial_imagesynth2_and_cancel():
@@ -605,3 +621,159 @@ def test_batched_window_mode_config2_768(monkeypatch):
     r0, _ = R.image_synth(img, mask, R.T_RGBA, R.default_params(), mode=R.MODE_BATCHED, seed=1)
     assert r0.stats["windowLaunches"] == 0
     assert r.meanError <= 1.08 * r0.meanError, (r.meanError, r0.meanError)
+
+
+# ---------------------------------------------------------------- ordering on the device
+
+def _order_vectors():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vectors.json")) as f:
+        return json.load(f)["order"]
+
+
+@pytest.mark.parametrize("mtype", range(2, 9))
+def test_device_target_ordering_matches_glib(mtype):
+    """The engine's device sort of types 2..8 plus the host's banded shuffle equal the whole
+    ordering computed by GLib's g_array_sort / g_rand_int_range and the C library's atan2
+    (tests/golden/make_golden.py), on shapes with ties, an off-target centre and a centre
+    pixel whose ratio is 0/0 (the NaN case of lib/brushfire.h:117-128)."""
+    import hashlib
+    for name, e in _order_vectors().items():
+        print("ordering type", mtype, "shape", name, flush=True)
+        err, got = R.order_targets(mtype, np.array(e["points"], np.int32))
+        assert err == 0
+        got = np.asarray(got, "<i4").reshape(-1, 2)
+        if str(mtype) in e["types"]:
+            assert got.tolist() == e["types"][str(mtype)], name
+        assert hashlib.sha256(got.tobytes()).hexdigest() == e["types"][str(mtype) + "_sha256"], name
+
+
+def test_serial_engine_uses_device_ordering_on_a_nan_shape():
+    """A whole serial engine run with the inward order (type 2) on a hole whose centre pixel
+    has a NaN ratio equals the oracle byte for byte."""
+    img = W.grating_noise(60, 80, 3, 31)
+    yy, xx = np.mgrid[0:60, 0:80]
+    mask = np.where((xx - 30) ** 2 + (yy - 30) ** 2 <= 144, 255, 0).astype(np.uint8)
+    mask[30, 30:] = 0
+    for mtype in (2, 5, 8):
+        p = R.default_params(matchContextType=mtype, patchSize=20, maxProbeCount=60)
+        rg, og = R.image_synth(img, mask, R.T_RGB, p, mode=R.MODE_SERIAL)
+        rc, oc = O.image_synth(img, mask, 0, _oparams(p))
+        assert rg.err == rc.err == 0 and np.array_equal(og, oc), mtype
+
+
+# ---------------------------------------------------------------- the reference's C entry points
+
+def _reference_api(tmp_path):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "reference_api")
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "c", "reference_api.c"), "-L", os.path.dirname(R.LIB_PATH),
+                    "-lresynth_mi355x", "-Wl,-rpath," + os.path.dirname(R.LIB_PATH), "-o", exe], check=True)
+    return exe
+
+
+def _run_c(exe, args, mode=None, prng=None):
+    import os
+    import subprocess
+    env = dict(os.environ)
+    env.pop("RESYNTH_MODE", None)
+    env.pop("RESYNTH_PRNG", None)
+    if mode:
+        env["RESYNTH_MODE"] = mode
+    if prng:
+        env["RESYNTH_PRNG"] = prng
+    print("reference_api", args[0], mode, prng, flush=True)
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, env=env, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    return r
+
+
+@pytest.mark.parametrize("mode", ["serial", None])
+def test_reference_entry_points_from_c_known_answers(tmp_path, mode):
+    """src/testSynth.c:163-212's known answers through the reference-signature imageSynth,
+    called from C, in the serial mode and in the default (batched) mode.  testSynth.c is
+    built with the stand-alone PRNG (glibc rand(), lib/glibProxy.c:30-49), and its Gray case
+    is a tie that the PRNG stream decides, so the library runs with RESYNTH_PRNG=libc."""
+    prng = "libc"
+    exe = _reference_api(tmp_path)
+    r = _run_c(exe, ["known"], mode, prng)
+    assert r.returncode == 0 and "0 failure(s)" in r.stdout
+
+
+def test_reference_entry_points_from_c_engine_and_imagesynth2(tmp_path):
+    """engine() from C with reference Maps reproduces Test/reference_out_images/
+    resynthfull-brick.ppm in the serial mode; imageSynth2 from C equals the oracle; the
+    default mode leaves everything outside the target as it was."""
+    c = pins.cases()["resynthfull-brick"]
+    params, (colors, maps), tp, cp, rect, expected = pins.build(c)
+    th, tw, depth = tp.shape
+    tp.tofile(tmp_path / "t.raw")
+    cp.tofile(tmp_path / "c.raw")
+    exe = _reference_api(tmp_path)
+    args = ["engine", tw, th, tmp_path / "t.raw", cp.shape[1], cp.shape[0], tmp_path / "c.raw", colors, maps,
+            params["tileH"], params["tileV"], params["matchContextType"], repr(params["mapWeight"]),
+            repr(params["sensitivity"]), params["patchSize"], params["maxProbeCount"], tmp_path / "o.raw"]
+    assert _run_c(exe, args, "serial").returncode == 0
+    out = np.fromfile(tmp_path / "o.raw", np.uint8).reshape(th, tw, depth)
+    x0, y0, x1, y1 = rect
+    assert np.array_equal(out[y0:y1, x0:x1, 1:4], expected)
+    assert _run_c(exe, args, None).returncode == 0
+    out = np.fromfile(tmp_path / "o.raw", np.uint8).reshape(th, tw, depth)
+    keep = np.ones((th, tw), bool)
+    keep[y0:y1, x0:x1] = False
+    assert np.array_equal(out[keep], tp[keep]) and np.array_equal(out[..., 0], tp[..., 0])
+    # imageSynth2 with a corpus band
+    img, mask = _case_inputs(R.T_RGB, (48, 60), 7)
+    corpus = np.zeros_like(mask)
+    corpus[:, :20] = 255
+    img.tofile(tmp_path / "i.raw")
+    mask.tofile(tmp_path / "m.raw")
+    corpus.tofile(tmp_path / "m2.raw")
+    r = _run_c(exe, ["synth2", R.T_RGB, 60, 48, tmp_path / "i.raw", tmp_path / "m.raw", tmp_path / "m2.raw",
+                     tmp_path / "o2.raw"], "serial")
+    assert r.returncode == 0
+    rc, oc = O.image_synth(img, mask, 0, _oparams(R.default_params()), mask2=corpus)
+    assert rc.err == 0 and np.array_equal(np.fromfile(tmp_path / "o2.raw", np.uint8).reshape(oc.shape), oc)
+
+
+# ---------------------------------------------------------------- failures return codes
+
+def test_out_of_device_memory_returns_an_error_and_recovers():
+    """An engine call that cannot get its device memory returns RESYNTH_ERROR_DEVICE and
+    leaves the caller's image as it was; the next call with memory succeeds and is exact."""
+    img, mask = _case_inputs(R.T_RGB, (64, 80), 3)
+    p = R.default_params(maxProbeCount=50)
+    try:
+        R.set_device_memory_limit(1 << 20)
+        r, out = R.image_synth(img, mask, R.T_RGB, p, mode=R.MODE_SERIAL)
+        assert r.err == R.RESYNTH_ERROR_DEVICE and np.array_equal(out, img)
+    finally:
+        R.set_device_memory_limit(0)
+    rg, og = R.image_synth(img, mask, R.T_RGB, p, mode=R.MODE_SERIAL)
+    rc, oc = O.image_synth(img, mask, 0, _oparams(p))
+    assert rg.err == rc.err == 0 and np.array_equal(og, oc)
+
+
+def test_batch_rejects_inconsistent_requests_and_honours_options():
+    """A request whose row pitch is shorter than its rows fails alone with
+    RESYNTH_ERROR_INVALID_ARGUMENT (and the autotuner skips it); maxPasses caps every
+    request's passes."""
+    imgs, masks = [], []
+    for k in range(3):
+        im, m = _case_inputs(R.T_RGB, (50 + 7 * k, 60 + 5 * k), 40 + k)
+        imgs.append(im)
+        masks.append(m)
+    reqs, keep = R._requests(imgs, masks, [R.T_RGB] * 3, [R.default_params(maxProbeCount=40)] * 3, False)
+    reqs[1].image.rowBytes = 10
+    st = R.ResynthBatchStats()
+    opt = R.options(max_passes=1)
+    import ctypes
+    assert R.lib().resynth_image_synth_batch(reqs, 3, 0, ctypes.byref(opt), None, ctypes.byref(st)) == 0
+    assert reqs[1].error == R.RESYNTH_ERROR_INVALID_ARGUMENT
+    assert reqs[0].error == 0 and reqs[2].error == 0 and reqs[0].passes == 1 and reqs[2].passes == 1
+    best = R.ResynthKernelConfig()
+    assert R.lib().resynth_autotune_batch(reqs, 3, 0, ctypes.byref(best), None) == 0

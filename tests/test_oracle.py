@@ -99,6 +99,42 @@ def test_merge_sort_directional_ties(vectors):
     more_h = lambda a, b: -1 if less_h(a, b) == 1 else 1
     assert _merge_sort(pts, less_h).tolist() == vectors["sort"]["points_300_lessHorizontal"]
     assert _merge_sort(pts, more_h).tolist() == vectors["sort"]["points_300_moreHorizontal"]
+    less_v = lambda a, b: -1 if a[1] * a[1] < b[1] * b[1] else 1
+    more_v = lambda a, b: -1 if less_v(a, b) == 1 else 1
+    assert _merge_sort(pts, less_v).tolist() == vectors["sort"]["points_300_lessVertical"]
+    assert _merge_sort(pts, more_v).tolist() == vectors["sort"]["points_300_moreVertical"]
+
+
+def _ordered(mtype, pts):
+    err, out, _ = O.order_targets(mtype, np.array(pts, dtype=np.int32))
+    assert err == 0
+    return np.asarray(out, dtype="<i4").reshape(-1, 2)
+
+
+@pytest.mark.parametrize("mtype", range(9))
+def test_target_ordering_matches_glib(vectors, mtype):
+    """orderTargetPoints (lib/orderTarget.h:269-343) for every match-context type, on hole
+    shapes that include ties, a target whose centre lies outside it, and targets whose
+    centre pixel has no other pixel on its ray (ratio 0/0 = NaN, lib/brushfire.h:117-128):
+    the oracle equals the whole ordering computed by GLib's g_array_sort and
+    g_rand_int_range and the C library's atan2 (tests/golden/make_golden.py)."""
+    import hashlib
+    for name, e in vectors["order"].items():
+        got = _ordered(mtype, e["points"])
+        if str(mtype) in e["types"]:
+            assert got.tolist() == e["types"][str(mtype)], name
+        assert hashlib.sha256(got.tobytes()).hexdigest() == e["types"][str(mtype) + "_sha256"], name
+
+
+def test_target_ordering_pins_catch_plausible_mistakes(vectors):
+    """Each ordering family differs from its plausible confusions on the pinned shapes, so
+    the vectors above would catch a swapped comparator or a dropped band shuffle."""
+    e = vectors["order"]["ellipse"]
+    t = {k: e["types"][k] for k in map(str, range(9))}
+    assert t["2"] != t["5"] and t["3"] != t["6"] and t["4"] != t["7"] and t["3"] != t["4"]
+    assert t["2"] == t["8"]          # the squeeze's result is the inward order (lib/orderTarget.h:209-262)
+    assert t["0"] == t["1"]
+    assert sorted(map(tuple, t["5"])) == sorted(map(tuple, e["points"]))
 
 
 # ---------------------------------------------------------------- tables and schedule
@@ -247,7 +283,7 @@ def test_match_context_type_range_error():
 
 # ---------------------------------------------------------------- reference regression images
 
-FAST_PINS = ["resynthfull-zap-texture", "resynthfull-ufo-input", "resynth-ufo-input",
+FAST_PINS = ["resynthfull-zap-texture", "resynthfull-ufo-input", "resynthfull-brick", "resynth-ufo-input",
              "resynthtwoimages-ufo-input", "resynthtileable-ufo-input", "resynthfull-donkey_original"]
 
 
