@@ -224,6 +224,8 @@ typedef struct {
   double learnedIndexMs;              /* host wall clock of embedding the corpus and indexing */
   double learnedProposeMs;            /* proposal kernels (features, embedding, search; timing != 0) */
   unsigned long long learnedKeysScored; /* query-key dot products of the fine search */
+  unsigned long long scratchBytes;    /* resynth_image_synth_device_noalloc: device scratch the
+                                         run used (0 for the other entries) */
 } ResynthStats;
 
 void resynth_set_default_mode(int mode);
@@ -249,6 +251,28 @@ int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsig
                                TImageFormat imageFormat, const TImageSynthParameters *parameters,
                                void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
                                const ResynthOptions *options, ResynthStats *stats);
+
+/* resynth_image_synth_device without any memory allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and error codes as
+ * resynth_image_synth_device, serial and batched modes, plain method only (options->model
+ * must be NULL: RESYNTH_ERROR_INVALID_ARGUMENT).  Every device buffer of the run is carved
+ * from a fixed RESYNTH_NOALLOC_SCRATCH_MIB MiB array in the library's device image, and
+ * every host staging buffer from a fixed array in its host image; images above
+ * RESYNTH_NOALLOC_MAX_PIXELS pixels return RESYNTH_ERROR_TOO_LARGE.  The first call on a
+ * device looks up that device's copy of the array (and, if it is the library's first call,
+ * sets up its stream); later calls allocate nothing.  All calls on one device share the
+ * array, which keeps the last call's contents (each run initialises what it reads); calls
+ * are serialized by the library's lock, and a caller that shares the array's device with
+ * another process's copy of the library shares nothing.  options->timing is ignored. */
+#define RESYNTH_NOALLOC_MAX_PIXELS (1u << 20)
+#define RESYNTH_NOALLOC_SCRATCH_MIB 192
+int resynth_image_synth_device_noalloc(unsigned char *d_image, unsigned int width, unsigned int height,
+                                       size_t rowBytes, const unsigned char *d_mask, size_t maskRowBytes,
+                                       const unsigned char *d_mask2, size_t mask2RowBytes,
+                                       TImageFormat imageFormat, const TImageSynthParameters *parameters,
+                                       void (*progressCallback)(int, void *), void *contextInfo,
+                                       int *cancelFlag, const ResynthOptions *options, ResynthStats *stats);
 
 /* ---- Many requests in one launch.  No counterpart in the reference (each imageSynth call
  * there is one request); for servers healing many images of uneven sizes at once.  Every

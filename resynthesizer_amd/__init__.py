@@ -27,6 +27,9 @@ IMAGE_SYNTH_ERROR_EMPTY_CORPUS = 6
 RESYNTH_ERROR_INVALID_ARGUMENT = -1
 RESYNTH_ERROR_DEVICE = 100
 RESYNTH_ERROR_TOO_LARGE = 101
+# limits of resynth_image_synth_device_noalloc
+NOALLOC_MAX_PIXELS = 1 << 20
+NOALLOC_SCRATCH_BYTES = 192 << 20
 
 MODE_SERIAL, MODE_BATCHED = 0, 1
 PRNG_GRAND, PRNG_LIBC = 0, 1
@@ -81,7 +84,7 @@ class ResynthStats(ctypes.Structure):
                 ("wavePatchTerms", ctypes.c_ulonglong), ("learnedLaunches", ctypes.c_uint),
                 ("learnedKeys", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
                 ("learnedIndexMs", ctypes.c_double), ("learnedProposeMs", ctypes.c_double),
-                ("learnedKeysScored", ctypes.c_ulonglong)]
+                ("learnedKeysScored", ctypes.c_ulonglong), ("scratchBytes", ctypes.c_ulonglong)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -151,6 +154,11 @@ SIGNATURES = {
                                                   ctypes.c_int, _P(TImageSynthParameters), PROGRESS,
                                                   ctypes.c_void_p, _P(ctypes.c_int), _P(ResynthOptions),
                                                   _P(ResynthStats)]),
+    "resynth_image_synth_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint,
+                                                          ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                                          ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                                          _P(TImageSynthParameters), PROGRESS, ctypes.c_void_p,
+                                                          _P(ctypes.c_int), _P(ResynthOptions), _P(ResynthStats)]),
     "resynth_image_synth_batch": (ctypes.c_int, [_P(ResynthRequest), ctypes.c_uint, ctypes.c_int,
                                                  _P(ResynthOptions), _P(ResynthKernelConfig),
                                                  _P(ResynthBatchStats)]),

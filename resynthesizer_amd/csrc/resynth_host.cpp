@@ -446,6 +446,55 @@ int default_mode()
   return RESYNTH_MODE_BATCHED;
 }
 
+// ------------------------------------------------------------------ allocation-free scratch
+
+// Device scratch of resynth_image_synth_device_noalloc: every buffer of its engine run is
+// carved from this array, so the entry allocates nothing.  Each device the library runs on
+// holds its own copy until the process ends.  Host code must use the address that
+// hipGetSymbolAddress returns for the current device (scratch_address), never the name.
+constexpr size_t kScratchBytes = (size_t)RESYNTH_NOALLOC_SCRATCH_MIB << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Host staging of the same runs (point counts, tables, target permutation, serial PRNG
+// stream): pageable memory in the library image, used only under the library lock.
+constexpr size_t kNoAllocHostBytes = (size_t)12 << 20;
+alignas(256) unsigned char g_noallocHost[kNoAllocHostBytes];
+
+constexpr int kMaxDevices = 64;
+char *g_scratchPtr[kMaxDevices] = {};
+
+// The scratch array's address on the library's device, looked up once per device.
+char *scratch_address()
+{
+  const int dev = g_dev.device;
+  if (dev < 0 || dev >= kMaxDevices) hip_fail(hipErrorInvalidDevice, "scratch device index", __FILE__, __LINE__);
+  if (!g_scratchPtr[dev]) {
+    int cur = -1;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIP_TRY(hipSetDevice(cur));
+    HIP_TRY(e);
+    g_scratchPtr[dev] = (char *)ptr;
+  }
+  return g_scratchPtr[dev];
+}
+
+// Makes Buf and PinnedBuf carve from the two arenas for the lifetime of the scope.
+struct ArenaScope {
+  ArenaScope(Arena *d, Arena *h)
+  {
+    g_devArena = d;
+    g_hostArena = h;
+  }
+  ~ArenaScope()
+  {
+    g_devArena = nullptr;
+    g_hostArena = nullptr;
+  }
+};
+
 // ------------------------------------------------------------------ engine job
 
 // Where the two images come from.
@@ -1901,6 +1950,34 @@ int resynth_image_synth_device(unsigned char *d_image, unsigned int width, unsig
     return image_synth_device_impl(g_dev.ws, d_image, width, height, rowBytes, d_mask, maskRowBytes, d_mask2,
                                    mask2RowBytes, imageFormat, parameters, progressCallback, contextInfo, cancelFlag,
                                    options, stats);
+  });
+}
+
+int resynth_image_synth_device_noalloc(unsigned char *d_image, unsigned int width, unsigned int height,
+                                       size_t rowBytes, const unsigned char *d_mask, size_t maskRowBytes,
+                                       const unsigned char *d_mask2, size_t mask2RowBytes, TImageFormat imageFormat,
+                                       const TImageSynthParameters *parameters,
+                                       void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
+                                       const ResynthOptions *options, ResynthStats *stats)
+{
+  if ((size_t)width * height > (size_t)RESYNTH_NOALLOC_MAX_PIXELS) return RESYNTH_ERROR_TOO_LARGE;
+  if (options && options->model) return RESYNTH_ERROR_INVALID_ARGUMENT;   // the learned index is not bounded
+  return guarded([&]() -> int {
+    std::lock_guard<std::mutex> lock(g_mutex);
+    if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
+    Arena dev{scratch_address(), kScratchBytes, 0};
+    Arena host{(char *)g_noallocHost, kNoAllocHostBytes, 0};
+    ResynthOptions o;
+    if (options) o = *options;
+    else resynth_default_options(&o);
+    o.timing = 0;   // timing events are created per launch
+    ArenaScope scope(&dev, &host);
+    Workspace W;    // every buffer of the run is carved from the arenas
+    const int err = image_synth_device_impl(W, d_image, width, height, rowBytes, d_mask, maskRowBytes, d_mask2,
+                                            mask2RowBytes, imageFormat, parameters, progressCallback, contextInfo,
+                                            cancelFlag, &o, stats);
+    if (stats) stats->scratchBytes = dev.used;
+    return err;
   });
 }
 
