@@ -197,6 +197,12 @@ typedef struct {
                                    refiner's rule would go on (0 = the rule alone) */
   unsigned int rampPermille;    /* batched mode: pass-0 chunks hold this many permille of the
                                    targets visited so far (0 = 60) */
+  unsigned int lanesPerTarget;  /* batched mode: 0 = the library's choice (large chunks: wavefronts
+                                   split into segments of 64, 32 or 16 lanes by the random-probe
+                                   count, one target per segment); 16, 32 or 64 = every chunk
+                                   visited with segments of that many lanes; 1 = every chunk by
+                                   one wavefront per target.  A target's result does not depend on
+                                   it: the choice trades issue slots against latency. */
 } ResynthOptions;
 
 typedef struct {

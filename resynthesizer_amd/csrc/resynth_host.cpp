@@ -764,6 +764,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
     return e ? std::atoi(e) : -1;
   }();
   g.probeByPixel = probeKnob >= 0 ? (probeKnob ? 1 : 0) : ((uint64_t)nC * 2 >= (uint64_t)nCpix ? 1 : 0);
+  g.drawScale = (unsigned)std::min<uint64_t>(((uint64_t)nCpix << 16) / nC + 1, 64ull << 16);
   // RESYNTH_XCD_BANDS=1: batched random probes come from the XCD's band of corpus rows
   // when the corpus words outgrow an XCD's L2.  Off by default: on a 2048x2048 heal it cuts
   // the visit kernel's HBM fetches about 9x but not its time (the scorer is bound by
@@ -1176,8 +1177,9 @@ int refine_engine(Job &job, Prepared &pr)
           } else if (learned.ok && !learnPass && job.opt.learnedProbesLater) {
             gl.maxProbe = job.opt.learnedProbesLater;
           }
-          record_event(job, ev, true, k1 - k >= synth_wave_min() || bl.props);
-          HIP_TRY(launch_synth_batched(s, job.wide, gl, bl, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass));
+          record_event(job, ev, true, job.opt.lanesPerTarget || k1 - k >= synth_wave_min() || bl.props);
+          HIP_TRY(launch_synth_batched(s, job.wide, gl, bl, k, k1, job.opt.seed ? job.opt.seed : kEngineSeed, pass,
+                                       (int)job.opt.lanesPerTarget));
           record_event(job, ev, false);
           launches++;
         }
@@ -1467,17 +1469,32 @@ unsigned env_permille(const char *name, long dflt)
   return (unsigned)std::min(1000L, std::max(1L, v));
 }
 
-// Sliding-window parameters of a request: the single-request schedule's chunk sizes
-// halved (see refine_engine), with the options' pass-0 ramp.
+// Sliding-window parameters of a request: the single-request schedule's chunk sizes divided
+// by RESYNTH_MANY_WINDOW_DIV0 in pass 0 (default 3) and RESYNTH_MANY_WINDOW_DIV later
+// (default 8), with the options' pass-0 ramp.  On the 96 requests of the uneven-batch
+// workload (every fourth request, three probe seeds) the mean match error against the
+// serial engine's, and the launch time, were: 2/2 1.025, 526 ms; 3/3 1.012, 584 ms;
+// 3/8 1.004, 597 ms; 8/2 1.003, 735 ms; 6/6 0.998, 653 ms.  Windows in the later passes
+// are cheap to narrow: every visit there is short.
 void request_window(const ResynthOptions &o, unsigned nT, unsigned &minW, unsigned &ramp, unsigned &capW)
 {
+  static const unsigned div = [] {
+    const char *e = std::getenv("RESYNTH_MANY_WINDOW_DIV");
+    const long v = e ? std::atol(e) : 8;
+    return (unsigned)std::min(64L, std::max(1L, v));
+  }();
+  static const unsigned div0 = [] {
+    const char *e = std::getenv("RESYNTH_MANY_WINDOW_DIV0");
+    const long v = e ? std::atol(e) : 3;
+    return (unsigned)std::min(64L, std::max(1L, v));
+  }();
   const unsigned cap = o.batch ? o.batch : std::min(65536u, std::max(64u, nT / 32u));
   const unsigned minChunk = std::min(cap, std::max(1u, std::min(64u, nT / 2048u)));
-  minW = std::max(1u, minChunk / 2);
+  minW = std::max(1u, minChunk / div0);
   const unsigned rampPermille =
       o.rampPermille ? std::min(1000u, o.rampPermille) : env_permille("RESYNTH_RAMP_PERMILLE", 60);
-  ramp = std::max(1u, rampPermille / 2);
-  capW = std::max(1u, cap / 2);
+  ramp = std::max(1u, rampPermille / div0);
+  capW = std::max(1u, cap / div);
 }
 
 // A request whose buffers cannot be what its sizes say: null data, or a row pitch shorter

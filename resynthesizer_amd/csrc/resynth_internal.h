@@ -41,6 +41,8 @@ struct Geometry {
   int probeByPixel;       // batched probes by rejection over corpus pixels (dense corpora)
   int bands;              // batched: 0, or kBands (a workgroup on XCD x draws its random probes
                           // from corpus rows [x*ch/kBands, (x+1)*ch/kBands))
+  unsigned drawScale;     // batched, probes by pixel: ceil(65536 * corpus pixels / corpus points), the
+                          // draws a round makes per remaining valid probe (16.16 fixed point)
 };
 
 constexpr int kBands = 8;   // one band of corpus rows per XCD
@@ -155,8 +157,9 @@ hipError_t launch_order_sort(hipStream_t s, const int2 *rm, unsigned n, int type
                              void *temp, size_t tempBytes, const uint32_t **order);
 hipError_t launch_order_compose(hipStream_t s, const int2 *rm, const uint32_t *order, const uint32_t *perm, int2 *out,
                                 unsigned n);
+// lpt: 0 = by chunk size and pass; 1 = one wavefront per target; 16/32/64 = segmented visits
 hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
-                                unsigned k1, unsigned seed, unsigned pass);
+                                unsigned k1, unsigned seed, unsigned pass, int lpt = 0);
 unsigned synth_wave_min();   // chunks of at least this many targets use k_synth_wave
 hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,

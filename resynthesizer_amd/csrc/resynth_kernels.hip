@@ -968,6 +968,307 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   myTerm += (unsigned long long)evals * count;
 }
 
+// ------------------------------------------------------------------ segmented visits
+
+// A wavefront split into 64/LPT segments of LPT lanes, one target each.  Phases with few
+// candidates (a patch of heuristic-1 candidates, 16 learned proposals, a handful of random
+// probes) then fill the wave with several targets instead of leaving most lanes idle, and
+// each instruction of the visit serves 64/LPT targets.  bits() is a ballot restricted to
+// the lane's own segment, in segment-local lane order.
+template <int LPT>
+struct Seg {
+  static_assert(LPT == 16 || LPT == 32 || LPT == 64, "segments of 16, 32 or 64 lanes");
+  unsigned lane, seg, sl, base;
+  __device__ __forceinline__ Seg()
+  {
+    lane = lane_id();
+    seg = lane / LPT;
+    sl = lane % LPT;
+    base = seg * LPT;
+  }
+  __device__ __forceinline__ uint64_t bits(uint64_t m) const
+  {
+    return LPT == 64 ? m : (m >> base) & ((1ull << LPT) - 1ull);
+  }
+  __device__ __forceinline__ uint64_t lt() const { return (1ull << sl) - 1ull; }
+};
+
+// Sequential strict-less resolution within each segment, in segment lane order (the
+// segmented form of resolve).  best/win/stop are the lane's segment's values.
+template <int LPT>
+__device__ __forceinline__ void resolve_seg(const Seg<LPT> &S, bool cand, uint32_t M, uint32_t F, uint32_t &best,
+                                            int &win, int &stop)
+{
+  uint64_t rem = __ballot(cand);
+  stop = LPT;
+  for (;;) {
+    const uint64_t m = __ballot(cand && M < best) & rem;
+    if (!m) break;
+    const uint64_t mine = S.bits(m);
+    const int i = mine ? (int)__ffsll((unsigned long long)mine) - 1 : 0;
+    const uint32_t Fi = __shfl(F, (int)S.base + i);
+    uint64_t segRem = S.bits(rem);
+    if (mine) {
+      best = Fi;
+      win = i;
+      segRem &= (i == LPT - 1) ? 0ull : ~((2ull << i) - 1ull);
+      if (Fi == 0) {
+        stop = i;
+        segRem = 0;
+      }
+    } else {
+      segRem = 0;   // nothing left in this segment can beat its best
+    }
+    rem = __ballot((segRem >> S.sl) & 1ull);
+  }
+}
+
+// Neighbour patch (lib/synthesize.h:189-241) of each segment's target, gathered by its
+// LPT lanes: neighbour 0 is the target itself, then the sorted offsets LPT at a time.
+// Returns the segment's patch size.
+template <typename Word, int LPT>
+__device__ unsigned gather_neighbours_seg(const Seg<LPT> &S, const Geometry &g, const Word *tword,
+                                          const int2 *sourceOf, const int2 *offsets, int tx, int ty, bool active,
+                                          int2 *nbOff, Word *nbPix, int2 *nbSrc, const uint8_t *tileHas,
+                                          const uint32_t *firstIdx)
+{
+  const size_t tp = (size_t)ty * (unsigned)g.tw + (unsigned)tx;
+  if (active && S.sl == 0) {
+    nbOff[0] = make_int2(0, 0);
+    nbPix[0] = tword[tp];
+    nbSrc[0] = sourceOf[tp];
+  }
+  unsigned count = 1;
+  const unsigned L = g.patchLimit;
+  unsigned base = 1;
+  if (g.skipEmpty && tileHas && active)
+    base = scan_start(g, tileHas, firstIdx, tx, ty, S.sl, (unsigned)LPT,
+                      [&S](bool v) { return S.bits(__ballot(v)) != 0ull; });
+  for (;;) {
+    const bool need = active && count < L && base < g.nO;
+    if (__ballot(need) == 0ull) break;
+    if (need) {
+      const unsigned j = base + S.sl;
+      const int2 off = offsets[j < g.nO ? j : 0u];
+      int px = tx + off.x, py = ty + off.y;
+      bool in = j < g.nO;
+      if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
+      else if (px >= g.tw) { if (g.tileH) px -= g.tw; else in = false; }
+      if (py < 0) { if (g.tileV) py += g.th; else in = false; }
+      else if (py >= g.th) { if (g.tileV) py -= g.th; else in = false; }
+      const size_t p = in ? (size_t)(__umul24((unsigned)py, (unsigned)g.tw) + (unsigned)px) : tp;
+      const Word w = tword[p];
+      const bool v = in && (w & 1u) != 0;
+      const uint64_t m = S.bits(__ballot(v));
+      const unsigned rank = (unsigned)__popcll(m & S.lt());
+      if (v && count + rank < L) {
+        const unsigned idx = count + rank;
+        nbOff[idx] = off;
+        nbPix[idx] = w;
+        nbSrc[idx] = sourceOf[p];
+      }
+      const unsigned c2 = count + (unsigned)__popcll(m);
+      count = c2 < L ? c2 : L;
+      base += LPT;
+    }
+  }
+  wave_sync();
+  return count;
+}
+
+// One batched visit (lib/synthesize.h:426-642) of each of the wave's 64/LPT targets
+// kb + seg (those below k1).  Per target it is visit_wave's visit: the same patch, the same
+// heuristic-1 candidates in neighbour order (deduplicated, prober-checked), the same
+// proposals, and the same per-target counter-based probe stream resolved in attempt order,
+// so a target gets the same result whatever LPT is.  A probe round draws only the attempts
+// the remaining budget needs (for probes by pixel, scaled by the corpus density).
+template <typename Word, int LPT, int G0, int GL, bool PUBLISH, int NC = 0>
+__device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers &b, unsigned kb, unsigned k1,
+                                          unsigned seed, unsigned pass, int2 *sOff, Word *sPix, int2 *sSrc,
+                                          const uint16_t *sT, const uint32_t *sTm, unsigned long long &myEval,
+                                          unsigned long long &myTerm, unsigned &myBet)
+{
+  const Seg<LPT> S;
+  Word *tword = (Word *)b.tword;
+  const Word *cword = (const Word *)b.cword;
+  const bool skip0 = g.nMaps == 0;
+  const unsigned k = kb + S.seg;
+  const bool active = k < k1;
+  const int2 t = b.targets[active ? k : kb];
+  // the proposals do not depend on the neighbourhood: their load overlaps the scan
+  int2 prop = make_int2(-1, -1);
+  if (b.props && active && S.sl < 16) prop = b.props[(size_t)(k - b.propsK0) * 16 + S.sl];
+  const unsigned count = gather_neighbours_seg<Word, LPT>(S, g, tword, b.sourceOf, b.offsets, t.x, t.y, active, sOff,
+                                                          sPix, sSrc, b.tileHas, b.firstIdx);
+  uint32_t best = 0xFFFFFFFFu;
+  int winX = 0, winY = 0;
+  bool perfect = !active, scored = false;
+  unsigned evals = 0;
+
+  // heuristic 1 (lib/synthesize.h:537-580): neighbour i proposes source(i) - offset(i).
+  // First each neighbour's candidate replaces its source in sSrc ((INT_MIN, INT_MIN) when
+  // the neighbour has no source), so that a candidate can be compared with every earlier one.
+  if (active)
+    for (unsigned i = S.sl; i < count; i += LPT) {
+      const int2 src = sSrc[i], o = sOff[i];
+      sSrc[i] = src.x != -1 ? make_int2(src.x - o.x, src.y - o.y) : make_int2(INT_MIN, INT_MIN);
+    }
+  wave_sync();
+  for (unsigned r0 = 0;; r0 += LPT) {
+    const bool go = !perfect && r0 < count;
+    if (__ballot(go) == 0ull) break;
+    const unsigned i = r0 + S.sl;
+    const bool mine = go && i < count;
+    const int2 c = sSrc[mine ? i : 0u];
+    const bool in = mine && c.x != INT_MIN && (unsigned)c.x < (unsigned)g.cw && (unsigned)c.y < (unsigned)g.ch;
+    const unsigned ci = in ? __umul24((unsigned)c.y, (unsigned)g.cw) + (unsigned)c.x : 0u;
+    const Word q = cword[ci];
+    const uint32_t pr = b.prober[ci];
+    bool valid = in && (q & 0xFEu) == 0xFEu && pr != k;
+    // an earlier neighbour proposing the same point was scored (and recorded) already
+    if (valid)
+      for (unsigned j = 0; j < i; j++) {
+        const int2 e = sSrc[j];
+        if (e.x == c.x && e.y == c.y) {
+          valid = false;
+          break;
+        }
+      }
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    if (valid) score_candidate<Word, 8, 8, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
+    const uint64_t used0 = S.bits(__ballot(valid));
+    int win = -1, stop = LPT;
+    uint32_t bb = best;
+    resolve_seg<LPT>(S, valid && alive, M, F, bb, win, stop);
+    const int wx = __shfl(c.x, (int)S.base + (win >= 0 ? win : 0));
+    const int wy = __shfl(c.y, (int)S.base + (win >= 0 ? win : 0));
+    if (win >= 0) {
+      best = bb;
+      winX = wx;
+      winY = wy;
+      scored = true;
+    }
+    uint64_t used = used0, rec = used0;
+    if (stop < LPT) {
+      perfect = true;
+      used &= (stop == LPT - 1) ? ~0ull : ((2ull << stop) - 1ull);
+      rec &= (1ull << stop) - 1ull;
+    }
+    if (go) evals += (unsigned)__popcll(used);
+    // recentProber (lib/synthesize.h:562-575): every candidate scored before the stop; the
+    // loop breaks on a perfect match before recording it
+    if (valid && ((rec >> S.sl) & 1ull)) b.prober[ci] = k;
+  }
+
+  // learned proposals: up to 16 corpus points, scored in rank order like probes
+  if (b.props) {
+    const bool go = !perfect;
+    const bool valid = go && S.sl < 16 && prop.x >= 0;
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    if (valid)
+      score_candidate<Word, G0, GL, NC>(g, cword, prop.x, prop.y, count, sOff, sPix, sT, sTm, best, F, M, alive,
+                                        skip0);
+    int win = -1, stop = LPT;
+    uint32_t bb = best;
+    resolve_seg<LPT>(S, valid && alive, M, F, bb, win, stop);
+    const int wx = __shfl(prop.x, (int)S.base + (win >= 0 ? win : 0));
+    const int wy = __shfl(prop.y, (int)S.base + (win >= 0 ? win : 0));
+    if (win >= 0) {
+      best = bb;
+      winX = wx;
+      winY = wy;
+      scored = true;
+    }
+    uint64_t used = S.bits(__ballot(valid));
+    if (stop < LPT) {
+      perfect = true;
+      used &= (stop == LPT - 1) ? ~0ull : ((2ull << stop) - 1ull);
+    }
+    if (go) evals += (unsigned)__popcll(used);
+  }
+
+  // random probes: the per-target counter-based stream (seed, pass, target, attempt) of
+  // visit_wave, taken in rounds of at most LPT attempts
+  const uint64_t key = ((uint64_t)seed << 40) ^ ((uint64_t)pass << 36) ^ ((uint64_t)k << 8);
+  const bool byPixel = g.probeByPixel != 0;
+  int r0, r1;
+  probe_band(g, r0, r1);
+  const uint64_t nPix = (uint64_t)g.cw * (uint64_t)(r1 - r0);
+  const uint64_t pix0 = (uint64_t)g.cw * (uint64_t)r0;
+  unsigned drawn = 0, attempt = 0;
+  for (;;) {
+    const bool go = !perfect && drawn < g.maxProbe;
+    if (__ballot(go) == 0ull) break;
+    const unsigned remaining = go ? g.maxProbe - drawn : 0u;
+    unsigned m = byPixel ? (unsigned)(((uint64_t)remaining * g.drawScale + 0xFFFFu) >> 16) + 1u : remaining;
+    m = m < (unsigned)LPT ? m : (unsigned)LPT;
+    const bool draw = go && S.sl < m;
+    const uint64_t h = mix64(key * 0x100000001B3ull + attempt + S.sl);
+    int2 c = make_int2(0, 0);
+    bool valid = false;
+    if (draw) {
+      if (byPixel) {
+        const uint64_t p = pix0 + (((h >> 32) * nPix) >> 32);
+        c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
+        valid = (cword[p] & 0xFFu) == 0xFFu;   // a corpus point: selected and opaque
+      } else {
+        c = b.corpusPts[probe_index(g, b, h)];
+        valid = true;
+      }
+    }
+    const uint64_t vm = S.bits(__ballot(valid));
+    const bool act = valid && (unsigned)__popcll(vm & S.lt()) < remaining;
+    uint32_t F = 0, M = 0;
+    bool alive = false;
+    if (act) score_candidate<Word, G0, GL, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
+    int win = -1, stop = LPT;
+    uint32_t bb = best;
+    resolve_seg<LPT>(S, act && alive, M, F, bb, win, stop);
+    const int wx = __shfl(c.x, (int)S.base + (win >= 0 ? win : 0));
+    const int wy = __shfl(c.y, (int)S.base + (win >= 0 ? win : 0));
+    if (win >= 0) {
+      best = bb;
+      winX = wx;
+      winY = wy;
+      scored = true;
+    }
+    uint64_t used = S.bits(__ballot(act));
+    drawn += (unsigned)__popcll(used);
+    if (stop < LPT) {
+      perfect = true;
+      used &= (stop == LPT - 1) ? ~0ull : ((2ull << stop) - 1ull);
+    }
+    if (go) evals += (unsigned)__popcll(used);
+    attempt += m;
+  }
+
+  // store (lib/synthesize.h:620-639)
+  if (active && S.sl == 0) {
+    const size_t tp = (size_t)t.y * (unsigned)g.tw + (unsigned)t.x;
+    const Word old = tword[tp];
+    Word nw = old | (Word)1;
+    if (scored) {
+      const int2 cur = b.sourceOf[tp];
+      if (cur.x != winX || cur.y != winY) {
+        Word cmask = 0;
+        for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
+        const Word src = cword[(size_t)winY * (unsigned)g.cw + (unsigned)winX];
+        nw = (old & ~cmask) | (src & cmask) | (Word)1;
+        const uint64_t packed = (uint64_t)(uint32_t)winX | ((uint64_t)(uint32_t)winY << 32);
+        put<PUBLISH>((uint64_t *)&b.sourceOf[tp], packed);
+        myBet++;
+      }
+    }
+    put<PUBLISH>(&tword[tp], nw);
+    b.lastBest[tp] = best;
+    if (g.skipEmpty) b.tileHas[(size_t)(t.y / kTile) * g.tilesW + t.x / kTile] = 1;
+    myEval += evals;
+    myTerm += (unsigned long long)evals * count;
+  }
+}
+
 template <typename Word, int WAVES, bool WAVE_KERNEL>
 __device__ __forceinline__ void flush_counters(const SynthBuffers &b, unsigned long long myEval,
                                                unsigned long long myTerm, unsigned myBet,
@@ -1022,6 +1323,40 @@ void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned
   for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES)
     visit_wave<Word, G0, GL, false, NC>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval,
                                         myTerm, myBet);
+  flush_counters<Word, WAVES, true>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
+}
+
+// Batched visits, 64/LPT targets per wavefront (segmented visits; see visit_seg).
+template <typename Word, int WAVES, int LPT, int G0, int GL, int NC = 0>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(6)))
+void k_synth_seg(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned seed, unsigned pass)
+{
+  constexpr int TPW = 64 / LPT;
+  __shared__ uint16_t sT[512];
+  __shared__ uint32_t sTm[512];
+  __shared__ int2 sOff[WAVES][TPW][64];
+  __shared__ Word sPix[WAVES][TPW][64];
+  __shared__ int2 sSrc[WAVES][TPW][64];
+  __shared__ unsigned long long sEval[WAVES], sTerm[WAVES];
+  __shared__ unsigned sBet[WAVES];
+  for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+    sT[i] = b.tables->T[i];
+    sTm[i] = b.tables->Tm[i];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6;
+  const unsigned seg = lane_id() / LPT;
+  unsigned long long myEval = 0, myTerm = 0;
+  unsigned myBet = 0;
+  for (unsigned kb = k0 + (blockIdx.x * WAVES + wave) * TPW; kb < k1; kb += gridDim.x * WAVES * TPW)
+    visit_seg<Word, LPT, G0, GL, false, NC>(g, b, kb, k1, seed, pass, sOff[wave][seg], sPix[wave][seg],
+                                            sSrc[wave][seg], sT, sTm, myEval, myTerm, myBet);
+  // each segment's first lane holds its counts
+  for (int o = 32; o > 0; o >>= 1) {
+    myEval += __shfl_xor(myEval, o);
+    myTerm += __shfl_xor(myTerm, o);
+    myBet += __shfl_xor(myBet, o);
+  }
   flush_counters<Word, WAVES, true>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
 }
 
@@ -1153,11 +1488,60 @@ static void launch_wave_g(hipStream_t s, dim3 grid, const Geometry &g, const Syn
   }
 }
 
+// Lanes per target of the large-chunk visits: as many as a visit's random probes can use
+// (a wave of 64 lanes per target wastes most of its issue slots on a visit that scores 16
+// proposals and one probe).  Pass 0 keeps at least 32: its scans are long.  Measured on one
+// MI355X: a 4096x4096 learned heal (16 probes in pass 0, 1 probe + 16 proposals later)
+// took 16.9 ms with 32 then 16 lanes against 19.6 ms with one wavefront per target; a
+// 2048x2048 plain heal (200 probes) 27.3 ms with 64 lanes against 29.2 ms.
+// RESYNTH_LPT0 / RESYNTH_LPT override pass 0 / later passes: 16, 32, 64, or 1 (the
+// wavefront-per-target kernel).
+static int lanes_per_target(const Geometry &g, unsigned pass)
+{
+  static const int v0 = [] {
+    const char *e = std::getenv("RESYNTH_LPT0");
+    const int x = e ? std::atoi(e) : 0;
+    return (x == 1 || x == 16 || x == 32 || x == 64) ? x : 0;
+  }();
+  static const int v1 = [] {
+    const char *e = std::getenv("RESYNTH_LPT");
+    const int x = e ? std::atoi(e) : 0;
+    return (x == 1 || x == 16 || x == 32 || x == 64) ? x : 0;
+  }();
+  const int forced = pass == 0 ? v0 : v1;
+  if (forced) return forced;
+  int l = g.maxProbe >= 48 ? 64 : g.maxProbe >= 12 ? 32 : 16;
+  if (pass == 0 && l < 32) l = 32;
+  return l;
+}
+
+template <typename Word, int LPT>
+static void launch_seg_l(hipStream_t s, const Geometry &g, const SynthBuffers &b, unsigned k0, unsigned k1,
+                         unsigned seed, unsigned pass)
+{
+  constexpr int WAVES = 4, TPW = 64 / LPT;
+  unsigned blocks = (k1 - k0 + WAVES * TPW - 1) / (WAVES * TPW);
+  if (blocks > 16384) blocks = 16384;
+  const bool rgb = sizeof(Word) == 4 && g.nColors == 3 && g.nMaps == 0 && rgb_fast_path();
+  if (rgb)
+    hipLaunchKernelGGL((k_synth_seg<Word, WAVES, LPT, 2, 4, 3>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1,
+                       seed, pass);
+  else
+    hipLaunchKernelGGL((k_synth_seg<Word, WAVES, LPT, 2, 4>), dim3(blocks), dim3(64 * WAVES), 0, s, g, b, k0, k1,
+                       seed, pass);
+}
+
 template <typename Word>
 static void launch_wave(hipStream_t s, const Geometry &g, const SynthBuffers &b, unsigned k0, unsigned k1,
-                        unsigned seed, unsigned pass)
+                        unsigned seed, unsigned pass, int lpt)
 {
   constexpr int WAVES = 4;
+  switch (lpt ? lpt : lanes_per_target(g, pass)) {
+  case 16: launch_seg_l<Word, 16>(s, g, b, k0, k1, seed, pass); return;
+  case 32: launch_seg_l<Word, 32>(s, g, b, k0, k1, seed, pass); return;
+  case 64: launch_seg_l<Word, 64>(s, g, b, k0, k1, seed, pass); return;
+  default: break;
+  }
   unsigned blocks = (k1 - k0 + WAVES - 1) / WAVES;
   if (blocks > 16384) blocks = 16384;
   switch (first_group()) {
@@ -1168,14 +1552,14 @@ static void launch_wave(hipStream_t s, const Geometry &g, const SynthBuffers &b,
 }
 
 hipError_t launch_synth_batched(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
-                                unsigned k1, unsigned seed, unsigned pass)
+                                unsigned k1, unsigned seed, unsigned pass, int lpt)
 {
   constexpr int WAVES = 4;
   if (k1 <= k0) return hipSuccess;
-  // learned proposals are scored by the wavefront-per-target kernel only
-  if (k1 - k0 >= wave_min() || b.props) {
-    if (wide) launch_wave<uint64_t>(s, g, b, k0, k1, seed, pass);
-    else launch_wave<uint32_t>(s, g, b, k0, k1, seed, pass);
+  // learned proposals are scored by the wavefront kernels only
+  if (lpt || k1 - k0 >= wave_min() || b.props) {
+    if (wide) launch_wave<uint64_t>(s, g, b, k0, k1, seed, pass, lpt);
+    else launch_wave<uint32_t>(s, g, b, k0, k1, seed, pass, lpt);
     return hipGetLastError();
   }
   unsigned blocks = k1 - k0;
@@ -1226,6 +1610,18 @@ __device__ __forceinline__ unsigned long long first_lane64(unsigned long long v)
 
 // Tickets a wave takes at once: one counter update per group instead of per visit.
 constexpr unsigned kTicketGroup = 4;
+// How far past its window a ticket may be handed out (it then waits for the visits before
+// it): a few groups, so that waves queue on a request instead of leaving it.
+constexpr unsigned kAdmitSlack = 4 * kTicketGroup;
+
+// Visits of a pass that ticket t may overlap: the pass-0 ramp, or the cap.
+__device__ __forceinline__ unsigned many_window(const ReqDesc &d, unsigned pass, unsigned t)
+{
+  if (pass != 0) return d.capW;
+  const unsigned rr = (unsigned)((uint64_t)t * d.ramp / 1000u);
+  const unsigned W = rr > d.minW ? rr : d.minW;
+  return W < d.capW ? W : d.capW;
+}
 
 // 32-bit words with groups of 2 then 4 ask for 6 waves per SIMD (at most 80 VGPRs), which
 // they reach without spilling; the resident grid below follows from the occupancy query.
@@ -1254,7 +1650,10 @@ void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq
     if (lane == 0) {
       const unsigned long long cur = load_agent64(&c.ctr);
       unsigned p = (unsigned)(cur >> 32);
-      if (p < 6 && (unsigned)cur < d.sizes[p]) {
+      // admission: take tickets of this request only while they are within reach of its
+      // window (a ticket far ahead would only spin); otherwise serve another request
+      if (p < 6 && (unsigned)cur < d.sizes[p] &&
+          (unsigned)cur <= (unsigned)load_agent64(&c.pass[p].word) + many_window(d, p, (unsigned)cur) + kAdmitSlack) {
         t64 = atomicAdd(&c.ctr, (unsigned long long)kTicketGroup);
         p = (unsigned)(t64 >> 32);
         got = p < 6 && (unsigned)t64 < d.sizes[p];
@@ -1277,15 +1676,11 @@ void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq
       continue;
     }
     idleRound = 0;
+    idleTotal = 0;
     t64 = first_lane64(t64);
     const unsigned pass = (unsigned)(t64 >> 32), tk0 = (unsigned)t64;
     const unsigned nGot = min(kTicketGroup, d.sizes[pass] - tk0);
-    unsigned W = d.capW;
-    if (pass == 0) {
-      const unsigned rr = (unsigned)((uint64_t)tk0 * d.ramp / 1000u);
-      W = rr > d.minW ? rr : d.minW;
-      if (W > d.capW) W = d.capW;
-    }
+    const unsigned W = many_window(d, pass, tk0);
     if (tk0 > W) {
       const unsigned need = tk0 - W;
       if (lane == 0) {

@@ -777,3 +777,40 @@ def test_batch_rejects_inconsistent_requests_and_honours_options():
     assert reqs[0].error == 0 and reqs[2].error == 0 and reqs[0].passes == 1 and reqs[2].passes == 1
     best = R.ResynthKernelConfig()
     assert R.lib().resynth_autotune_batch(reqs, 3, 0, ctypes.byref(best), None) == 0
+
+
+# ---------------------------------------------------------------- segmented visits
+
+@pytest.mark.parametrize("fmt,probes,mtype", [(R.T_RGB, 200, 1), (R.T_RGBA, 30, 2), (R.T_Gray, 7, 1)])
+def test_segmented_visits_give_each_target_the_wave_kernels_result(fmt, probes, mtype):
+    """With one target per launch (batch 1: deterministic), wavefronts split into 16-, 32- or
+    64-lane segments give every target exactly what one wavefront per target gives: the same
+    image, the same candidates scored and the same betters."""
+    img, mask = _case_inputs(fmt, (40, 52), 60 + probes)
+    p = R.default_params(maxProbeCount=probes, matchContextType=mtype, patchSize=25)
+    ref = None
+    for lpt in (1, 16, 32, 64):
+        print("lanes per target", lpt, flush=True)
+        r, out = R.image_synth(img, mask, fmt, p, batch=1, lanes_per_target=lpt)
+        assert r.err == 0
+        got = (out, r.candidates, list(r.betters), r.passes, r.meanError)
+        if ref is None:
+            ref = got
+        else:
+            assert np.array_equal(got[0], ref[0]) and got[1:] == ref[1:], lpt
+
+
+@pytest.mark.parametrize("lpt", [16, 32, 64])
+def test_segmented_visits_keep_batched_quality(lpt):
+    """Concurrent segmented visits keep the batched mode's invariants and its mean error
+    (within the seed-to-seed spread of the wavefront kernel on a 256 x 256 heal)."""
+    img, mask = W.config1()
+    p = R.default_params()
+    e_wave = np.mean([R.image_synth(img, mask, R.T_RGB, p, seed=s, lanes_per_target=1)[0].meanError
+                      for s in range(1, 5)])
+    errs = []
+    for s in range(1, 5):
+        r, out = R.image_synth(img, mask, R.T_RGB, p, seed=s, lanes_per_target=lpt)
+        assert r.err == 0 and np.array_equal(out[mask == 0], img[mask == 0])
+        errs.append(r.meanError)
+    assert np.mean(errs) <= 1.04 * e_wave, (np.mean(errs), e_wave)
