@@ -7,20 +7,26 @@ Default workload (config 5, learned proposals vs the plain method): a 4096x4096 
 value-noise texture with a centred 1024x1024 hole (1 048 576 targets), patchSize 30,
 sensitivity 0.117, batched mode.  The proposal network is first trained on pairs the plain
 engine logs on 200 seeded textures (not timed); `value` is the learned mode's target px/s;
-the plain method (200 probes) is timed the same way, and plain runs with more probes give
-the plain method's time at the learned error (or a lower bound of it).
+the plain method (200 probes) is timed the same way.  At equal error: the fastest learned
+setting whose mean match error is at most the plain method's, timed against the plain
+method; plain runs with more probes bound the plain time at the learned error; the
+configured setting (proposals per chunk + 32 random probes, the refiner's whole schedule)
+is timed beside them.
 
 --workload config2: the plain method on a 2048x2048 RGBA value-noise texture with 12 seeded
 ellipses as the hole (531 632 targets), against the exact serial engine's error.
 --workload config3: 96 uneven heal requests in one multi-request launch.
+--workload config4: map-guided transfer through engine().
 
 Prints one JSON line.  `value` is target pixels per second of the whole job with the inputs
-already in HBM.  `roofline` prices the dominant kernel (the wavefront-per-target visit
-kernel) by its algorithmic gather bytes: one corpus pixel word per patch neighbour per
-scored candidate.  `cpu_baseline` is the CPU oracle in the reference's threaded mode
-(8 threads, lib/refinerThreaded.h) on a bounded sample of the same workload.
+already in HBM.  `roofline` prices the dominant kernels (the large-chunk visit kernels) by
+their algorithmic bytes: a corpus pixel word per patch neighbour per scored candidate, plus
+the per-visit neighbour scan (9 B per offset read), staging (12 B per neighbour) and
+write-back (13 B).  `cpu_baseline` is the CPU oracle in the reference's threaded mode
+(8 threads, lib/refinerThreaded.h) on the whole workload, with a config-2 leg.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W] [--no-cpu] [--no-quality]
+                       [--no-ladder] [--save-model F.npy | --load-model F.npy]
 """
 import argparse
 import json
@@ -36,18 +42,19 @@ HBM_PEAK_GBS = 8000.0
 
 
 def cpu_baseline(threads=8):
-    """Oracle in threaded-reference mode on config 2 scaled to 1024x1024 (same generators,
-    same ellipse count, semi-axes scaled by 1/2): px/s of a whole engine run."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as O
+    """The CPU oracle in the reference's threaded mode (8 threads) on the whole config-2 heal:
+    px/s of one engine run."""
+    O = _oracle()
     from resynthesizer_amd import workloads as W
-    img, mask = W.config2(size=1024)
+    img, mask = W.config2()
     t = time.time()
     r, _ = O.image_synth(img, mask, 1, O.default_params(), threads=threads)
     dt = time.time() - t
-    return {"value": r.nTargets / dt, "unit": "target px/s", "cores": threads, "kind": "port",
-            "sample": "config 2 generators at 1024x1024 (%d target px), one whole engine run, %.1f s, "
-                      "8-thread reference mode of the CPU oracle" % (r.nTargets, dt)}
+    out = {"value": round(r.nTargets / dt, 1), "unit": "target px/s", "cores": threads, "kind": "port",
+           "sample": "the whole config-2 heal (%d target px), one engine run of %.1f s, 8-thread reference "
+                     "mode of the CPU oracle" % (r.nTargets, dt)}
+    out.update(_cpu_info())
+    return out
 
 
 def bench_config3(args, np, torch, R, W):
@@ -115,6 +122,80 @@ def bench_config3(args, np, torch, R, W):
 # visited targets, 2 passes.
 LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
                ramp_permille=300)
+# The configured learned setting: proposals for every chunk from its targets' current
+# neighbourhoods (top 16 of 2 inverted-file clusters) beside 32 random probes, in every pass
+# of the refiner's own schedule (up to 6 passes, 10% stop rule).
+CONFIGURED = dict(learned_probes=32)
+# Cheaper learned settings (fewer random probes in pass 0, and in pass 1 beside the
+# proposals), searched for the fastest one whose error still reaches the plain method's.
+LEARNED_LADDER = [dict(LEARNED, learned_probes_later=8), dict(LEARNED, learned_probes_later=4),
+                  dict(LEARNED, learned_probes_later=2), dict(LEARNED, learned_probes_later=1, learned_probes=0)]
+# bytes priced per unit of visit-kernel work: a corpus pixel word per patch term of a
+# scored candidate (priced at its word width), 9 B per patch offset the neighbour scan
+# reads (an 8 B offset and the target's presence byte), 12 B per staged neighbour (offset,
+# pixel, source) and 13 B per visit's write-back (colour, source, best)
+SCAN_B, STAGE_B, VISIT_B = 9, 12, 13
+
+
+def visit_bytes(st):
+    """Algorithmic bytes of the large-chunk visit kernels of one run (ResynthStats)."""
+    return (st.wavePatchTerms * st.pixelWordBytes + st.waveScanned * SCAN_B + st.waveStaged * STAGE_B
+            + st.waveVisits * VISIT_B)
+
+
+def _profile_json(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def _roofline(st, kernel, traffic_file=None, counters_file=None):
+    """roofline object of the large-chunk visit kernels from one timed run's statistics:
+    algorithmic bytes per launch over the mean launch time (HIP events recorded on the
+    library's stream around every launch)."""
+    launches = max(st.waveLaunches, 1)
+    kern_ms = st.waveKernelMs
+    alg = visit_bytes(st)
+    achieved = alg / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    tr = _profile_json(traffic_file) if traffic_file else None
+    ct = _profile_json(counters_file) if counters_file else None
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "traffic": tr.get("hbm_bytes_per_launch") if tr else None,
+           "kernel": kernel, "launches": st.waveLaunches, "avg_launch_ms": round(kern_ms / launches, 4),
+           "algorithmic_bytes_per_launch": int(alg / launches),
+           "algorithmic_bytes_split": {"gathers": st.wavePatchTerms * st.pixelWordBytes,
+                                       "scan": st.waveScanned * SCAN_B, "staging": st.waveStaged * STAGE_B,
+                                       "write_back": st.waveVisits * VISIT_B},
+           "candidate_evals_per_s": round(st.candidates / (st.synthKernelMs / 1e3), 1) if st.synthKernelMs else None,
+           "valu_busy": ct.get("valu_busy") if ct else None,
+           "valu_insts_per_visit": ct.get("valu_insts_per_visit") if ct else None}
+    return out
+
+
+def _train_or_load(args, R, W, np):
+    """The proposal network: trained on pairs the plain engine logs on seeded textures, or
+    loaded from --load-model (a .npy of its parameters written by --save-model)."""
+    model = R.Model(seed=1)
+    info = {}
+    t = time.time()
+    if args.load_model:
+        model.set_params(np.load(args.load_model))
+        info["loaded"] = os.path.basename(args.load_model)
+    else:
+        for i, (img, mask) in enumerate(W.training_textures(args.train_textures, 512)):
+            err, _ = model.log_pairs(img, mask, R.T_RGB, R.default_params(), max_pairs=4096, seed=i + 1)
+            assert err == 0, err
+        losses = model.fit(args.train_steps, batch=1024, lr=1e-3, tau=0.1, seed=3)
+        info.update(train_pairs=int(model.pair_count()), train_textures=args.train_textures,
+                    train_steps=args.train_steps,
+                    train_loss_first_last=[round(float(losses[:20].mean()), 3), round(float(losses[-20:].mean()), 3)])
+    info["train_s"] = round(time.time() - t, 1)
+    if args.save_model:
+        np.save(args.save_model, model.params())
+    return model, info
 
 
 def bench_config5(args, np, torch, R, W):
@@ -122,15 +203,10 @@ def bench_config5(args, np, torch, R, W):
     with a centred 1024x1024 hole (1 048 576 targets).  The network is trained first (pairs
     logged by the plain engine on 200 seeded 512x512 textures of config 2's generator,
     InfoNCE, Adam); training is reported, not timed.  Timed steps run the learned mode; the
-    plain method (200 probes) is timed the same way, and plain runs with more probes bound
-    the plain method's time at the learned error."""
-    t = time.time()
-    model = R.Model(seed=1)
-    for i, (img, mask) in enumerate(W.training_textures(args.train_textures, 512)):
-        err, _ = model.log_pairs(img, mask, R.T_RGB, R.default_params(), max_pairs=4096, seed=i + 1)
-        assert err == 0, err
-    losses = model.fit(args.train_steps, batch=1024, lr=1e-3, tau=0.1, seed=3)
-    train_s = time.time() - t
+    plain method (200 probes) is timed the same way.  Equal error: the fastest learned
+    setting whose mean match error is at most the plain method's is timed against it; plain
+    runs with more probes bound the plain method's time at the learned error."""
+    model, train = _train_or_load(args, R, W, np)
     img, mask = W.config5(size=args.size or 4096)
     h, w, c = img.shape
     d_src = torch.from_numpy(img).cuda()
@@ -140,7 +216,7 @@ def bench_config5(args, np, torch, R, W):
     def run(probes, learned, seed=0, timing=False):
         d_img.copy_(d_src)
         torch.cuda.synchronize()
-        kw = dict(LEARNED, model=model) if learned else {}
+        kw = dict(learned, model=model) if learned is not None else {}
         r = R.image_synth_device(d_img.data_ptr(), w, h, w * c, d_mask.data_ptr(), w, R.T_RGB,
                                  R.default_params(maxProbeCount=probes), mode=R.MODE_BATCHED, seed=seed,
                                  timing=timing, **kw)
@@ -158,32 +234,41 @@ def bench_config5(args, np, torch, R, W):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, float(np.mean(errs))
 
-    sec_l, err_l = timed(200, True, args.steps, args.warmup)
-    sec_p, err_p = timed(200, False, max(2, args.steps // 2), 1)
-    rl = run(200, True, timing=True)
-    rp = run(200, False, timing=True)
+    sec_l, err_l = timed(200, LEARNED, args.steps, args.warmup)
+    sec_p, err_p = timed(200, None, max(2, args.steps // 2), 1)
+    rl = run(200, LEARNED, timing=True)
+    rp = run(200, None, timing=True)
     n_t = rl.nTargets
-    # the plain method with more probes, until its error reaches the learned error or the
-    # probe budget ends: its time there bounds the plain time at equal error from below
-    ladder = []
-    for probes in (() if args.no_ladder else (1000, 4000)):
-        sec, err = timed(probes, False, 1, 0)
-        ladder.append({"probes": probes, "ms": round(sec * 1e3, 1), "mean_match_error": round(err, 1)})
-        if err <= err_l:
-            break
-    if not ladder:   # profiling runs: the plain 200-probe run stands in
-        ladder.append({"probes": 200, "ms": round(sec_p * 1e3, 1), "mean_match_error": round(err_p, 1)})
-    reached = ladder[-1]["mean_match_error"] <= err_l
-    kern_ms = rl.waveKernelMs
-    # HBM bytes per launch of the learned call's k_synth_wave, from FETCH_SIZE/WRITE_SIZE
-    # counter runs of this workload (profiles/synth_wave_config5_traffic.md), when present
-    traffic5 = None
-    tfile5 = os.path.join(ROOT, "profiles", "synth_wave_config5_traffic.json")
-    if os.path.exists(tfile5):
-        with open(tfile5) as f:
-            traffic5 = json.load(f).get("hbm_bytes_per_launch")
-    alg = rl.wavePatchTerms * rl.pixelWordBytes
-    achieved = alg / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
+    ladder, learned_ladder, configured = [], [], None
+    if not args.no_ladder:
+        # the plain method with more probes, until its error reaches the learned error or the
+        # probe budget ends: its time there bounds the plain time at equal error from below
+        for probes in (1000, 4000):
+            sec, err = timed(probes, None, 1, 0)
+            ladder.append({"probes": probes, "ms": round(sec * 1e3, 1), "mean_match_error": round(err, 1)})
+            if err <= err_l:
+                break
+        # cheaper learned settings, while their error stays at or below the plain method's
+        for st in LEARNED_LADDER:
+            sec, err = timed(200, st, 3, 1)
+            learned_ladder.append({"setting": {k: v for k, v in st.items() if LEARNED.get(k) != v},
+                                   "ms": round(sec * 1e3, 2), "mean_match_error": round(err, 1)})
+            if err > err_p:
+                break
+        sec_c, err_c = timed(200, CONFIGURED, 3, 1)
+        rc = run(200, CONFIGURED, timing=True)
+        configured = {"setting": "proposals per chunk (top 16 of 2 clusters) + 32 random probes, every pass, "
+                                 "the refiner's schedule", "ms": round(sec_c * 1e3, 2),
+                      "px_per_s": round(n_t / sec_c, 1), "mean_match_error": round(err_c, 1),
+                      "passes": rc.passes, "speedup_vs_plain200": round(sec_p / sec_c, 3),
+                      "error_ratio_vs_plain200": round(err_c / err_p, 4)}
+    # equal error: the fastest learned setting (bench setting or a cheaper one) whose error
+    # is at most the plain 200-probe error
+    eq = [(sec_l * 1e3, err_l, "bench setting")] + [(e["ms"], e["mean_match_error"], e["setting"])
+                                                     for e in learned_ladder]
+    eq = [e for e in eq if e[1] <= err_p]
+    best_eq = min(eq, key=lambda e: e[0])
+    reached = bool(ladder) and ladder[-1]["mean_match_error"] <= err_l
     out = {
         "metric": METRIC, "value": round(n_t / sec_l, 1), "unit": "target px/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec_l * 1e3, 3),
@@ -196,24 +281,28 @@ def bench_config5(args, np, torch, R, W):
                    "plain200_mean_match_error": round(err_p, 1),
                    "speedup_vs_plain200": round(sec_p / sec_l, 3),
                    "error_ratio_learned_vs_plain200": round(err_l / err_p, 4),
+                   "speedup_at_equal_error": round(sec_p * 1e3 / best_eq[0], 3),
+                   "equal_error_basis": "plain 200-probe time over the fastest learned setting whose mean match "
+                                        "error is at most the plain error (%s: %.1f ms, error %.0f)"
+                                        % (best_eq[2], best_eq[0], best_eq[1]),
+                   "learned_ladder": learned_ladder,
+                   "configured_setting": configured,
                    "plain_probe_ladder": ladder,
-                   "speedup_at_equal_error": (round(ladder[-1]["ms"] / (sec_l * 1e3), 2) if reached else None),
-                   "speedup_at_equal_error_lower_bound": round(ladder[-1]["ms"] / (sec_l * 1e3), 2),
-                   "train_s": round(train_s, 1), "train_pairs": int(model.pair_count()),
-                   "train_loss_first_last": [round(float(losses[:20].mean()), 3),
-                                             round(float(losses[-20:].mean()), 3)],
+                   "plain_time_at_learned_error_over_learned": (round(ladder[-1]["ms"] / (sec_l * 1e3), 2)
+                                                                if reached else None),
+                   "plain_time_at_learned_error_over_learned_lower_bound": (
+                       round(ladder[-1]["ms"] / (sec_l * 1e3), 2) if ladder else None),
+                   "train": train,
                    "index_ms": round(rl.learnedIndexMs, 2), "keys": rl.learnedKeys,
                    "clusters": rl.learnedClusters, "propose_ms": round(rl.learnedProposeMs, 2),
                    "fine_search_keys_scored": rl.learnedKeysScored,
                    "synth_kernels_ms": round(rl.synthKernelMs, 2), "passes": rl.passes,
                    "plain_synth_kernels_ms": round(rp.synthKernelMs, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic5, "kernel": "k_synth_wave",
-                     "launches": rl.waveLaunches,
-                     "avg_launch_ms": round(kern_ms / max(rl.waveLaunches, 1), 4)},
+        "roofline": _roofline(rl, "k_synth_seg", "synth_seg_config5_traffic.json",
+                              "synth_seg_config5_counters.json"),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline_config5()
+        out["cpu_baseline"] = cpu_baseline_config5(np, torch, R, W)
     print(json.dumps(out), flush=True)
 
 
@@ -236,9 +325,6 @@ def bench_config4(args, np, torch, R, W):
     elapsed = time.perf_counter() - t0
     rt, _ = R.engine(p, ix, tp, cp, mode=R.MODE_BATCHED, timing=True)
     n_t = rt.nTargets
-    kern_ms = rt.waveKernelMs
-    alg = rt.wavePatchTerms * rt.pixelWordBytes
-    achieved = alg / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
     out = {
         "metric": METRIC, "value": round(n_t * args.steps / elapsed, 1), "unit": "target px/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
@@ -249,27 +335,71 @@ def bench_config4(args, np, torch, R, W):
                    "target_px": n_t, "passes": rt.passes, "mean_match_error": round(float(np.mean(errs)), 1),
                    "synth_kernels_ms": round(rt.synthKernelMs, 2), "prep_ms": round(rt.prepMs, 2),
                    "total_ms": round(rt.totalMs, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_synth_wave",
-                     "launches": rt.waveLaunches,
-                     "avg_launch_ms": round(kern_ms / max(rt.waveLaunches, 1), 4)},
+        "roofline": _roofline(rt, "k_synth_seg"),
     }
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline_config5(threads=8):
-    """Oracle in threaded-reference mode on config 5's generator at 1024x1024 with a centred
-    256x256 hole (the plain method, 200 probes): px/s of a whole engine run."""
+def _cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
-    from resynthesizer_amd import workloads as W
-    img, mask = W.config5(size=1024)
+    return O
+
+
+def cpu_baseline_config5(np, torch, R, W, threads=8):
+    """The CPU oracle in the reference's threaded mode (8 threads, lib/refinerThreaded.h,
+    lib/buildSwitches.h:89) on the whole config-5 heal (the plain method, 200 probes; the
+    reference has no learned proposals): px/s of one engine run.  Beside it, the config-2
+    leg: the oracle on the whole 2048x2048 config-2 heal against the plain GPU batched mode
+    on the same input."""
+    O = _oracle()
+    img, mask = W.config5()
     t = time.time()
     r, _ = O.image_synth(img, mask, 0, O.default_params(), threads=threads)
-    dt = time.time() - t
-    return {"value": r.nTargets / dt, "unit": "target px/s", "cores": threads, "kind": "port",
-            "sample": "config 5 generator at 1024x1024 with a 256x256 hole (%d target px), one whole engine run "
-                      "(plain method), %.1f s, 8-thread reference mode of the CPU oracle" % (r.nTargets, dt)}
+    dt5 = time.time() - t
+    out = {"value": round(r.nTargets / dt5, 1), "unit": "target px/s", "cores": threads, "kind": "port",
+           "sample": "the whole config-5 heal (4096x4096 RGB, 1024x1024 hole, %d target px), plain method, "
+                     "one engine run of %.1f s, 8-thread reference mode of the CPU oracle" % (r.nTargets, dt5)}
+    out.update(_cpu_info())
+    # config-2 leg
+    img2, mask2 = W.config2()
+    t = time.time()
+    r2, _ = O.image_synth(img2, mask2, 1, O.default_params(), threads=threads)
+    dt2 = time.time() - t
+    h, w, c = img2.shape
+    d_src = torch.from_numpy(img2).cuda()
+    d_img = torch.empty_like(d_src)
+    d_mask = torch.from_numpy(mask2).cuda()
+    secs = []
+    for k in range(4):
+        d_img.copy_(d_src)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g = R.image_synth_device(d_img.data_ptr(), w, h, w * c, d_mask.data_ptr(), w, R.T_RGBA, R.default_params(),
+                                 mode=R.MODE_BATCHED, seed=k + 1)
+        torch.cuda.synchronize()
+        assert g.err == 0
+        if k:
+            secs.append(time.perf_counter() - t)
+    gpu2 = g.nTargets / float(np.median(secs))
+    out["config2"] = {"cpu_px_per_s": round(r2.nTargets / dt2, 1), "cpu_s": round(dt2, 1),
+                      "gpu_plain_px_per_s": round(gpu2, 1), "gpu_over_cpu": round(gpu2 / (r2.nTargets / dt2), 1),
+                      "target_px": r2.nTargets}
+    return out
 
 
 def main():
@@ -283,7 +413,11 @@ def main():
     ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--train-textures", type=int, default=200, help="config5: training textures (profiling runs)")
     ap.add_argument("--train-steps", type=int, default=1500, help="config5: Adam steps (profiling runs)")
-    ap.add_argument("--no-ladder", action="store_true", help="config5: skip the plain probe ladder")
+    ap.add_argument("--no-ladder", action="store_true", help="config5: skip the probe ladders and the "
+                    "configured-setting run")
+    ap.add_argument("--save-model", default="", help="config5: write the trained parameters here (.npy)")
+    ap.add_argument("--load-model", default="", help="config5: load parameters from --save-model instead of "
+                    "training (profiling runs)")
     ap.add_argument("--workload", default="config5", choices=["config2", "config3", "config4", "config5"],
                     help="config5 (default): learned proposals vs the plain method on a 4096x4096 heal; "
                          "config2: the plain method on a 2048x2048 heal; config3: 96 uneven heal "
@@ -346,36 +480,21 @@ def main():
     n_t = r.nTargets
     value = n_t * args.steps / elapsed
 
-    # the dominant kernel (k_synth_wave, one wavefront per target): HIP events recorded by
-    # the library around each of its launches on the library's stream, and the patch terms
-    # those launches scored (one corpus pixel word gathered per term)
+    # the dominant kernels (the large-chunk visit kernels): HIP events recorded by the
+    # library around each of their launches on its stream, and the work they tallied
     rt = step(timing=True)
-    launches = max(rt.waveLaunches, 1)
-    kern_ms = rt.waveKernelMs
-    alg_bytes = rt.wavePatchTerms * rt.pixelWordBytes
-    achieved = alg_bytes / (kern_ms / 1e3) / 1e9 if kern_ms > 0 else 0.0
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "synth_wave_config2_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "target px/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u%d" % (8 * rt.pixelWordBytes),
         "data": "synthetic (seeded value noise + grating, 12 seeded ellipses)",
         "config": {"workload": "config 2: plain heal, %dx%d RGBA, %d target px, patch 30, 200 probes, "
                                "sensitivity 0.117, batched mode" % (w, h, n_t),
                    "proposal": "plain (200 random probes)", "target_px": n_t,
                    "passes": rt.passes, "visits": rt.visits, "candidates": rt.candidates,
                    "mean_match_error": round(rt.meanError, 1)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_synth_wave", "launches": launches,
-                     "avg_launch_ms": round(kern_ms / launches, 4),
-                     "algorithmic_bytes_per_launch": int(alg_bytes / launches),
-                     "all_synth_kernels_ms": round(rt.synthKernelMs, 3), "all_synth_launches": rt.launches},
+        "roofline": dict(_roofline(rt, "k_synth_seg", "synth_seg_config2_traffic.json"),
+                         all_synth_kernels_ms=round(rt.synthKernelMs, 3), all_synth_launches=rt.launches),
     }
     if not args.no_quality:
         # the exact serial engine on the same input: the reference's own mean match error

@@ -224,6 +224,9 @@ typedef struct {
   unsigned int waveLaunches;          /* batched: launches of the wavefront-per-target kernel */
   double waveKernelMs;                /* their summed durations (timing != 0) */
   unsigned long long wavePatchTerms;  /* patch terms they scored */
+  unsigned long long waveScanned;     /* patch offsets their neighbour scans read */
+  unsigned long long waveStaged;      /* neighbours they staged for scoring */
+  unsigned long long waveVisits;      /* their target visits */
   unsigned int learnedLaunches;       /* launches whose visits scored learned proposals */
   unsigned int learnedKeys;           /* corpus keys embedded (0: no proposals) */
   unsigned int learnedClusters;       /* inverted-file clusters */

@@ -81,7 +81,9 @@ class ResynthStats(ctypes.Structure):
                 ("totalMs", ctypes.c_double), ("pixelWordBytes", ctypes.c_uint),
                 ("windowLaunches", ctypes.c_uint), ("windowWaitsExpired", ctypes.c_uint),
                 ("waveLaunches", ctypes.c_uint), ("waveKernelMs", ctypes.c_double),
-                ("wavePatchTerms", ctypes.c_ulonglong), ("learnedLaunches", ctypes.c_uint),
+                ("wavePatchTerms", ctypes.c_ulonglong), ("waveScanned", ctypes.c_ulonglong),
+                ("waveStaged", ctypes.c_ulonglong), ("waveVisits", ctypes.c_ulonglong),
+                ("learnedLaunches", ctypes.c_uint),
                 ("learnedKeys", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
                 ("learnedIndexMs", ctypes.c_double), ("learnedProposeMs", ctypes.c_double),
                 ("learnedKeysScored", ctypes.c_ulonglong), ("scratchBytes", ctypes.c_ulonglong)]

@@ -366,7 +366,7 @@ struct Workspace {
       tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, ctxRings, ringTemp, firstIdx, wctl,
       bandStart;
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
-  Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, centIdx, keyCl, bCount, bStart, bCursor,
+  Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, keyCl, bCount, bStart, bCursor,
       bMembers, qX, qH1, qH2, qE, qZ, qCl, props, propsAll, pairPts, pairSrc, superCent, superId, sCount, sStart,
       sCursor, sMembers, subCent, tileStart, qSuper, sortK0, sortV0, sortK1, sortV1, sortTemp, keyZs, keyPtsS, qK0,
       qV0, qK1, qV1, qSortTemp;
@@ -382,6 +382,8 @@ struct DeviceState {
   bool ok = false;
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;                     // learned index builds, beside pass 0
+  hipEvent_t indexReady = nullptr;                // recorded on `side` when an index is built
   Workspace ws;                                   // single-request entries
   std::vector<std::unique_ptr<Workspace>> pool;   // one per request of a multi-request call
   Buf manyDesc, manyCtl, manyGlobal;
@@ -405,6 +407,8 @@ bool ensure_device()
   }
   if (hipGetDevice(&g_dev.device) != hipSuccess ||
       hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&g_dev.side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&g_dev.indexReady, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&g_dev.countsReady, hipEventDisableTiming) != hipSuccess) {
     std::fprintf(stderr, "resynth_mi355x: the HIP device could not be set up\n");
     return false;
@@ -418,7 +422,10 @@ bool ensure_device()
 void recover_after_failure()
 {
   std::lock_guard<std::mutex> lock(g_mutex);
-  if (g_dev.ok) (void)hipStreamSynchronize(g_dev.stream);
+  if (g_dev.ok) {
+    (void)hipStreamSynchronize(g_dev.stream);
+    (void)hipStreamSynchronize(g_dev.side);
+  }
   (void)hipGetLastError();
 }
 
@@ -819,6 +826,7 @@ constexpr int kKeyBlock = 1 << 20;   // corpus keys embedded per block (256 B of
 // by their nearest centroid.
 struct LearnedIndex {
   bool ok = false;
+  bool pending = false;   // built on the side stream; the library stream has not waited yet
   unsigned nK = 0, C = 0, S = 0, Q = 0;
   int2 *keyPts = nullptr, *keyPtsS = nullptr;      // keyPtsS: in cluster order
   uint16_t *keyZ = nullptr, *keyZs = nullptr;      // keyZs: embeddings in cluster order
@@ -854,6 +862,11 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   HIP_TRY(hipStreamSynchronize(s));
   L.nK = nK;
   if (nK < 64) return;
+  // From here the build reads only the corpus words, which synthesis never writes, and
+  // buffers of its own: it runs on the side stream, overlapping pass 0 (whose early chunks
+  // leave most compute units idle), and the first proposal waits for indexReady.  The
+  // synchronization above has completed everything it depends on.
+  s = g_dev.side;
   const float *w = model_weights(M);
   const uint16_t *wt = model_weights_t(M, s);
   L.keyZ = (uint16_t *)W.keyZ.get((size_t)nK * kEmb * 2 + 64);
@@ -869,13 +882,10 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   L.S = S;
   L.Q = Q;
   L.C = S * Q;
-  // super-centroids
-  std::vector<uint32_t> ci(S);
-  for (unsigned c = 0; c < S; c++) ci[c] = (uint32_t)((uint64_t)c * nK / S);
-  uint32_t *dci = (uint32_t *)W.centIdx.get((size_t)S * 4 + 64);
-  HIP_TRY(hipMemcpyAsync(dci, ci.data(), (size_t)S * 4, hipMemcpyHostToDevice, s));
+  // super-centroids: keys c * nK / S, c < S (gathered on the device: a copy from pageable
+  // host memory would hold the host until the side stream reached it)
   L.superCent = (uint16_t *)W.superCent.get((size_t)S * kEmb * 2 + 64);
-  HIP_TRY(gather_rows_bf16(s, L.keyZ, kEmb, dci, (int)S, L.superCent));
+  HIP_TRY(gather_spread_rows_bf16(s, L.keyZ, kEmb, nK, (int)S, L.superCent));
   int32_t *sid = (int32_t *)W.superId.get((size_t)nK * 4 + 64);
   HIP_TRY(learn_topk_cols(s, L.keyZ, (int)nK, L.superCent, (int)S, 1, sid));
   // keys sorted by super-cluster, then by cluster (radix sorts: no atomics on hot counters)
@@ -899,8 +909,17 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   L.keyZs = (uint16_t *)W.keyZs.get((size_t)nK * kEmb * 2 + 64);
   L.keyPtsS = (int2 *)W.keyPtsS.get((size_t)nK * sizeof(int2) + 64);
   HIP_TRY(learn_permute_keys(s, v1, (int)nK, L.keyZ, L.keyPts, L.keyZs, L.keyPtsS));
-  HIP_TRY(hipStreamSynchronize(s));   // the model's weights are read on this stream
+  HIP_TRY(hipEventRecord(g_dev.indexReady, s));
   L.ok = true;
+  L.pending = true;
+}
+
+// Makes the library stream wait for an index still building on the side stream.
+void learned_wait(LearnedIndex &L)
+{
+  if (!L.pending) return;
+  HIP_TRY(hipStreamWaitEvent(g_dev.stream, g_dev.indexReady, 0));
+  L.pending = false;
 }
 
 // Proposals (16 corpus points each) for targets k0..k1-1 from their current neighbourhoods.
@@ -1053,12 +1072,19 @@ int refine_engine(Job &job, Prepared &pr)
   LearnedIndex learned;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> proposeEvents;
   int2 *propsAll = nullptr;
+  std::pair<hipEvent_t, hipEvent_t> indexEvents{nullptr, nullptr};
   if (job.opt.model && !serial) {
     const double tl = now_ms();
+    if (job.opt.timing) {
+      HIP_TRY(hipEventCreate(&indexEvents.first));
+      HIP_TRY(hipEventCreate(&indexEvents.second));
+      HIP_TRY(hipEventRecord(indexEvents.first, s));
+    }
     learned_build(job, pr, learned);
+    if (job.opt.timing) HIP_TRY(hipEventRecord(indexEvents.second, learned.pending ? g_dev.side : s));
     job.st.learnedKeys = learned.nK;
     job.st.learnedClusters = learned.ok ? learned.C : 0;
-    job.st.learnedIndexMs = now_ms() - tl;
+    job.st.learnedIndexMs = now_ms() - tl;   // host time; the device time replaces it when timed
   }
   finish_order(pr);
   unsigned pass;
@@ -1078,6 +1104,7 @@ int refine_engine(Job &job, Prepared &pr)
         HIP_TRY(hipEventRecord(pa, s));
       }
       propsAll = (int2 *)W.propsAll.get((size_t)nT * 16 * sizeof(int2) + 64);
+      learned_wait(learned);
       learned_propose(job, pr, learned, 0, nT, job.opt.learnedClusters == 1 ? 1 : 2, propsAll);
       if (job.opt.timing) {
         HIP_TRY(hipEventRecord(pb, s));
@@ -1166,6 +1193,7 @@ int refine_engine(Job &job, Prepared &pr)
               HIP_TRY(hipEventCreate(&pb));
               HIP_TRY(hipEventRecord(pa, s));
             }
+            learned_wait(learned);
             bl.props = learned_propose(job, pr, learned, k, k1, job.opt.learnedClusters == 1 ? 1 : 2);
             if (job.opt.timing) {
               HIP_TRY(hipEventRecord(pb, s));
@@ -1204,6 +1232,7 @@ int refine_engine(Job &job, Prepared &pr)
     if ((double)((float)betters / (float)nT) < 0.1) break;
     if (!serial && job.opt.maxPasses && passesRun >= job.opt.maxPasses) break;
   }
+  learned_wait(learned);   // an index no pass used still finishes before the call returns
   job.st.passes = passesRun;
   job.st.launches = launches;
   job.st.windowLaunches = windowLaunches;
@@ -1227,6 +1256,9 @@ int refine_engine(Job &job, Prepared &pr)
       job.st.candidates += hc.candidates[i];
       job.st.patchTerms += hc.terms[i];
       job.st.wavePatchTerms += hc.termsWave[i];
+      job.st.waveScanned += hc.scanWave[i];
+      job.st.waveStaged += hc.stageWave[i];
+      job.st.waveVisits += hc.visitsWave[i];
     }
     job.st.learnedKeysScored = hc.keysScored;
     double ms = 0;
@@ -1250,6 +1282,13 @@ int refine_engine(Job &job, Prepared &pr)
       job.st.learnedProposeMs += f;
       HIP_TRY(hipEventDestroy(e.first));
       HIP_TRY(hipEventDestroy(e.second));
+    }
+    if (indexEvents.first) {
+      float f = 0;
+      HIP_TRY(hipEventElapsedTime(&f, indexEvents.first, indexEvents.second));
+      job.st.learnedIndexMs = f;
+      HIP_TRY(hipEventDestroy(indexEvents.first));
+      HIP_TRY(hipEventDestroy(indexEvents.second));
     }
   }
   job.st.synthMs = now_ms() - t1;

@@ -59,7 +59,11 @@ enum { CNT_SLOTS = 64 };
 struct Counters {
   unsigned long long candidates[CNT_SLOTS];
   unsigned long long terms[CNT_SLOTS];
-  unsigned long long termsWave[CNT_SLOTS];   // terms scored by k_synth_wave launches only
+  unsigned long long termsWave[CNT_SLOTS];   // terms scored by the large-chunk (wavefront and
+                                             // segmented) visit kernels only
+  unsigned long long scanWave[CNT_SLOTS];    // offsets their neighbour scans read
+  unsigned long long stageWave[CNT_SLOTS];   // neighbours they staged
+  unsigned long long visitsWave[CNT_SLOTS];  // their visits
   unsigned int betters[CNT_SLOTS];
   unsigned long long keysScored;             // dot products of the learned fine search
 };

@@ -293,7 +293,8 @@ constexpr unsigned kNearTargets = 4096, kNearMax = 64;
 template <typename Word>
 __device__ unsigned gather_neighbours(const Geometry &g, const Word *tword, const int2 *sourceOf, const int2 *offsets,
                                       int tx, int ty, int2 *nbOff, Word *nbPix, int2 *nbSrc,
-                                      const uint8_t *tileHas = nullptr, const uint32_t *firstIdx = nullptr)
+                                      const uint8_t *tileHas = nullptr, const uint32_t *firstIdx = nullptr,
+                                      unsigned *scanned = nullptr)
 {
   const unsigned lane = lane_id();
   const size_t tp = (size_t)ty * (unsigned)g.tw + (unsigned)tx;
@@ -331,6 +332,7 @@ __device__ unsigned gather_neighbours(const Geometry &g, const Word *tword, cons
     }
     unsigned c2 = count + (unsigned)__popcll(m);
     count = c2 < L ? c2 : L;
+    if (scanned) *scanned += 64u;
   }
   wave_sync();
   return count;
@@ -842,11 +844,18 @@ __device__ __forceinline__ void put(T *p, T v)
 }
 
 // One batched visit of target k by one wavefront (lib/synthesize.h:426-642).
+// Per-wave tallies of the large-chunk visit kernels, kept in LDS: candidates, patch terms,
+// betters, and the visits' offsets read, neighbours staged and count.
+struct VisitTally {
+  unsigned long long eval, term, scan, stage, visits;
+  unsigned bet;
+};
+
 template <typename Word, int G0, int GL, bool PUBLISH, int NC = 0>
 __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers &b, unsigned k, unsigned seed,
                                            unsigned pass, int2 *sOff, Word *sPix, int2 *sSrc, const uint16_t *sT,
                                            const uint32_t *sTm, unsigned long long &myEval,
-                                           unsigned long long &myTerm, unsigned &myBet)
+                                           unsigned long long &myTerm, unsigned &myBet, VisitTally *tally = nullptr)
 {
   const unsigned lane = lane_id();
   Word *tword = (Word *)b.tword;
@@ -857,8 +866,14 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   // the neighbour scan
   int2 prop = make_int2(-1, -1);
   if (b.props && lane < 16) prop = b.props[(size_t)(k - b.propsK0) * 16 + lane];
-  const unsigned count =
-      gather_neighbours<Word>(g, tword, b.sourceOf, b.offsets, t.x, t.y, sOff, sPix, sSrc, b.tileHas, b.firstIdx);
+  unsigned scanned = 0;
+  const unsigned count = gather_neighbours<Word>(g, tword, b.sourceOf, b.offsets, t.x, t.y, sOff, sPix, sSrc,
+                                                 b.tileHas, b.firstIdx, &scanned);
+  if (tally && lane == 0) {
+    tally->scan += scanned;
+    tally->stage += count;
+    tally->visits++;
+  }
   uint32_t best = 0xFFFFFFFFu;
   int winX = 0, winY = 0;
   bool perfect = false, scored = false;
@@ -1030,7 +1045,7 @@ template <typename Word, int LPT>
 __device__ unsigned gather_neighbours_seg(const Seg<LPT> &S, const Geometry &g, const Word *tword,
                                           const int2 *sourceOf, const int2 *offsets, int tx, int ty, bool active,
                                           int2 *nbOff, Word *nbPix, int2 *nbSrc, const uint8_t *tileHas,
-                                          const uint32_t *firstIdx)
+                                          const uint32_t *firstIdx, unsigned &scanned)
 {
   const size_t tp = (size_t)ty * (unsigned)g.tw + (unsigned)tx;
   if (active && S.sl == 0) {
@@ -1070,6 +1085,7 @@ __device__ unsigned gather_neighbours_seg(const Seg<LPT> &S, const Geometry &g, 
       const unsigned c2 = count + (unsigned)__popcll(m);
       count = c2 < L ? c2 : L;
       base += LPT;
+      scanned += LPT;
     }
   }
   wave_sync();
@@ -1085,8 +1101,7 @@ __device__ unsigned gather_neighbours_seg(const Seg<LPT> &S, const Geometry &g, 
 template <typename Word, int LPT, int G0, int GL, bool PUBLISH, int NC = 0>
 __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers &b, unsigned kb, unsigned k1,
                                           unsigned seed, unsigned pass, int2 *sOff, Word *sPix, int2 *sSrc,
-                                          const uint16_t *sT, const uint32_t *sTm, unsigned long long &myEval,
-                                          unsigned long long &myTerm, unsigned &myBet)
+                                          const uint16_t *sT, const uint32_t *sTm, VisitTally *tally)
 {
   const Seg<LPT> S;
   Word *tword = (Word *)b.tword;
@@ -1098,8 +1113,9 @@ __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers 
   // the proposals do not depend on the neighbourhood: their load overlaps the scan
   int2 prop = make_int2(-1, -1);
   if (b.props && active && S.sl < 16) prop = b.props[(size_t)(k - b.propsK0) * 16 + S.sl];
+  unsigned scanned = 0;
   const unsigned count = gather_neighbours_seg<Word, LPT>(S, g, tword, b.sourceOf, b.offsets, t.x, t.y, active, sOff,
-                                                          sPix, sSrc, b.tileHas, b.firstIdx);
+                                                          sPix, sSrc, b.tileHas, b.firstIdx, scanned);
   uint32_t best = 0xFFFFFFFFu;
   int winX = 0, winY = 0;
   bool perfect = !active, scored = false;
@@ -1258,14 +1274,54 @@ __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers 
         nw = (old & ~cmask) | (src & cmask) | (Word)1;
         const uint64_t packed = (uint64_t)(uint32_t)winX | ((uint64_t)(uint32_t)winY << 32);
         put<PUBLISH>((uint64_t *)&b.sourceOf[tp], packed);
-        myBet++;
+        atomicAdd(&tally->bet, 1u);
       }
     }
     put<PUBLISH>(&tword[tp], nw);
     b.lastBest[tp] = best;
     if (g.skipEmpty) b.tileHas[(size_t)(t.y / kTile) * g.tilesW + t.x / kTile] = 1;
-    myEval += evals;
-    myTerm += (unsigned long long)evals * count;
+    // segment leaders of one wave add at once: LDS atomics
+    atomicAdd(&tally->eval, (unsigned long long)evals);
+    atomicAdd(&tally->term, (unsigned long long)evals * count);
+    atomicAdd(&tally->scan, (unsigned long long)scanned);
+    atomicAdd(&tally->stage, (unsigned long long)count);
+    atomicAdd(&tally->visits, 1ull);
+  }
+}
+
+// The large-chunk kernels' tallies: one LDS record per wave, cleared before the block's
+// first barrier, summed over the block's waves at the end, then one atomic per counter
+// into the block's slot.
+template <int WAVES>
+__device__ __forceinline__ void tally_clear(VisitTally *sT)
+{
+  if (threadIdx.x < WAVES) sT[threadIdx.x] = VisitTally{};
+}
+
+template <int WAVES>
+__device__ __forceinline__ void flush_tally(const SynthBuffers &b, const VisitTally *sT)
+{
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    VisitTally a = {};
+    for (int w = 0; w < WAVES; w++) {
+      a.eval += sT[w].eval;
+      a.term += sT[w].term;
+      a.scan += sT[w].scan;
+      a.stage += sT[w].stage;
+      a.visits += sT[w].visits;
+      a.bet += sT[w].bet;
+    }
+    const unsigned slot = blockIdx.x & (CNT_SLOTS - 1);
+    if (a.eval) atomicAdd(&b.counters->candidates[slot], a.eval);
+    if (a.term) {
+      atomicAdd(&b.counters->terms[slot], a.term);
+      atomicAdd(&b.counters->termsWave[slot], a.term);
+    }
+    if (a.scan) atomicAdd(&b.counters->scanWave[slot], a.scan);
+    if (a.stage) atomicAdd(&b.counters->stageWave[slot], a.stage);
+    if (a.visits) atomicAdd(&b.counters->visitsWave[slot], a.visits);
+    if (a.bet) atomicAdd(&b.counters->betters[slot], a.bet);
   }
 }
 
@@ -1310,8 +1366,8 @@ void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned
   __shared__ int2 sOff[WAVES][64];
   __shared__ Word sPix[WAVES][64];
   __shared__ int2 sSrc[WAVES][64];
-  __shared__ unsigned long long sEval[WAVES], sTerm[WAVES];
-  __shared__ unsigned sBet[WAVES];
+  __shared__ VisitTally sTally[WAVES];
+  tally_clear<WAVES>(sTally);
   for (int i = threadIdx.x; i < 512; i += blockDim.x) {
     sT[i] = b.tables->T[i];
     sTm[i] = b.tables->Tm[i];
@@ -1322,8 +1378,13 @@ void k_synth_wave(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned
   unsigned myBet = 0;
   for (unsigned k = k0 + blockIdx.x * WAVES + wave; k < k1; k += gridDim.x * WAVES)
     visit_wave<Word, G0, GL, false, NC>(g, b, k, seed, pass, sOff[wave], sPix[wave], sSrc[wave], sT, sTm, myEval,
-                                        myTerm, myBet);
-  flush_counters<Word, WAVES, true>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
+                                        myTerm, myBet, &sTally[wave]);
+  if (lane_id() == 0) {
+    sTally[wave].eval = myEval;
+    sTally[wave].term = myTerm;
+    sTally[wave].bet = myBet;
+  }
+  flush_tally<WAVES>(b, sTally);
 }
 
 // Batched visits, 64/LPT targets per wavefront (segmented visits; see visit_seg).
@@ -1337,8 +1398,8 @@ void k_synth_seg(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned 
   __shared__ int2 sOff[WAVES][TPW][64];
   __shared__ Word sPix[WAVES][TPW][64];
   __shared__ int2 sSrc[WAVES][TPW][64];
-  __shared__ unsigned long long sEval[WAVES], sTerm[WAVES];
-  __shared__ unsigned sBet[WAVES];
+  __shared__ VisitTally sTally[WAVES];
+  tally_clear<WAVES>(sTally);
   for (int i = threadIdx.x; i < 512; i += blockDim.x) {
     sT[i] = b.tables->T[i];
     sTm[i] = b.tables->Tm[i];
@@ -1346,18 +1407,10 @@ void k_synth_seg(Geometry g, SynthBuffers b, unsigned k0, unsigned k1, unsigned 
   __syncthreads();
   const int wave = threadIdx.x >> 6;
   const unsigned seg = lane_id() / LPT;
-  unsigned long long myEval = 0, myTerm = 0;
-  unsigned myBet = 0;
   for (unsigned kb = k0 + (blockIdx.x * WAVES + wave) * TPW; kb < k1; kb += gridDim.x * WAVES * TPW)
     visit_seg<Word, LPT, G0, GL, false, NC>(g, b, kb, k1, seed, pass, sOff[wave][seg], sPix[wave][seg],
-                                            sSrc[wave][seg], sT, sTm, myEval, myTerm, myBet);
-  // each segment's first lane holds its counts
-  for (int o = 32; o > 0; o >>= 1) {
-    myEval += __shfl_xor(myEval, o);
-    myTerm += __shfl_xor(myTerm, o);
-    myBet += __shfl_xor(myBet, o);
-  }
-  flush_counters<Word, WAVES, true>(b, myEval, myTerm, myBet, sEval, sTerm, sBet);
+                                            sSrc[wave][seg], sT, sTm, &sTally[wave]);
+  flush_tally<WAVES>(b, sTally);
 }
 
 // Sliding-window batched visits in one persistent launch.  Waves take targets in

@@ -46,6 +46,7 @@ hipError_t transpose_weights(hipStream_t s, const float *w, uint16_t *wt);
 hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, const uint16_t *x, int n, uint16_t *zb,
                            float *z);
 hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst);
+hipError_t gather_spread_rows_bf16(hipStream_t s, const uint16_t *src, int cols, uint32_t nSrc, int n, uint16_t *dst);
 hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uint64_t key);
 
 // ---- the model object (resynth_learn_host.cpp)

@@ -425,6 +425,26 @@ hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const 
   return hipGetLastError();
 }
 
+// dst row i = src row floor(i * nSrc / n): n rows spread evenly over the nSrc rows.
+__global__ void k_gather_spread_rows(const uint16_t *src, int cols, uint32_t nSrc, int n, uint16_t *dst)
+{
+  const int per = cols / 8;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * per) return;
+  const int i = (int)(t / per), c = (int)(t % per);
+  const uint32_t r = (uint32_t)((uint64_t)i * nSrc / (uint32_t)n);
+  ((uint4 *)(dst + (size_t)i * cols))[c] = ((const uint4 *)(src + (size_t)r * cols))[c];
+}
+
+hipError_t gather_spread_rows_bf16(hipStream_t s, const uint16_t *src, int cols, uint32_t nSrc, int n, uint16_t *dst)
+{
+  if (n <= 0) return hipSuccess;
+  const size_t work = (size_t)n * (cols / 8);
+  hipLaunchKernelGGL(k_gather_spread_rows, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, src, cols, nSrc, n,
+                     dst);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ uint64_t lmix64(uint64_t z)
 {
   z += 0x9E3779B97F4A7C15ull;
