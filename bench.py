@@ -118,18 +118,19 @@ def bench_config3(args, np, torch, R, W):
 
 # Learned-mode settings of the config-5 line: proposals of every target computed once at
 # the start of pass 1 from the pass-0 result (1 inverted-file cluster searched, top 16),
-# 1 random probe beside them, 16 random probes in pass 0 whose chunks grow by 30% of the
-# visited targets, 2 passes.
+# 1 random probe beside them, 16 random probes in pass 0, whose first chunks hold 256
+# targets and whose chunks then grow by 30% of the visited targets, 2 passes.
 LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
-               ramp_permille=300)
+               ramp_permille=300, first_chunk=256)
 # The configured learned setting: proposals for every chunk from its targets' current
 # neighbourhoods (top 16 of 2 inverted-file clusters) beside 32 random probes, in every pass
 # of the refiner's own schedule (up to 6 passes, 10% stop rule).
 CONFIGURED = dict(learned_probes=32)
 # Cheaper learned settings (fewer random probes in pass 0, and in pass 1 beside the
 # proposals), searched for the fastest one whose error still reaches the plain method's.
-LEARNED_LADDER = [dict(LEARNED, learned_probes_later=8), dict(LEARNED, learned_probes_later=4),
-                  dict(LEARNED, learned_probes_later=2), dict(LEARNED, learned_probes_later=1, learned_probes=0)]
+LEARNED_LADDER = [dict(LEARNED, first_chunk=1024), dict(LEARNED, first_chunk=1024, learned_probes_later=4),
+                  dict(LEARNED, first_chunk=4096, learned_probes_later=4),
+                  dict(LEARNED, first_chunk=4096, learned_probes_later=1, learned_probes=0)]
 # bytes priced per unit of visit-kernel work: a corpus pixel word per patch term of a
 # scored candidate (priced at its word width), 9 B per patch offset the neighbour scan
 # reads (an 8 B offset and the target's presence byte), 12 B per staged neighbour (offset,

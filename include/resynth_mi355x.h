@@ -203,6 +203,10 @@ typedef struct {
                                    visited with segments of that many lanes; 1 = every chunk by
                                    one wavefront per target.  A target's result does not depend on
                                    it: the choice trades issue slots against latency. */
+  unsigned int firstChunk;      /* batched mode: targets in each of pass 0's first chunks (0 = automatic:
+                                   nT/2048 clamped to [1, 64]).  Larger first chunks take fewer, fuller
+                                   launches at the start of pass 0, where every visit scans far, at some
+                                   cost in match error (their visits do not see each other). */
 } ResynthOptions;
 
 typedef struct {

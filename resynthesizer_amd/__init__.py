@@ -69,7 +69,8 @@ class ResynthOptions(ctypes.Structure):
                 ("timing", ctypes.c_int), ("model", ctypes.c_void_p), ("learnedProbes", ctypes.c_uint),
                 ("learnedPasses", ctypes.c_uint), ("learnedClusters", ctypes.c_uint),
                 ("learnedProbesLater", ctypes.c_uint), ("learnedRefresh", ctypes.c_uint),
-                ("maxPasses", ctypes.c_uint), ("rampPermille", ctypes.c_uint), ("lanesPerTarget", ctypes.c_uint)]
+                ("maxPasses", ctypes.c_uint), ("rampPermille", ctypes.c_uint), ("lanesPerTarget", ctypes.c_uint),
+                ("firstChunk", ctypes.c_uint)]
 
 
 class ResynthStats(ctypes.Structure):
@@ -248,10 +249,10 @@ def params_from_dict(d):
 
 def options(mode=MODE_BATCHED, prng=PRNG_GRAND, seed=0, batch=0, timing=False, model=None, learned_probes=0,
             learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0,
-            ramp_permille=0, lanes_per_target=0):
+            ramp_permille=0, lanes_per_target=0, first_chunk=0):
     return ResynthOptions(mode, prng, seed, batch, int(bool(timing)), model.h if model is not None else None,
                           learned_probes, learned_passes, learned_clusters, learned_probes_later, learned_refresh,
-                          max_passes, ramp_permille, lanes_per_target)
+                          max_passes, ramp_permille, lanes_per_target, first_chunk)
 
 
 def format_indices(fmt):
@@ -282,7 +283,7 @@ class Result:
 def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAND, cancel=0, mask2=None,
                 width=None, mask_width=None, mask2_width=None, seed=0, batch=0, timing=False, model=None,
                 learned_probes=0, learned_passes=0, learned_clusters=0, learned_probes_later=0, learned_refresh=0,
-                max_passes=0, ramp_permille=0, lanes_per_target=0):
+                max_passes=0, ramp_permille=0, lanes_per_target=0, first_chunk=0):
     """imageSynth / imageSynth2 (lib/imageSynth.c:62-217) on the GPU.
 
     image: uint8 (H, W, C) or a padded row buffer (H, rowBytes) with `width` given;
@@ -310,7 +311,7 @@ def image_synth(image, mask, fmt, params=None, mode=MODE_BATCHED, prng=PRNG_GRAN
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
     opt = options(mode, prng, seed, batch, timing, model, learned_probes, learned_passes, learned_clusters,
-                  learned_probes_later, learned_refresh, max_passes, ramp_permille, lanes_per_target)
+                  learned_probes_later, learned_refresh, max_passes, ramp_permille, lanes_per_target, first_chunk)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().imageSynthEx(ctypes.byref(ib), ctypes.byref(mb), mb2p, fmt, pp, cb, None, ctypes.byref(cf),
                              ctypes.byref(opt), ctypes.byref(st))
@@ -341,7 +342,7 @@ def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes
                        mode=MODE_BATCHED, prng=PRNG_GRAND, d_mask2=None, mask2_row_bytes=0, seed=0, batch=0,
                        timing=False, cancel=0, model=None, learned_probes=0, learned_passes=0,
                        learned_clusters=0, learned_probes_later=0, learned_refresh=0, max_passes=0,
-                       ramp_permille=0, lanes_per_target=0):
+                       ramp_permille=0, lanes_per_target=0, first_chunk=0):
     """imageSynth on device-resident buffers (integer device addresses, e.g. torch data_ptr()).
     Synthesizes d_image in place.  Returns a Result."""
     log = []
@@ -349,7 +350,7 @@ def image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes
     cf = ctypes.c_int(cancel)
     st = ResynthStats()
     opt = options(mode, prng, seed, batch, timing, model, learned_probes, learned_passes, learned_clusters,
-                  learned_probes_later, learned_refresh, max_passes, ramp_permille, lanes_per_target)
+                  learned_probes_later, learned_refresh, max_passes, ramp_permille, lanes_per_target, first_chunk)
     pp = ctypes.byref(params) if params is not None else None
     err = lib().resynth_image_synth_device(d_image, width, height, row_bytes, d_mask, mask_row_bytes, d_mask2,
                                            mask2_row_bytes, fmt, pp, cb, None, ctypes.byref(cf), ctypes.byref(opt),

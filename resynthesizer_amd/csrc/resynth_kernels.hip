@@ -2431,6 +2431,28 @@ hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *
   return hipGetLastError();
 }
 
+// Sort keys of the batched mode's random target order: a counter-based hash of (seed, i)
+// per row-major target i; sorted (stably, all 32 bits) with the indices as values they give
+// a uniformly random permutation, drawn on the device.
+__global__ void k_order_keys(uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
+{
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = ((uint64_t)seed << 32 | i) + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  keys[i] = (uint32_t)(z >> 32);
+  vals[i] = i;
+}
+
+hipError_t launch_order_keys(hipStream_t s, uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
+{
+  if (!n) return hipSuccess;
+  hipLaunchKernelGGL(k_order_keys, dim3((n + 255) / 256), dim3(256), 0, s, seed, n, keys, vals);
+  return hipGetLastError();
+}
+
 // lib/adaptSimple.h:111-168 writes every pixel back; only target pixels can differ, so
 // only their colour channels are written (alpha and context are left as they are).
 template <typename Word>

@@ -530,27 +530,32 @@ def test_batch_of_uneven_requests():
     masks += [masks[0][:-1], masks[1], np.zeros_like(masks[2])]
     fmts += [fmts[0], 9, fmts[2]]
     params += [params[0], params[1], params[2]]
-    rc, res, st, outs = R.image_synth_batch(imgs, masks, fmts, params, seed=1)
-    assert rc == 0
     n = len(imgs) - 3
-    assert [r["error"] for r in res[n:]] == [R.IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH,
-                                           R.IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT,
-                                           R.IMAGE_SYNTH_ERROR_EMPTY_TARGET]
-    assert st["requests"] == n and st["waitsExpired"] == 0 and st["graphLaunches"] == 1
-    ratios = []
-    for i in range(n):
-        r = res[i]
-        assert r["error"] == 0 and r["passes"] >= 1
-        sel = masks[i] != 0
-        assert r["nTargets"] == int(sel.sum())
-        out, img = outs[i], imgs[i]
-        assert np.array_equal(out[~sel], img[~sel])
-        ncol = 1 if fmts[i] == R.T_Gray else 3
-        corpus = {tuple(c) for c in img[~sel].reshape(-1, img.shape[-1])[:, :ncol]}
-        assert all(tuple(c) in corpus for c in out[sel].reshape(-1, img.shape[-1])[::3, :ncol])
-        single = np.mean([R.image_synth(img, masks[i], fmts[i], params[i], mode=R.MODE_BATCHED, seed=s)[0].meanError
-                          for s in (1, 2)])
-        ratios.append(r["meanError"] / single)
+    seeds = (1, 2, 3)
+    multi = np.zeros((len(seeds), n))
+    for si, seed in enumerate(seeds):
+        rc, res, st, outs = R.image_synth_batch(imgs, masks, fmts, params, seed=seed)
+        assert rc == 0
+        assert [r["error"] for r in res[n:]] == [R.IMAGE_SYNTH_ERROR_IMAGE_MASK_MISMATCH,
+                                               R.IMAGE_SYNTH_ERROR_INVALID_IMAGE_FORMAT,
+                                               R.IMAGE_SYNTH_ERROR_EMPTY_TARGET]
+        assert st["requests"] == n and st["waitsExpired"] == 0 and st["graphLaunches"] == 1
+        for i in range(n):
+            r = res[i]
+            assert r["error"] == 0 and r["passes"] >= 1
+            sel = masks[i] != 0
+            assert r["nTargets"] == int(sel.sum())
+            out, img = outs[i], imgs[i]
+            assert np.array_equal(out[~sel], img[~sel])
+            ncol = 1 if fmts[i] == R.T_Gray else 3
+            corpus = {tuple(c) for c in img[~sel].reshape(-1, img.shape[-1])[:, :ncol]}
+            assert all(tuple(c) in corpus for c in out[sel].reshape(-1, img.shape[-1])[::3, :ncol])
+            multi[si, i] = r["meanError"]
+    # the same requests one by one, over the same probe seeds (tiny requests: the error of
+    # one run spreads by +-15% from seed to seed)
+    single = np.array([[R.image_synth(imgs[i], masks[i], fmts[i], params[i], mode=R.MODE_BATCHED, seed=s)[0].meanError
+                        for i in range(n)] for s in seeds])
+    ratios = multi.mean(0) / single.mean(0)
     print("batch vs single mean error ratios", np.round(ratios, 3))
     assert np.mean(ratios) <= 1.05, ratios
 

@@ -65,13 +65,14 @@ def test_noalloc_matches_device_entry_on_consecutive_calls(mode, batch):
         assert np.array_equal(o1[~sel], img[~sel])
 
 
-def test_noalloc_largest_image_fits_its_scratch():
+@pytest.mark.parametrize("order", [1, 2])
+def test_noalloc_largest_image_fits_its_scratch(order):
     """1024 x 1024 RGBA (the maximum), nearly all of it a target (the largest target and
-    ordering buffers), inward order: it runs, and its scratch leaves room for the serial
-    mode's 4 MiB PRNG stream."""
+    ordering buffers), random order (sorted on the device) and inward order: it runs, and
+    its scratch leaves room for the serial mode's 4 MiB PRNG stream."""
     h = w = 1024
     img, mask = _case(h, w, R.T_RGBA, 5, (16, 16, h - 16, w - 16))
-    params = R.default_params(maxProbeCount=8, patchSize=10, matchContextType=2)
+    params = R.default_params(maxProbeCount=8, patchSize=10, matchContextType=order)
     opt = R.options(mode=R.MODE_BATCHED, max_passes=1)
     print("no-alloc largest image", flush=True)
     err, st, out = _call(R.lib().resynth_image_synth_device_noalloc, img, mask, R.T_RGBA, params, opt)
