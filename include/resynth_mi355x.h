@@ -443,6 +443,75 @@ int resynth_model_fit(ResynthModel *model, unsigned int steps, unsigned int batc
  * a binding check its mirror of these structs. */
 unsigned int resynth_struct_sizes(size_t *out, unsigned int n);
 
+/* ======================================================================= Part 3 */
+
+/* The GIMP plug-in's engine call and the heal-selection helper, with GIMP's drawables
+ * passed as plain buffers.  A GIMP build of the plug-in keeps its libgimp glue (PDB
+ * registration, pixel regions, selection bounds, progress bar) and calls these in place of
+ * its adapter and engine code.  Progress and cancel stay on the calling thread. */
+
+/* Error of the plug-in's own checks (the plug-in returns a PDB execution error and a
+ * message); *message then holds the reference's text. */
+#define RESYNTH_ERROR_PLUGIN 102
+
+/* A drawable as the plug-in reads it (src/resynthesizer/adaptGimp.h:200-322): colour
+ * channels then an optional alpha channel, rows of rowBytes; and the image selection over
+ * the drawable, one byte per pixel in rows of selectionRowBytes.  A NULL or all-zero
+ * selection is no selection intersecting the drawable: the whole drawable is selected
+ * (fetch_mask, src/resynthesizer/adaptGimp.h:178-256). */
+typedef struct {
+  unsigned char *pixels;
+  unsigned int width;
+  unsigned int height;
+  size_t rowBytes;
+  unsigned int colorChannels;   /* 3: RGB, 1: gray (count_color_channels) */
+  int hasAlpha;                 /* gimp_drawable_has_alpha */
+  const unsigned char *selection;
+  size_t selectionRowBytes;
+} ResynthDrawable;
+
+/* src/resynth-parameters.h:28-42.  The drawable ids are not read: the drawables are passed
+ * to resynth_plugin_run themselves. */
+typedef struct GIMPAdapterParametersStruct {
+  int h_tile;
+  int v_tile;
+  int use_border;
+  int corpus_id, input_map_id, output_map_id;
+  double map_weight;
+  double autism;
+  int neighbours;
+  int trys;
+} TGimpAdapterParameters;
+
+/* The engine plug-in's run() (src/resynthesizer/resynthesizer.c:286-546) from its checks
+ * on: image mode and base types, maps (used only when both are given; same mode, sizes of
+ * the corpus and of the drawable), neighbours clamped to 64, format indices, the target
+ * and corpus pixmaps with their masks and map channels (fetch_image_mask_map), engine()
+ * with the caller's progress callback and cancel flag, and the result written back into the
+ * drawable's pixels (colour and alpha channels; alpha is never changed).  corpus NULL:
+ * the drawable itself.  engine() runs in the default mode (resynth_set_default_mode).
+ * Returns 0, a TImageSynthError of engine(), RESYNTH_ERROR_PLUGIN, or a Part 2 code;
+ * message (may be NULL) receives the plug-in's message for a failed check or an empty
+ * target or corpus, else NULL. */
+int resynth_plugin_run(const TGimpAdapterParameters *parameters, ResynthDrawable *drawable,
+                       const ResynthDrawable *corpus, const ResynthDrawable *inputMap,
+                       const ResynthDrawable *outputMap, void (*progressCallback)(int, void *),
+                       void *contextInfo, int *cancelFlag, const char **message);
+
+/* Heal selection (PluginScripts/plugin-heal-selection.py:36-152): the drawable's selection
+ * is synthesized from a band around it.  The selection is grown by samplingRadius pixels
+ * as GIMP grows selections (a disc), the original selection is subtracted, and the copy of
+ * the drawable that serves as corpus is cropped to that band -- all of it (direction 0),
+ * the rows of the selection (1: sides) or its columns (2: above and below).  The filling
+ * order is random (order 0), inwards (1) or outwards (2), then the engine plug-in runs with
+ * the script's parameters (no tiling, no maps, sensitivity 0.117, 16 neighbours, 500
+ * probes).  Returns as resynth_plugin_run; an empty selection returns RESYNTH_ERROR_PLUGIN
+ * ("You must first select a region to heal."), a direction or order out of range
+ * RESYNTH_ERROR_INVALID_ARGUMENT. */
+int resynth_heal_selection(ResynthDrawable *drawable, int samplingRadius, int direction, int order,
+                           void (*progressCallback)(int, void *), void *contextInfo, int *cancelFlag,
+                           const char **message);
+
 /* Library/device information: returns 0 when a gfx950 device is usable. */
 int resynth_device_info(char *name, size_t nameLen, int *computeUnits);
 const char *resynth_version(void);

@@ -27,6 +27,7 @@ IMAGE_SYNTH_ERROR_EMPTY_CORPUS = 6
 RESYNTH_ERROR_INVALID_ARGUMENT = -1
 RESYNTH_ERROR_DEVICE = 100
 RESYNTH_ERROR_TOO_LARGE = 101
+RESYNTH_ERROR_PLUGIN = 102
 # limits of resynth_image_synth_device_noalloc
 NOALLOC_MAX_PIXELS = 1 << 20
 NOALLOC_SCRATCH_BYTES = 192 << 20
@@ -94,6 +95,19 @@ class ResynthStats(ctypes.Structure):
         d["passSize"] = list(self.passSize)[: self.passes]
         d["betters"] = list(self.betters)[: self.passes]
         return d
+
+
+class ResynthDrawable(ctypes.Structure):
+    _fields_ = [("pixels", ctypes.c_void_p), ("width", ctypes.c_uint), ("height", ctypes.c_uint),
+                ("rowBytes", ctypes.c_size_t), ("colorChannels", ctypes.c_uint), ("hasAlpha", ctypes.c_int),
+                ("selection", ctypes.c_void_p), ("selectionRowBytes", ctypes.c_size_t)]
+
+
+class TGimpAdapterParameters(ctypes.Structure):
+    _fields_ = [("h_tile", ctypes.c_int), ("v_tile", ctypes.c_int), ("use_border", ctypes.c_int),
+                ("corpus_id", ctypes.c_int), ("input_map_id", ctypes.c_int), ("output_map_id", ctypes.c_int),
+                ("map_weight", ctypes.c_double), ("autism", ctypes.c_double), ("neighbours", ctypes.c_int),
+                ("trys", ctypes.c_int)]
 
 
 class ResynthRequest(ctypes.Structure):
@@ -203,6 +217,11 @@ SIGNATURES = {
     "resynth_order_targets": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint]),
     "resynth_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, _P(ctypes.c_int)]),
     "resynth_version": (ctypes.c_char_p, []),
+    "resynth_plugin_run": (ctypes.c_int, [_P(TGimpAdapterParameters), _P(ResynthDrawable), _P(ResynthDrawable),
+                                          _P(ResynthDrawable), _P(ResynthDrawable), PROGRESS, ctypes.c_void_p,
+                                          _P(ctypes.c_int), _P(ctypes.c_char_p)]),
+    "resynth_heal_selection": (ctypes.c_int, [_P(ResynthDrawable), ctypes.c_int, ctypes.c_int, ctypes.c_int, PROGRESS,
+                                              ctypes.c_void_p, _P(ctypes.c_int), _P(ctypes.c_char_p)]),
 }
 
 _lib = None
@@ -617,3 +636,66 @@ def device_info():
     cus = ctypes.c_int(0)
     rc = lib().resynth_device_info(name, 64, ctypes.byref(cus))
     return rc, name.value.decode(), cus.value
+
+
+# ---------------------------------------------------------------- GIMP plug-in shim, heal selection
+
+def _drawable(pixels, selection=None):
+    """ResynthDrawable over uint8 (H, W, C) pixels, C = 1 gray, 2 gray + alpha, 3 RGB, 4 RGBA,
+    and an optional uint8 (H, W) selection.  Returns (struct, keep-alive list)."""
+    if pixels.ndim == 2:
+        pixels = pixels[..., None]
+    h, w, c = pixels.shape
+    if c not in (1, 2, 3, 4):
+        raise ValueError("drawables have 1 to 4 channels")
+    d = ResynthDrawable(pixels.ctypes.data, w, h, pixels.strides[0], 3 if c >= 3 else 1, int(c in (2, 4)), None, 0)
+    keep = [pixels]
+    if selection is not None:
+        m = np.ascontiguousarray(selection, dtype=np.uint8)
+        if m.shape != (h, w):
+            raise ValueError("the selection must cover the drawable")
+        d.selection, d.selectionRowBytes = m.ctypes.data, m.strides[0]
+        keep.append(m)
+    return d, keep
+
+
+def plugin_run(image, selection=None, corpus=None, corpus_selection=None, input_map=None, output_map=None,
+               h_tile=0, v_tile=0, use_border=1, map_weight=0.5, autism=0.117, neighbours=30, trys=200, cancel=0):
+    """The engine plug-in (resynth_plugin_run) on uint8 (H, W, C) drawables.  corpus None: the
+    image itself.  Returns (error, message, image_out, progress)."""
+    img = np.array(image, dtype=np.uint8, copy=True, order="C")
+    d, keep = _drawable(img, selection)
+    cp = None
+    if corpus is not None:
+        c, k = _drawable(np.ascontiguousarray(corpus, dtype=np.uint8), corpus_selection)
+        cp, keep = ctypes.byref(c), keep + k
+    maps = []
+    for mimg in (input_map, output_map):
+        if mimg is None:
+            maps.append(None)
+        else:
+            md, k = _drawable(np.ascontiguousarray(mimg, dtype=np.uint8))
+            maps.append(ctypes.byref(md))
+            keep += k
+    p = TGimpAdapterParameters(h_tile, v_tile, use_border, -1, -1, -1, map_weight, autism, neighbours, trys)
+    log = []
+    cb = PROGRESS(lambda pct, ctx: log.append(pct))
+    cf = ctypes.c_int(cancel)
+    msg = ctypes.c_char_p()
+    err = lib().resynth_plugin_run(ctypes.byref(p), ctypes.byref(d), cp, maps[0], maps[1], cb, None,
+                                   ctypes.byref(cf), ctypes.byref(msg))
+    return err, (msg.value.decode() if msg.value else None), img.reshape(np.shape(image)), log
+
+
+def heal_selection(image, selection, sampling_radius=50, direction=0, order=0, cancel=0):
+    """Heal selection (resynth_heal_selection) on a uint8 (H, W, C) drawable and its uint8
+    (H, W) selection.  Returns (error, message, image_out, progress)."""
+    img = np.array(image, dtype=np.uint8, copy=True, order="C")
+    d, keep = _drawable(img, selection)
+    log = []
+    cb = PROGRESS(lambda pct, ctx: log.append(pct))
+    cf = ctypes.c_int(cancel)
+    msg = ctypes.c_char_p()
+    err = lib().resynth_heal_selection(ctypes.byref(d), sampling_radius, direction, order, cb, None, ctypes.byref(cf),
+                                       ctypes.byref(msg))
+    return err, (msg.value.decode() if msg.value else None), img.reshape(np.shape(image)), log

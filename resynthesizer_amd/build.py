@@ -19,7 +19,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-
           "-ffp-contract=off", "-I" + os.path.join(HERE, "..", "include")]
 
 SOURCES = ["resynth_kernels.hip", "resynth_host.cpp", "resynth_learn.hip", "resynth_learn_host.cpp",
-           "resynth_order.hip"]
+           "resynth_order.hip", "resynth_plugin.cpp"]
 
 
 def _compile(src):

@@ -192,3 +192,30 @@ def test_extension_struct_layouts_match_library():
             ctypes.sizeof(R.ResynthKernelConfig), ctypes.sizeof(R.ResynthBatchStats),
             R.ResynthRequest.params.offset, R.ResynthRequest.error.offset, R.ResynthBatchStats.prepMs.offset]
     assert got == want
+
+
+def test_plugin_shim_checks_and_messages():
+    """resynth_plugin_run / resynth_heal_selection make the engine plug-in's and the heal
+    script's checks before any device work, with the reference's messages
+    (src/resynthesizer/resynthesizer.c:357-421, PluginScripts/plugin-heal-selection.py:41-43)."""
+    rgb = np.zeros((20, 30, 3), np.uint8)
+    gray = np.zeros((20, 30, 1), np.uint8)
+    sel = np.zeros((20, 30), np.uint8)
+    sel[5:9, 5:9] = 255
+    err, msg, _, _ = R.plugin_run(rgb, sel, corpus=gray)
+    assert err == R.RESYNTH_ERROR_PLUGIN
+    assert msg == "The input texture and output image must have the same number of color channels."
+    err, msg, _, _ = R.plugin_run(rgb, sel, input_map=gray, output_map=rgb)
+    assert (err, msg) == (R.RESYNTH_ERROR_PLUGIN, "The input and output maps must have the same mode")
+    err, msg, _, _ = R.plugin_run(rgb, sel, input_map=rgb[:10], output_map=rgb)
+    assert (err, msg) == (R.RESYNTH_ERROR_PLUGIN, "The input map should be the same size as the input texture image")
+    err, msg, _, _ = R.plugin_run(rgb, sel, input_map=rgb, output_map=rgb[:10])
+    assert (err, msg) == (R.RESYNTH_ERROR_PLUGIN, "The output map should be the same size as the output image")
+    err, msg, _, _ = R.heal_selection(rgb, np.zeros_like(sel))
+    assert (err, msg) == (R.RESYNTH_ERROR_PLUGIN, "You must first select a region to heal.")
+    assert R.heal_selection(rgb, sel, direction=3)[0] == R.RESYNTH_ERROR_INVALID_ARGUMENT
+    assert R.heal_selection(rgb, sel, order=-1)[0] == R.RESYNTH_ERROR_INVALID_ARGUMENT
+    if not _has_device():
+        # past the checks the engine needs the device
+        assert R.plugin_run(rgb, sel)[0] == R.RESYNTH_ERROR_DEVICE
+        assert R.heal_selection(rgb, sel)[0] == R.RESYNTH_ERROR_DEVICE

@@ -312,6 +312,60 @@ def test_reference_regression_image_mapstyle():
     assert np.array_equal(got, expected)
 
 
+# ---------------------------------------------------------------- heal selection (plug-in level)
+
+HEAL = sorted(pins.heal_cases())
+
+
+def _heal_region(name, **kw):
+    import heal_oracle as H
+    c = pins.heal_cases()[name]
+    px, sel, expected, (rx0, ry0, rx1, ry1), _ = pins.heal_input(c)
+    r, out = H.heal(px, sel, c["radius"], c["direction"], c["order"], prng=O.PRNG_GRAND, **kw)
+    return r, pins.gimp_flatten(out)[ry0:ry1, rx0:rx1], expected
+
+
+@pytest.mark.parametrize("name", HEAL)
+def test_heal_regression_image(name):
+    """Test/testResynth.py:331-366: the heal-selection script's corpus (selection grown by
+    GIMP's disc, less the selection, cropped by direction), the plug-in's adaptation and the
+    serial GRand engine reproduce Test/reference_out_images/<name>.ppm byte for byte over
+    the selection and an 8-pixel ring around it (after GIMP's flattening onto white)."""
+    r, got, expected = _heal_region(name)
+    assert r.err == 0
+    assert np.array_equal(got, expected)
+
+
+def test_heal_pins_catch_plausible_mistakes():
+    """The heal pins reject a grow disc measured at the column centres (the all-around
+    crop keeps the disc's corners) and inwards orders encoded without their offset."""
+    import heal_oracle as H
+    _, got, expected = _heal_region("healaroundrandom-ufo-input", inner_edge=False)
+    assert not np.array_equal(got, expected)
+    c = pins.heal_cases()["heal-ufo-input"]
+    px, sel, expected, (rx0, ry0, rx1, ry1), _ = pins.heal_input(c)
+    corpus, csel, ub = H.heal_corpus(px, sel, 50, 1, 1)
+    assert ub == 3
+    ix, tp, cp = H.plugin_maps(px, sel, corpus, csel)
+    params = dict(tileH=0, tileV=0, matchContextType=1, mapWeight=0.0, sensitivity=0.117, patchSize=16,
+                  maxProbeCount=500)
+    r, out = O.engine(params, ix, tp, cp, prng=O.PRNG_GRAND)
+    assert not np.array_equal(out[ry0:ry1, rx0:rx1, 1:4], expected)
+
+
+def test_heal_grow_disc_properties():
+    """The grow disc is symmetric, spans r rows at its centre column, and a single selected
+    pixel grows into exactly the disc's pixels."""
+    import heal_oracle as H
+    for r in (1, 5, 50):
+        c = H.grow_disc(r)
+        assert c == c[::-1] and c[r] == r and max(c) == r
+        sel = np.zeros((2 * r + 3, 2 * r + 3), np.uint8)
+        sel[r + 1, r + 1] = 255
+        g = H.grow(sel, r)
+        assert int((g > 0).sum()) == sum(2 * v + 1 for v in c)
+
+
 # ---------------------------------------------------------------- invariants
 
 def _texture(h, w, seed, c=3):
