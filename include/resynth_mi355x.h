@@ -304,10 +304,13 @@ typedef struct {
 } ResynthRequest;
 
 typedef struct {
-  int wavesPerBlock;              /* targets per workgroup, one wavefront each: 2, 4 or 8 */
+  int wavesPerBlock;              /* wavefronts per workgroup: 2, 4 or 8 */
   int firstGroup;                 /* corpus loads issued together for a candidate's first
-                                     neighbours: 2 or 4 */
-  int laterGroup;                 /* loads issued together for the rest: 4 or 8 */
+                                     neighbours: 2 (0 = 2) */
+  int laterGroup;                 /* loads issued together for the rest: 4 (0 = 4) */
+  int lanesPerTarget;             /* lanes of a wavefront per target visit, i.e. candidates scored
+                                     at once per target: 16, 32 or 64 (4, 2 or 1 targets per
+                                     wavefront at a time; 0 = 64) */
 } ResynthKernelConfig;
 
 typedef struct {
@@ -320,27 +323,35 @@ typedef struct {
   double totalMs;
   unsigned int graphLaunches;     /* one per pixel-word width present */
   unsigned int waitsExpired;      /* waits cut short by their spin limit (0 normally) */
+  unsigned int graphInstantiations; /* graphs instantiated by this call: the instantiated graph
+                                     of each pixel-word width is kept, and a call with its kernel
+                                     shape only updates its nodes' arguments (0 then) */
 } ResynthBatchStats;
 
 /* Runs n requests.  deviceBuffers != 0: image.data and mask.data are device pointers.
  * options may be NULL (batched defaults; the mode field is ignored: always batched);
- * config may be NULL (the autotuned shape if resynth_autotune_batch ran, else 4/2/4).
+ * config may be NULL (the shape resynth_autotune_batch chose for the launch's class, else
+ * 4 waves per block, 64 lanes per target).
  * Returns 0, RESYNTH_ERROR_INVALID_ARGUMENT, or RESYNTH_ERROR_DEVICE; per-request results
  * are in requests[i].error (RESYNTH_ERROR_INVALID_ARGUMENT for a request whose rowBytes is
  * shorter than width * bytes per pixel, or whose mask rowBytes is shorter than width) and
  * the other out fields.  Of the options, seed, batch, maxPasses and rampPermille apply per
- * request; mode, timing, model and the learned* fields are ignored (always the plain
- * batched method). */
+ * request; mode, timing, model, lanesPerTarget, firstChunk and the learned* fields are
+ * ignored (always the plain batched method, shaped by config). */
 int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int deviceBuffers,
                               const ResynthOptions *options, const ResynthKernelConfig *config,
                               ResynthBatchStats *stats);
 
-/* Autotuner of the multi-request launch: times every kernel shape (wavesPerBlock x
- * firstGroup x laterGroup, 12 shapes) on copies of the sample requests, keeps the fastest
- * for later resynth_image_synth_batch calls and returns it in *best.  msPerConfig (may be
- * NULL) receives the 12 timings in the order waves {2,4,8} x first {2,4} x later {4,8}.
- * The sample's images are left unchanged.  Returns 0, RESYNTH_ERROR_INVALID_ARGUMENT or
- * RESYNTH_ERROR_DEVICE. */
+/* Autotuner of the multi-request launch: times every kernel shape (wavesPerBlock {2,4,8},
+ * i.e. targets per workgroup, x lanesPerTarget {16,32,64}, i.e. candidates per wavefront
+ * step: 9 shapes), three times each on copies of the sample requests, and keeps the one
+ * with the lowest median for later resynth_image_synth_batch calls of the same class, which
+ * it returns in *best.  A class is a pixel-word width, a patch-size class (target-weighted
+ * mean patchSize up to 16, up to 32, larger) and a size class (mean targets per request
+ * below 16384, below 131072, larger); a launch of a class never tuned uses
+ * {4, 2, 4, 64}.  msPerConfig (may be NULL) receives the 9 medians in the order waves
+ * {2,4,8} x lanes {16,32,64}.  The sample's images are left unchanged.  Returns 0,
+ * RESYNTH_ERROR_INVALID_ARGUMENT or RESYNTH_ERROR_DEVICE. */
 int resynth_autotune_batch(const ResynthRequest *sample, unsigned int n, int deviceBuffers,
                            ResynthKernelConfig *best, double *msPerConfig);
 

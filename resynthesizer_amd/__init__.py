@@ -117,19 +117,21 @@ class ResynthRequest(ctypes.Structure):
 
 
 class ResynthKernelConfig(ctypes.Structure):
-    _fields_ = [("wavesPerBlock", ctypes.c_int), ("firstGroup", ctypes.c_int), ("laterGroup", ctypes.c_int)]
+    _fields_ = [("wavesPerBlock", ctypes.c_int), ("firstGroup", ctypes.c_int), ("laterGroup", ctypes.c_int),
+                ("lanesPerTarget", ctypes.c_int)]
 
 
 class ResynthBatchStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_uint), ("targets", ctypes.c_ulonglong), ("config", ResynthKernelConfig),
                 ("candidates", ctypes.c_ulonglong), ("patchTerms", ctypes.c_ulonglong), ("prepMs", ctypes.c_double),
                 ("synthKernelMs", ctypes.c_double), ("totalMs", ctypes.c_double),
-                ("graphLaunches", ctypes.c_uint), ("waitsExpired", ctypes.c_uint)]
+                ("graphLaunches", ctypes.c_uint), ("waitsExpired", ctypes.c_uint),
+                ("graphInstantiations", ctypes.c_uint)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         c = self.config
-        d["config"] = (c.wavesPerBlock, c.firstGroup, c.laterGroup)
+        d["config"] = (c.wavesPerBlock, c.lanesPerTarget)
         return d
 
 
@@ -401,12 +403,12 @@ def image_synth_batch(images, masks, fmts, params, device=False, config=None, se
     """Many imageSynth requests in one multi-request launch.  Host mode: images are uint8
     (H, W, C) arrays (copied, the copies synthesized); device mode: images and masks are
     (data_ptr, width, height, rowBytes) tuples synthesized in place.  config: None or
-    (wavesPerBlock, firstGroup, laterGroup).  Returns (rc, per-request dicts, stats dict,
+    (wavesPerBlock, lanesPerTarget).  Returns (rc, per-request dicts, stats dict,
     output arrays in host mode)."""
     if not device:
         images = [np.array(a, dtype=np.uint8, copy=True, order="C") for a in images]
     reqs, keep = _requests(images, masks, fmts, params, device)
-    cfg = ResynthKernelConfig(*config) if config else None
+    cfg = ResynthKernelConfig(config[0], 2, 4, config[1]) if config else None
     st = ResynthBatchStats()
     opt = options(MODE_BATCHED, PRNG_GRAND, seed, batch, False)
     rc = lib().resynth_image_synth_batch(reqs, len(images), int(device), ctypes.byref(opt),
@@ -420,13 +422,13 @@ def image_synth_batch(images, masks, fmts, params, device=False, config=None, se
 
 
 def autotune_batch(images, masks, fmts, params, device=False):
-    """Runs the autotuner on a sample of requests.  Returns (rc, best (waves, first, later),
-    the 12 timings in ms)."""
+    """Runs the autotuner on a sample of requests.  Returns (rc, best (waves per block, lanes
+    per target), the 9 median timings in ms, waves {2, 4, 8} x lanes {16, 32, 64})."""
     reqs, keep = _requests(images, masks, fmts, params, device)
     best = ResynthKernelConfig()
-    ms = np.zeros(12, np.float64)
+    ms = np.zeros(9, np.float64)
     rc = lib().resynth_autotune_batch(reqs, len(images), int(device), ctypes.byref(best), ms.ctypes.data)
-    return rc, (best.wavesPerBlock, best.firstGroup, best.laterGroup), ms
+    return rc, (best.wavesPerBlock, best.lanesPerTarget), ms
 
 
 class Model:

@@ -387,6 +387,7 @@ struct DeviceState {
   Workspace ws;                                   // single-request entries
   std::vector<std::unique_ptr<Workspace>> pool;   // one per request of a multi-request call
   Buf manyDesc, manyCtl, manyGlobal;
+  ManyGraph manyGraph[2];                         // per pixel-word width, kept across calls
   PinnedBuf pinDesc, pinCtl;
   hipEvent_t countsReady = nullptr;               // prepare_engine: point counts on the host
 };
@@ -1515,8 +1516,26 @@ int image_synth_device_impl(Workspace &W, unsigned char *d_image, unsigned int w
 
 // ------------------------------------------------------------------ many requests
 
-ManyConfig g_tuned[2] = {{0, 0, 0}, {0, 0, 0}};   // per pixel-word width, from the autotuner
-const ManyConfig kDefaultMany = {4, 2, 4};
+const ManyConfig kDefaultMany = {4, 2, 4, 64};
+
+// Tuned kernel shapes per launch class: pixel-word width, patch-size class and size class
+// (zero waves: not tuned).
+struct ManyClass {
+  int wide, patch, size;
+};
+ManyConfig g_tuned[2][3][3] = {};
+
+ManyClass many_class(bool wide, const std::vector<unsigned> &idx, const std::vector<Job> &jobs,
+                     const std::vector<Prepared> &prs)
+{
+  double targets = 0, weighted = 0;
+  for (unsigned i : idx) {
+    targets += prs[i].nT;
+    weighted += (double)prs[i].nT * jobs[i].P.patchSize;
+  }
+  const double patch = targets > 0 ? weighted / targets : 30.0, mean = idx.empty() ? 0 : targets / idx.size();
+  return {wide ? 1 : 0, patch <= 16 ? 0 : (patch <= 32 ? 1 : 2), mean < 16384 ? 0 : (mean < 131072 ? 1 : 2)};
+}
 
 unsigned env_permille(const char *name, long dflt)
 {
@@ -1564,7 +1583,7 @@ bool request_invalid(const ResynthRequest &q)
 }
 
 int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOptions *opt, const ManyConfig *force,
-              ResynthBatchStats *st)
+              ResynthBatchStats *st, std::vector<ManyClass> *classesOut = nullptr)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
   if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
@@ -1666,26 +1685,24 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
     ReqCtl *dC = (ReqCtl *)g_dev.manyCtl.get((size_t)m * sizeof(ReqCtl));
     uint32_t *dG = (uint32_t *)g_dev.manyGlobal.get(MG_WORDS * sizeof(uint32_t));
     HIP_TRY(hipMemcpyAsync(dD, hD, (size_t)m * sizeof(ReqDesc), hipMemcpyHostToDevice, s));
-    const ManyConfig cfg = force ? *force : (g_tuned[wide].waves ? g_tuned[wide] : kDefaultMany);
+    const ManyClass mc = many_class(wide != 0, idx, jobs, prs);
+    if (classesOut) classesOut->push_back(mc);
+    const ManyConfig &tuned = g_tuned[mc.wide][mc.patch][mc.size];
+    const ManyConfig cfg = force ? *force : (tuned.waves ? tuned : kDefaultMany);
     bs.config.wavesPerBlock = cfg.waves;
     bs.config.firstGroup = cfg.firstGroup;
     bs.config.laterGroup = cfg.laterGroup;
-    hipGraph_t graph;
-    hipGraphExec_t exec;
-    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    HIP_TRY(launch_fill_u32(s, (uint32_t *)dC, 0u, (size_t)m * sizeof(ReqCtl) / 4));
-    HIP_TRY(launch_fill_u32(s, dG, 0u, MG_WORDS));
-    HIP_TRY(launch_fill_u32(s, dG + MG_ACTIVE, m, 1));
+    bs.config.lanesPerTarget = cfg.lanes;
     bool allRgb = true;
     for (unsigned j = 0; j < m; j++) allRgb = allRgb && hD[j].g.nColors == 3 && hD[j].g.nMaps == 0;
-    HIP_TRY(launch_synth_many(s, wide != 0, cfg, dD, dC, m, dG, allRgb));
-    HIP_TRY(hipStreamEndCapture(s, &graph));
-    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    bool instantiated = false;
+    HIP_TRY(many_graph_prepare(g_dev.manyGraph[wide], wide != 0, cfg, allRgb, dD, dC, m, dG, instantiated));
+    if (instantiated) bs.graphInstantiations++;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, s));
-    HIP_TRY(hipGraphLaunch(exec, s));
+    HIP_TRY(hipGraphLaunch(g_dev.manyGraph[wide].exec, s));
     HIP_TRY(hipEventRecord(e1, s));
     ReqCtl *hC = (ReqCtl *)g_dev.pinCtl.get((size_t)m * sizeof(ReqCtl) + MG_WORDS * sizeof(uint32_t));
     uint32_t *hG = (uint32_t *)(hC + m);
@@ -1700,8 +1717,6 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
     for (unsigned j = 0; j < m; j++) passesRun[idx[j]] = (unsigned)hC[j].passesRun;
     HIP_TRY(hipEventDestroy(e0));
     HIP_TRY(hipEventDestroy(e1));
-    HIP_TRY(hipGraphExecDestroy(exec));
-    HIP_TRY(hipGraphDestroy(graph));
   }
   if (bs.waitsExpired)
     std::fprintf(stderr, "resynth_mi355x: a multi-request launch cut a wait short at its spin limit\n");
@@ -1746,7 +1761,8 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
   return 0;
 }
 
-// Times every kernel shape on private device copies of the sample and keeps the fastest.
+// Times every kernel shape on private device copies of the sample (the median of three runs
+// each) and keeps the fastest for the sample's launch classes.
 int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, ResynthKernelConfig *best, double *msOut)
 {
   {
@@ -1757,7 +1773,6 @@ int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, Resynt
   std::vector<ResynthRequest> reqs(sample, sample + n);
   std::vector<TmpBuf> pristine(n), work(n), masks(n);
   std::vector<size_t> bytes(n, 0), mbytes(n, 0);
-  bool wideSeen[2] = {false, false};
   for (unsigned i = 0; i < n; i++) {
     const ResynthRequest &q = sample[i];
     if (request_invalid(q)) {
@@ -1776,9 +1791,6 @@ int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, Resynt
     HIP_TRY(hipMemcpyAsync(masks[i].p, q.mask.data, mbytes[i], k, s));
     reqs[i].image.data = (unsigned char *)work[i].p;
     reqs[i].mask.data = (unsigned char *)masks[i].p;
-    TFormatIndices ix;
-    if (!prepareImageFormatIndicesFromFormatType(&ix, q.format))
-      wideSeen[(ix.img_match_bpp + ix.map_match_bpp) > 3 ? 1 : 0] = true;
   }
   auto restore = [&]() {
     for (unsigned i = 0; i < n; i++)
@@ -1787,35 +1799,41 @@ int autotune_batch(const ResynthRequest *sample, unsigned n, bool device, Resynt
   double bestMs = 1e30;
   ManyConfig bestCfg = kDefaultMany;
   int c = 0, rc = 0;
+  std::vector<ManyClass> classes;
   restore();
-  rc = run_batch(reqs.data(), n, true, nullptr, &kDefaultMany, nullptr);   // warm-up
-  const int wavesList[3] = {2, 4, 8}, firstList[2] = {2, 4}, laterList[2] = {4, 8};
+  rc = run_batch(reqs.data(), n, true, nullptr, &kDefaultMany, nullptr, &classes);   // warm-up
+  const int wavesList[3] = {2, 4, 8}, lanesList[3] = {16, 32, 64};
   for (int wv : wavesList)
-    for (int fg : firstList)
-      for (int lg : laterList) {
-        if (rc) break;
-        const ManyConfig cfg = {wv, fg, lg};
+    for (int ln : lanesList) {
+      if (rc) break;
+      const ManyConfig cfg = {wv, 2, 4, ln};
+      double t[3];
+      for (int rep = 0; rep < 3 && !rc; rep++) {
         restore();
         ResynthBatchStats st;
         rc = run_batch(reqs.data(), n, true, nullptr, &cfg, &st);
-        if (msOut) msOut[c] = st.synthKernelMs;
-        if (!rc && st.synthKernelMs < bestMs) {
-          bestMs = st.synthKernelMs;
-          bestCfg = cfg;
-        }
-        c++;
+        t[rep] = st.synthKernelMs;
       }
+      if (rc) break;
+      std::sort(t, t + 3);
+      if (msOut) msOut[c] = t[1];
+      if (t[1] < bestMs) {
+        bestMs = t[1];
+        bestCfg = cfg;
+      }
+      c++;
+    }
   HIP_TRY(hipStreamSynchronize(s));
   if (rc) return rc;
   {
     std::lock_guard<std::mutex> lock(g_mutex);
-    for (int w = 0; w < 2; w++)
-      if (wideSeen[w]) g_tuned[w] = bestCfg;
+    for (const ManyClass &mc : classes) g_tuned[mc.wide][mc.patch][mc.size] = bestCfg;
   }
   if (best) {
     best->wavesPerBlock = bestCfg.waves;
     best->firstGroup = bestCfg.firstGroup;
     best->laterGroup = bestCfg.laterGroup;
+    best->lanesPerTarget = bestCfg.lanes;
   }
   return 0;
 }
@@ -2062,7 +2080,8 @@ int resynth_image_synth_batch(ResynthRequest *requests, unsigned int n, int devi
   ManyConfig cfg;
   const ManyConfig *force = nullptr;
   if (config && config->wavesPerBlock) {
-    cfg = {config->wavesPerBlock, config->firstGroup, config->laterGroup};
+    cfg = {config->wavesPerBlock, config->firstGroup ? config->firstGroup : 2, config->laterGroup ? config->laterGroup : 4,
+           config->lanesPerTarget ? config->lanesPerTarget : 64};
     if (!many_config_valid(cfg)) return -1;
     force = &cfg;
   }

@@ -128,9 +128,10 @@ enum { MG_ACTIVE = 0, MG_EXPIRED = 32, MG_WORDS = 64 };
 
 // Kernel shape of the multi-request launch, chosen by the autotuner.
 struct ManyConfig {
-  int waves;              // targets per workgroup (one wavefront each): 2, 4 or 8
-  int firstGroup;         // loads issued together for a candidate's first neighbours: 2 or 4
-  int laterGroup;         // loads issued together afterwards: 4 or 8
+  int waves;              // wavefronts per workgroup: 2, 4 or 8
+  int firstGroup;         // loads issued together for a candidate's first neighbours: 2
+  int laterGroup;         // loads issued together afterwards: 4
+  int lanes;              // lanes per target visit (64 / targets per wave): 16, 32 or 64
 };
 
 // ---- kernel launchers (resynth_kernels.hip) ----
@@ -169,9 +170,19 @@ unsigned synth_wave_min();   // chunks of at least this many targets use k_synth
 hipError_t launch_synth_window(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, unsigned seed, unsigned pass, unsigned minW, unsigned rampPermille,
                                unsigned cap);
-hipError_t launch_synth_many(hipStream_t s, bool wide, const ManyConfig &cfg, const ReqDesc *descs, ReqCtl *ctls,
-                             unsigned nReq, uint32_t *global, bool allRgb);
 bool many_config_valid(const ManyConfig &cfg);
+// The multi-request launch as an instantiated HIP graph (init kernel, persistent kernel),
+// kept across calls: a call with the kernel shape of the last one updates the nodes'
+// arguments in place; another shape rebuilds it (instantiated = true).
+struct ManyGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraphNode_t init = nullptr, many = nullptr;
+  const void *fn = nullptr;   // the persistent kernel of the instantiated graph
+};
+hipError_t many_graph_prepare(ManyGraph &G, bool wide, const ManyConfig &cfg, bool allRgb, const ReqDesc *descs,
+                              ReqCtl *ctls, unsigned nReq, uint32_t *global, bool &instantiated);
+void many_graph_release(ManyGraph &G);
 hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, const SynthBuffers &b, unsigned k0,
                                unsigned k1, int prngKind, const uint32_t *stream, uint32_t streamLen,
                                StreamState *state);

@@ -1676,20 +1676,24 @@ __device__ __forceinline__ unsigned many_window(const ReqDesc &d, unsigned pass,
   return W < d.capW ? W : d.capW;
 }
 
-// 32-bit words with groups of 2 then 4 ask for 6 waves per SIMD (at most 80 VGPRs), which
-// they reach without spilling; the resident grid below follows from the occupancy query.
-template <typename Word, int WAVES, int G0, int GL, int NC = 0>
-__global__ __launch_bounds__(64 * WAVES)
-__attribute__((amdgpu_waves_per_eu((sizeof(Word) == 4 && G0 == 2 && GL == 4) ? 6 : 1)))
+// A wave visits its ticket group 64/LPT targets at a time (segmented visits, visit_seg);
+// the resident grid below follows from the occupancy query.  32-bit words ask for 6 waves
+// per SIMD (at most 80 VGPRs), which they reach without spilling.
+template <typename Word, int WAVES, int LPT, int NC = 0>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(sizeof(Word) == 4 ? 6 : 1)))
 void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
 {
+  constexpr int TPW = 64 / LPT;
+  constexpr int G0 = 2, GL = 4;
   __shared__ uint16_t sT[WAVES][512];
   __shared__ uint32_t sTm[WAVES][512];
-  __shared__ int2 sOff[WAVES][64];
-  __shared__ Word sPix[WAVES][64];
-  __shared__ int2 sSrc[WAVES][64];
+  __shared__ int2 sOff[WAVES][TPW][64];
+  __shared__ Word sPix[WAVES][TPW][64];
+  __shared__ int2 sSrc[WAVES][TPW][64];
+  __shared__ VisitTally sTally[WAVES];
   const int wave = threadIdx.x >> 6;
   const unsigned lane = lane_id();
+  const unsigned seg = lane / LPT;
   unsigned *active = global + MG_ACTIVE, *expired = global + MG_EXPIRED;
   unsigned r = (blockIdx.x * WAVES + wave) % nReq;
   const Tables *loaded = nullptr;
@@ -1759,11 +1763,14 @@ void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq
     }
     Geometry g = d.g;
     g.skipEmpty = (pass == 0 && d.g.skipEmpty) ? 1 : 0;
-    unsigned long long ev = 0, tm = 0;
-    unsigned bet = 0;
-    for (unsigned j = 0; j < nGot; j++)
-      visit_wave<Word, G0, GL, true, NC>(g, d.b, tk0 + j, d.seed, pass, sOff[wave], sPix[wave], sSrc[wave],
-                                         sT[wave], sTm[wave], ev, tm, bet);
+    if (lane == 0) sTally[wave] = VisitTally{};
+    wave_sync();
+    for (unsigned j = 0; j < nGot; j += TPW)
+      visit_seg<Word, LPT, G0, GL, true, NC>(g, d.b, tk0 + j, tk0 + nGot, d.seed, pass, sOff[wave][seg],
+                                             sPix[wave][seg], sSrc[wave][seg], sT[wave], sTm[wave], &sTally[wave]);
+    wave_sync();
+    const unsigned long long ev = sTally[wave].eval, tm = sTally[wave].term;
+    const unsigned bet = sTally[wave].bet;
     if (lane == 0) {
       const unsigned slot = tk0 & (CNT_SLOTS - 1);
       if (ev) atomicAdd(&d.b.counters->candidates[slot], ev);
@@ -1790,68 +1797,123 @@ void k_synth_many(const ReqDesc *__restrict__ descs, ReqCtl *ctls, unsigned nReq
 
 bool many_config_valid(const ManyConfig &c)
 {
-  return (c.waves == 2 || c.waves == 4 || c.waves == 8) && (c.firstGroup == 2 || c.firstGroup == 4) &&
-         (c.laterGroup == 4 || c.laterGroup == 8);
+  return (c.waves == 2 || c.waves == 4 || c.waves == 8) && c.firstGroup == 2 && c.laterGroup == 4 &&
+         (c.lanes == 16 || c.lanes == 32 || c.lanes == 64);
 }
 
-template <typename Word, int WAVES, int G0, int GL, int NC>
-static hipError_t launch_many_nc(hipStream_t s, const ReqDesc *descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
+// The kernel of a shape and its resident grid (workgroups per CU from the occupancy query).
+struct ManyKernel {
+  const void *fn;
+  unsigned grid, block;
+};
+
+template <typename Word, int WAVES, int LPT, int NC>
+static ManyKernel many_kernel_nc()
 {
-  static int resident = [] {
+  static const unsigned resident = [] {
     int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 1024;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_synth_many<Word, WAVES, G0, GL, NC>, 64 * WAVES, 0) !=
+    if (hipGetDevice(&dev) != hipSuccess) return 1024u;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024u;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_synth_many<Word, WAVES, LPT, NC>, 64 * WAVES, 0) !=
             hipSuccess || per < 1)
       per = 1;
-    return cus * per;
+    return (unsigned)(cus * per);
   }();
-  hipLaunchKernelGGL((k_synth_many<Word, WAVES, G0, GL, NC>), dim3(resident), dim3(64 * WAVES), 0, s, descs, ctls,
-                     nReq, global);
-  return hipGetLastError();
+  return {(const void *)k_synth_many<Word, WAVES, LPT, NC>, resident, 64u * WAVES};
 }
 
-// Every request of the launch RGB without maps (32-bit words): the branch-free scorer.
-static thread_local bool g_manyRgb = false;
-
-template <typename Word, int WAVES, int G0, int GL>
-static hipError_t launch_many_t(hipStream_t s, const ReqDesc *descs, ReqCtl *ctls, unsigned nReq, uint32_t *global)
+// RGB without maps (32-bit words) in every request of the launch: the branch-free scorer.
+template <typename Word, int WAVES, int LPT>
+static ManyKernel many_kernel_t(bool rgb)
 {
-  if (sizeof(Word) == 4 && g_manyRgb) return launch_many_nc<Word, WAVES, G0, GL, 3>(s, descs, ctls, nReq, global);
-  return launch_many_nc<Word, WAVES, G0, GL, 0>(s, descs, ctls, nReq, global);
+  if (sizeof(Word) == 4 && rgb) return many_kernel_nc<Word, WAVES, LPT, 3>();
+  return many_kernel_nc<Word, WAVES, LPT, 0>();
 }
 
 template <typename Word, int WAVES>
-static hipError_t launch_many_w(hipStream_t s, const ManyConfig &c, const ReqDesc *d, ReqCtl *x, unsigned n,
-                                uint32_t *gl)
+static ManyKernel many_kernel_w(const ManyConfig &c, bool rgb)
 {
-  if (c.firstGroup == 2)
-    return c.laterGroup == 4 ? launch_many_t<Word, WAVES, 2, 4>(s, d, x, n, gl)
-                             : launch_many_t<Word, WAVES, 2, 8>(s, d, x, n, gl);
-  return c.laterGroup == 4 ? launch_many_t<Word, WAVES, 4, 4>(s, d, x, n, gl)
-                           : launch_many_t<Word, WAVES, 4, 8>(s, d, x, n, gl);
-}
-
-template <typename Word>
-static hipError_t launch_many_word(hipStream_t s, const ManyConfig &c, const ReqDesc *d, ReqCtl *x, unsigned n,
-                                   uint32_t *gl)
-{
-  switch (c.waves) {
-  case 2: return launch_many_w<Word, 2>(s, c, d, x, n, gl);
-  case 8: return launch_many_w<Word, 8>(s, c, d, x, n, gl);
-  default: return launch_many_w<Word, 4>(s, c, d, x, n, gl);
+  switch (c.lanes) {
+  case 16: return many_kernel_t<Word, WAVES, 16>(rgb);
+  case 32: return many_kernel_t<Word, WAVES, 32>(rgb);
+  default: return many_kernel_t<Word, WAVES, 64>(rgb);
   }
 }
 
-// The caller zeroes ctls, sets each ctr to pass 0 and global[MG_ACTIVE] to nReq.
-hipError_t launch_synth_many(hipStream_t s, bool wide, const ManyConfig &cfg, const ReqDesc *descs, ReqCtl *ctls,
-                             unsigned nReq, uint32_t *global, bool allRgb)
+template <typename Word>
+static ManyKernel many_kernel_word(const ManyConfig &c, bool rgb)
 {
-  if (!nReq) return hipSuccess;
-  if (!many_config_valid(cfg)) return hipErrorInvalidValue;
-  g_manyRgb = allRgb && rgb_fast_path();
-  return wide ? launch_many_word<uint64_t>(s, cfg, descs, ctls, nReq, global)
-              : launch_many_word<uint32_t>(s, cfg, descs, ctls, nReq, global);
+  switch (c.waves) {
+  case 2: return many_kernel_w<Word, 2>(c, rgb);
+  case 8: return many_kernel_w<Word, 8>(c, rgb);
+  default: return many_kernel_w<Word, 4>(c, rgb);
+  }
+}
+
+// Clears the requests' control words and the launch's global words; global[MG_ACTIVE] =
+// nReq.  (A kernel, not a memset node: fills in graphs here are kernels.)
+__global__ void k_many_init(uint32_t *ctl, size_t ctlWords, uint32_t *global, unsigned nReq)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < ctlWords; i += (size_t)gridDim.x * blockDim.x)
+    ctl[i] = 0;
+  if (blockIdx.x == 0)
+    for (unsigned i = threadIdx.x; i < MG_WORDS; i += blockDim.x) global[i] = i == MG_ACTIVE ? nReq : 0u;
+}
+
+void many_graph_release(ManyGraph &G)
+{
+  if (G.exec) (void)hipGraphExecDestroy(G.exec);
+  if (G.graph) (void)hipGraphDestroy(G.graph);
+  G = ManyGraph{};
+}
+
+// The multi-request graph, two kernel nodes (init, then the persistent launch), built with
+// the graph API so that later calls with the same kernel shape update the nodes'
+// parameters (request count, buffers) in the instantiated graph instead of capturing and
+// instantiating it again.
+hipError_t many_graph_prepare(ManyGraph &G, bool wide, const ManyConfig &cfg, bool allRgb, const ReqDesc *descs,
+                              ReqCtl *ctls, unsigned nReq, uint32_t *global, bool &instantiated)
+{
+  instantiated = false;
+  if (!nReq || !many_config_valid(cfg)) return hipErrorInvalidValue;
+  const bool rgb = allRgb && rgb_fast_path();
+  const ManyKernel K = wide ? many_kernel_word<uint64_t>(cfg, rgb) : many_kernel_word<uint32_t>(cfg, rgb);
+  uint32_t *ctlWords = (uint32_t *)ctls;
+  size_t nCtl = (size_t)nReq * sizeof(ReqCtl) / sizeof(uint32_t);
+  const ReqDesc *d = descs;
+  ReqCtl *c = ctls;
+  unsigned n = nReq;
+  uint32_t *gl = global;
+  void *initArgs[] = {&ctlWords, &nCtl, &gl, &n};
+  void *manyArgs[] = {&d, &c, &n, &gl};
+  hipKernelNodeParams pi = {};
+  pi.func = (void *)k_many_init;
+  pi.gridDim = dim3((unsigned)std::min<size_t>(1024, (nCtl + 255) / 256));
+  pi.blockDim = dim3(256);
+  pi.kernelParams = initArgs;
+  hipKernelNodeParams pm = {};
+  pm.func = const_cast<void *>(K.fn);
+  pm.gridDim = dim3(K.grid);
+  pm.blockDim = dim3(K.block);
+  pm.kernelParams = manyArgs;
+  if (G.exec && G.fn == K.fn) {
+    hipError_t e = hipGraphExecKernelNodeSetParams(G.exec, G.init, &pi);
+    if (e == hipSuccess) e = hipGraphExecKernelNodeSetParams(G.exec, G.many, &pm);
+    if (e == hipSuccess) return hipSuccess;
+    (void)hipGetLastError();   // build it again below
+  }
+  many_graph_release(G);
+  hipError_t e = hipGraphCreate(&G.graph, 0);
+  if (e == hipSuccess) e = hipGraphAddKernelNode(&G.init, G.graph, nullptr, 0, &pi);
+  if (e == hipSuccess) e = hipGraphAddKernelNode(&G.many, G.graph, &G.init, 1, &pm);
+  if (e == hipSuccess) e = hipGraphInstantiate(&G.exec, G.graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    many_graph_release(G);
+    return e;
+  }
+  G.fn = K.fn;
+  instantiated = true;
+  return hipSuccess;
 }
 
 template <typename Word, int GL>
