@@ -741,7 +741,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
     uint32_t *k1 = (uint32_t *)W.orderKeys2.get((size_t)nT * sizeof(uint32_t) + 64);
     uint32_t *v0 = (uint32_t *)W.orderVals.get((size_t)nT * sizeof(uint32_t) + 64);
     const size_t tb = sort_pairs_temp_bytes(nT);
-    HIP_TRY(launch_order_keys(s, (job.opt.seed ? job.opt.seed : kEngineSeed) ^ 0x5EED0DE5u, nT, k0, v0));
+    HIP_TRY(launch_random_order_keys(s, (job.opt.seed ? job.opt.seed : kEngineSeed) ^ 0x5EED0DE5u, nT, k0, v0));
     HIP_TRY(launch_sort_pairs(s, k0, k1, v0, pr.dPerm, nT, 32, W.orderTemp.get(tb + 64), tb));
     HIP_TRY(launch_gather_targets(s, targetsRM, pr.dPerm, targets, nT));
     pr.orderPending = false;

@@ -2496,7 +2496,7 @@ hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *
 // Sort keys of the batched mode's random target order: a counter-based hash of (seed, i)
 // per row-major target i; sorted (stably, all 32 bits) with the indices as values they give
 // a uniformly random permutation, drawn on the device.
-__global__ void k_order_keys(uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
+__global__ void k_random_order_keys(uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
 {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -2508,10 +2508,10 @@ __global__ void k_order_keys(uint32_t seed, unsigned n, uint32_t *keys, uint32_t
   vals[i] = i;
 }
 
-hipError_t launch_order_keys(hipStream_t s, uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
+hipError_t launch_random_order_keys(hipStream_t s, uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
 {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_order_keys, dim3((n + 255) / 256), dim3(256), 0, s, seed, n, keys, vals);
+  hipLaunchKernelGGL(k_random_order_keys, dim3((n + 255) / 256), dim3(256), 0, s, seed, n, keys, vals);
   return hipGetLastError();
 }
 
