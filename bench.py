@@ -118,19 +118,22 @@ def bench_config3(args, np, torch, R, W):
 
 # Learned-mode settings of the config-5 line: proposals of every target computed once at
 # the start of pass 1 from the pass-0 result (1 inverted-file cluster searched, top 16),
-# 1 random probe beside them, 16 random probes in pass 0, whose first chunks hold 256
-# targets and whose chunks then grow by 30% of the visited targets, 2 passes.
+# 1 random probe beside them, 16 random probes in pass 0, whose chunks grow by 30% of the
+# visited targets, 2 passes.
 LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
-               ramp_permille=300, first_chunk=256)
+               ramp_permille=300)
+# The same with 32 random probes beside the proposals (the probe count of the configured
+# setting).
+LEARNED32 = dict(LEARNED, learned_probes=32)
 # The configured learned setting: proposals for every chunk from its targets' current
 # neighbourhoods (top 16 of 2 inverted-file clusters) beside 32 random probes, in every pass
 # of the refiner's own schedule (up to 6 passes, 10% stop rule).
 CONFIGURED = dict(learned_probes=32)
 # Cheaper learned settings (fewer random probes in pass 0, and in pass 1 beside the
 # proposals), searched for the fastest one whose error still reaches the plain method's.
-LEARNED_LADDER = [dict(LEARNED, first_chunk=1024), dict(LEARNED, first_chunk=1024, learned_probes_later=4),
-                  dict(LEARNED, first_chunk=4096, learned_probes_later=4),
-                  dict(LEARNED, first_chunk=4096, learned_probes_later=1, learned_probes=0)]
+LEARNED_LADDER = [dict(LEARNED, first_chunk=256, ramp_permille=500), dict(LEARNED, first_chunk=1024, ramp_permille=500),
+                  dict(LEARNED, first_chunk=1024, ramp_permille=500, learned_probes_later=4),
+                  dict(LEARNED, first_chunk=4096, ramp_permille=500, learned_probes_later=1, learned_probes=0)]
 # bytes priced per unit of visit-kernel work: a corpus pixel word per patch term of a
 # scored candidate (priced at its word width), 9 B per patch offset the neighbour scan
 # reads (an 8 B offset and the target's presence byte), 12 B per staged neighbour (offset,
@@ -240,7 +243,7 @@ def bench_config5(args, np, torch, R, W):
     rl = run(200, LEARNED, timing=True)
     rp = run(200, None, timing=True)
     n_t = rl.nTargets
-    ladder, learned_ladder, configured = [], [], None
+    ladder, learned_ladder, configured, probes32 = [], [], None, None
     if not args.no_ladder:
         # the plain method with more probes, until its error reaches the learned error or the
         # probe budget ends: its time there bounds the plain time at equal error from below
@@ -256,6 +259,11 @@ def bench_config5(args, np, torch, R, W):
                                    "ms": round(sec * 1e3, 2), "mean_match_error": round(err, 1)})
             if err > err_p:
                 break
+        sec_32, err_32 = timed(200, LEARNED32, 5, 1)
+        probes32 = {"setting": "the bench setting with 32 random probes beside the proposals",
+                    "ms": round(sec_32 * 1e3, 2), "px_per_s": round(n_t / sec_32, 1),
+                    "mean_match_error": round(err_32, 1), "speedup_vs_plain200": round(sec_p / sec_32, 3),
+                    "error_ratio_vs_plain200": round(err_32 / err_p, 4)}
         sec_c, err_c = timed(200, CONFIGURED, 3, 1)
         rc = run(200, CONFIGURED, timing=True)
         configured = {"setting": "proposals per chunk (top 16 of 2 clusters) + 32 random probes, every pass, "
@@ -267,6 +275,8 @@ def bench_config5(args, np, torch, R, W):
     # is at most the plain 200-probe error
     eq = [(sec_l * 1e3, err_l, "bench setting")] + [(e["ms"], e["mean_match_error"], e["setting"])
                                                      for e in learned_ladder]
+    if probes32:
+        eq.append((probes32["ms"], probes32["mean_match_error"], "32 random probes"))
     eq = [e for e in eq if e[1] <= err_p]
     best_eq = min(eq, key=lambda e: e[0])
     reached = bool(ladder) and ladder[-1]["mean_match_error"] <= err_l
@@ -287,6 +297,7 @@ def bench_config5(args, np, torch, R, W):
                                         "error is at most the plain error (%s: %.1f ms, error %.0f)"
                                         % (best_eq[2], best_eq[0], best_eq[1]),
                    "learned_ladder": learned_ladder,
+                   "with_32_probes": probes32,
                    "configured_setting": configured,
                    "plain_probe_ladder": ladder,
                    "plain_time_at_learned_error_over_learned": (round(ladder[-1]["ms"] / (sec_l * 1e3), 2)

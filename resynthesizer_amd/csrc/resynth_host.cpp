@@ -363,7 +363,7 @@ struct TmpBuf {
 // Device and pinned buffers of one engine run (grow-only, reused across calls).
 struct Workspace {
   Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
-      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, ctxRings, ringTemp, firstIdx, wctl,
+      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, ctxRadius, ringTemp, firstIdx, wctl,
       bandStart;
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
   Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, keyCl, bCount, bStart, bCursor,
@@ -698,10 +698,10 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   uint8_t *tileHas = (uint8_t *)W.tileHas.get((nTiles + 3) / 4 * 4);
   HIP_TRY(launch_fill_u32(s, (uint32_t *)tileHas, 0u, (nTiles + 3) / 4));
   HIP_TRY(launch_tile_init(s, job.wide, tword, tw, th, tileHas));
-  // empty rings of the context-only occupancy, before any visit updates tileHas
-  int32_t *ctxRings = (int32_t *)W.ctxRings.get(nTiles * sizeof(int32_t) + 64);
-  HIP_TRY(launch_ctx_rings(s, tileHas, tilesW, tilesH, (int32_t *)W.ringTemp.get(nTiles * sizeof(int32_t) + 64),
-                           ctxRings));
+  // context radius of the context-only occupancy, before any visit updates tileHas
+  int32_t *ctxRadius = (int32_t *)W.ctxRadius.get(nTiles * sizeof(int32_t) + 64);
+  HIP_TRY(launch_ctx_radius(s, tileHas, tilesW, tilesH, (int32_t *)W.ringTemp.get(nTiles * sizeof(int32_t) + 64),
+                            ctxRadius));
   const unsigned rhoMax = (unsigned)std::ceil(std::sqrt((double)(ow - 1) * (ow - 1) + (double)(oh - 1) * (oh - 1))) + 1;
   uint32_t *firstIdx = (uint32_t *)W.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
   HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
@@ -807,7 +807,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   b.tables = dT;
   b.counters = cnt;
   b.tileHas = tileHas;
-  b.ctxRings = ctxRings;
+  b.ctxRadius = ctxRadius;
   b.firstIdx = firstIdx;
   b.ctl = (uint32_t *)W.wctl.get(WCTL_WORDS * sizeof(uint32_t));
   b.bandStart = bandStart;
@@ -856,6 +856,20 @@ unsigned bits_for(unsigned v)
   return b;
 }
 
+// Share (permille) of the compute units the key embeddings of the index build take: each of
+// their workgroups holds a whole CU's registers, and the rest of the CUs stay free for pass
+// 0's first chunks on the library stream, which have no other way to start
+// (RESYNTH_INDEX_CU_PERMILLE overrides it).
+int index_cu_share()
+{
+  static const int v = [] {
+    const char *e = std::getenv("RESYNTH_INDEX_CU_PERMILLE");
+    const long x = e ? std::atol(e) : 600;
+    return (int)std::min(1000L, std::max(1L, x));
+  }();
+  return v;
+}
+
 // Two-level inverted file over the corpus keys: S super-centroids (every (nK/S)-th key),
 // Q = 128 sub-centroids inside each, about 240 keys per cluster.
 void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
@@ -890,7 +904,7 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
     const int nb = (int)std::min<unsigned>(kKeyBlock, nK - b0);
     HIP_TRY(learn_features(s, job.wide, pr.b.cword, g.cw, g.ch, 0, 0, g.nColors, pr.b.offsets, L.keyPts + b0, nb, true,
                            true, 0, x));
-    HIP_TRY(mlp_embed_fused(s, w, wt, x, nb, L.keyZ + (size_t)b0 * kEmb, nullptr));
+    HIP_TRY(mlp_embed_fused(s, w, wt, x, nb, L.keyZ + (size_t)b0 * kEmb, nullptr, index_cu_share()));
   }
   const unsigned Q = 128;
   const unsigned S = std::max(1u, std::min(128u, nK / (Q * 240u)));

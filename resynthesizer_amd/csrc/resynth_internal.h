@@ -80,8 +80,8 @@ struct SynthBuffers {
   const Tables *tables;
   Counters *counters;
   uint8_t *tileHas;       // [tilesH*tilesW] 1 when the tile holds a pixel with a value
-  const int32_t *ctxRings; // [tilesH*tilesW] empty tile rings around each tile in the occupancy
-                           // before any visit (context only); see empty_rings
+  const int32_t *ctxRadius; // [tilesH*tilesW] radius around each tile inside which no pixel of the
+                            // occupancy before any visit (the context) lies; see k_ctx_radius
   const uint32_t *firstIdx; // [rhoMax+1] number of sorted offsets with x^2+y^2 < rho^2
   uint32_t *ctl;          // [WCTL_WORDS] sliding-window launch control
   const uint32_t *bandStart; // [kBands+1] index in corpusPts of each band's first point
@@ -152,7 +152,7 @@ hipError_t launch_sort_pairs(hipStream_t s, const uint32_t *kIn, uint32_t *kOut,
 hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 hipError_t launch_band_starts(hipStream_t s, const int2 *corpusPts, const unsigned *nC, int ch, uint32_t *out);
-hipError_t launch_ctx_rings(hipStream_t s, const uint8_t *mark, int W, int H, int32_t *scratch, int32_t *rings);
+hipError_t launch_ctx_radius(hipStream_t s, const uint8_t *mark, int W, int H, int32_t *scratch, int32_t *radius);
 hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas);
 hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx);
 hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);

@@ -235,11 +235,12 @@ __device__ __forceinline__ unsigned scan_start(const Geometry &g, const uint8_t 
   return R < 0 ? 1u : offset_index(g, firstIdx, (unsigned)(kTile * R + 1));
 }
 
-// Empty Chebyshev rings of tiles around every tile of a static occupancy map (the
-// context-only snapshot): rings[i] = d - 1 for d the Chebyshev distance in tiles from
-// tile i to the nearest marked tile (-1 when tile i is marked), or the tile's largest
-// ring when nothing is marked -- what empty_rings walks to, in one load.  Rows first:
-// h = horizontal distance to the nearest marked tile of the row.
+// Context radius of every tile of a static occupancy map (the context-only snapshot): the
+// largest rho such that no pixel of a marked tile lies nearer than rho to any pixel of the
+// tile -- floor of the least Euclidean distance between the tile's box and a marked tile's
+// box (boxes of tiles t apart in a row or column are (t - 1) * kTile + 1 pixels apart); 0
+// for a marked tile, and past every offset when nothing is marked.  Rows first:
+// h = horizontal distance in tiles to the nearest marked tile of the row.
 __global__ void k_ring_rows(const uint8_t *mark, int W, int H, int32_t *h)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,27 +257,46 @@ __global__ void k_ring_rows(const uint8_t *mark, int W, int H, int32_t *h)
   h[i] = d;
 }
 
-// Then columns: d(x, y) = min over y' of max(h(x, y'), |y - y'|), searched outward from y
-// and stopped once |y - y'| reaches the best found.
-__global__ void k_ring_cols(const int32_t *h, int W, int H, int32_t *rings)
+__device__ __forceinline__ uint64_t box_gap(int tiles)
+{
+  return tiles <= 0 ? 0u : (uint64_t)(tiles - 1) * kTile + 1u;
+}
+
+// Then columns: d2(x, y) = min over rows y' of gap(h(x, y'))^2 + gap(|y - y'|)^2, searched
+// outward from y and stopped once the row gap alone reaches the best found.
+__global__ void k_ctx_radius(const int32_t *h, int W, int H, int32_t *radius)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W * H) return;
   constexpr int kFar = 1 << 30;
   const int x = i % W, y = i / W;
-  int best = kFar;
-  for (int dist = 0; dist < best && (y - dist >= 0 || y + dist < H); dist++) {
-    if (y - dist >= 0) best = min(best, max(h[(size_t)(y - dist) * W + x], dist));
-    if (y + dist < H) best = min(best, max(h[(size_t)(y + dist) * W + x], dist));
+  uint64_t best = ~0ull;
+  for (int dist = 0; y - dist >= 0 || y + dist < H; dist++) {
+    const uint64_t gy = box_gap(dist);
+    if (gy * gy >= best) break;
+    for (int side = 0; side < (dist ? 2 : 1); side++) {
+      const int yy = side ? y + dist : y - dist;
+      if (yy < 0 || yy >= H) continue;
+      const int hd = h[(size_t)yy * W + x];
+      if (hd >= kFar) continue;
+      const uint64_t gx = box_gap(hd), d2 = gx * gx + gy * gy;
+      if (d2 < best) best = d2;
+    }
   }
-  const int maxR = max(max(x, W - 1 - x), max(y, H - 1 - y));
-  rings[i] = best >= kFar ? maxR : best - 1;
+  if (best == ~0ull) {
+    radius[i] = 1 << 30;
+    return;
+  }
+  uint64_t r = (uint64_t)sqrt((double)best);
+  while (r * r > best) r--;
+  while ((r + 1) * (r + 1) <= best) r++;
+  radius[i] = (int32_t)(r < (1u << 30) ? r : (1u << 30));
 }
 
-hipError_t launch_ctx_rings(hipStream_t s, const uint8_t *mark, int W, int H, int32_t *scratch, int32_t *rings)
+hipError_t launch_ctx_radius(hipStream_t s, const uint8_t *mark, int W, int H, int32_t *scratch, int32_t *radius)
 {
   hipLaunchKernelGGL(k_ring_rows, dim3((W * H + 255) / 256), dim3(256), 0, s, mark, W, H, scratch);
-  hipLaunchKernelGGL(k_ring_cols, dim3((W * H + 255) / 256), dim3(256), 0, s, scratch, W, H, rings);
+  hipLaunchKernelGGL(k_ctx_radius, dim3((W * H + 255) / 256), dim3(256), 0, s, scratch, W, H, radius);
   return hipGetLastError();
 }
 
@@ -488,7 +508,7 @@ struct VisitLDS {
   uint32_t streamPos;
   uint32_t lastUsed;
   const uint8_t *tileHas;
-  const int32_t *ctxRings;
+  const int32_t *ctxRadius;
   const uint32_t *firstIdx;
 };
 
@@ -503,12 +523,13 @@ struct VisitLDS {
 // Early in pass 0 (nVisited = targets[0, nVisited) visited, at most kNearTargets), a
 // target deep in a hole would scan everything between a few scattered visited pixels and
 // the hole's border.  Instead the context-only tile map (a snapshot taken before any
-// visit) gives a radius rho inside which no context pixel lies; inside it the only
-// valued pixels are visited targets, which are found directly and ordered like the
-// sorted offsets (x^2 + y^2 ascending, row-major offset index descending); the scan then
-// resumes at rho.  When more of them lie inside rho than the patch holds, the plain scan
-// runs instead.  The patch is the same either way (targets of the current chunk that
-// land early are the only difference a batched visit may see).
+// visit) gives a radius rho inside which no context pixel lies (k_ctx_radius); inside it
+// the only valued pixels are visited targets, which are found directly and ordered like
+// the sorted offsets (x^2 + y^2 ascending, row-major offset index descending).  When at
+// least L - 1 of them lie inside rho, the nearest L - 1 are the patch and nothing is
+// scanned; else the scan resumes at rho.  The patch is the same as the plain scan's
+// (targets of the current chunk that land early are the only difference a batched visit
+// may see).
 template <typename Word, int WAVES>
 __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, const int2 *sourceOf,
                                          const int2 *offsets, int tx, int ty, VisitLDS<Word, WAVES> &S,
@@ -528,42 +549,90 @@ __device__ unsigned gather_neighbours_wg(const Geometry &g, const Word *tword, c
   unsigned start = 1;
   auto anyTile = [](bool v) { return __syncthreads_or(v) != 0; };
   bool near = false;
-  if (g.skipEmpty && S.ctxRings && nVisited > 0 && nVisited <= kNearTargets) {
-    const int R = S.ctxRings[(size_t)(ty / kTile) * g.tilesW + tx / kTile];
-    const unsigned rho = R < 0 ? 0u : (unsigned)(kTile * R + 1);
-    const uint64_t rho2 = (uint64_t)rho * rho;
+  if (g.skipEmpty && S.ctxRadius && nVisited <= kNearTargets) {
+    const unsigned rho = (unsigned)S.ctxRadius[(size_t)(ty / kTile) * g.tilesW + tx / kTile];
+    // visited targets with d2 < lim count as near (no context pixel lies that near)
+    uint64_t lim = (uint64_t)rho * rho;
+    if (lim > 0xFFFFFFFFull) lim = 0xFFFFFFFFull;
     const int ow = min(g.cw, g.tw), oh = min(g.ch, g.th);
-    for (unsigned j = tid; j < nVisited; j += LANES) {
-      const int2 q = targets[j];
+    // squared distance of visited target j, or ~0 for the target itself and points the
+    // offset table does not reach
+    auto dist2 = [&](unsigned j, int2 &q) -> uint64_t {
+      q = targets[j];
       const int dx = q.x - tx, dy = q.y - ty;
       const uint64_t d2 = (uint64_t)((int64_t)dx * dx) + (uint64_t)((int64_t)dy * dy);
-      if (d2 > 0 && d2 < rho2 && dx > -ow && dx < ow && dy > -oh && dy < oh) {
-        const unsigned slot = atomicAdd(&S.nearCnt, 1u);
-        if (slot < kNearMax) {
-          S.nearD2[slot] = (uint32_t)d2;
-          S.nearIdx[slot] = (uint32_t)(dy + oh - 1) * (uint32_t)(2 * ow - 1) + (uint32_t)(dx + ow - 1);
-          S.nearPt[slot] = q;
+      return (d2 > 0 && dx > -ow && dx < ow && dy > -oh && dy < oh) ? d2 : ~0ull;
+    };
+    // how many visited targets lie nearer than sqrt(bound) (uniform over the workgroup)
+    auto count_within = [&](uint64_t bound) -> unsigned {
+      unsigned c = 0;
+      int2 q;
+      for (unsigned j = tid; j < nVisited; j += LANES) c += dist2(j, q) < bound ? 1u : 0u;
+      __syncthreads();
+      if (tid == 0) S.nearCnt = 0;
+      __syncthreads();
+      if (c) atomicAdd(&S.nearCnt, c);
+      __syncthreads();
+      return S.nearCnt;
+    };
+    unsigned n = nVisited ? count_within(lim) : 0u;
+    bool ok = true;
+    if (n > kNearMax) {
+      // more than the slots hold: the patch takes the nearest L - 1 of them, so look for a
+      // bound that keeps between L - 1 and kNearMax (a binary search on the squared
+      // distance; none exists only when many of them share one distance)
+      ok = false;
+      uint64_t lo = 0, hi = lim;
+      while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        const unsigned c = count_within(mid);
+        if (c > kNearMax) {
+          hi = mid;
+        } else if (c + 1 < L) {
+          lo = mid;
+        } else {
+          lim = mid;
+          n = c;
+          ok = true;
+          break;
         }
       }
     }
-    __syncthreads();
-    const unsigned nNear = S.nearCnt;
-    if (nNear + 1 <= L) {
+    if (ok) {
       near = true;
-      if (tid < nNear) {
+      __syncthreads();
+      if (tid == 0) S.nearCnt = 0;
+      __syncthreads();
+      for (unsigned j = tid; j < nVisited; j += LANES) {
+        int2 q;
+        const uint64_t d2 = dist2(j, q);
+        if (d2 < lim) {
+          const unsigned slot = atomicAdd(&S.nearCnt, 1u);
+          if (slot < kNearMax) {
+            S.nearD2[slot] = (uint32_t)d2;
+            S.nearIdx[slot] = (uint32_t)(q.y - ty + oh - 1) * (uint32_t)(2 * ow - 1) + (uint32_t)(q.x - tx + ow - 1);
+            S.nearPt[slot] = q;
+          }
+        }
+      }
+      __syncthreads();
+      // ordered like the sorted offsets; the first L - 1 complete the patch
+      if (tid < n) {
         const uint32_t d2 = S.nearD2[tid], id = S.nearIdx[tid];
         unsigned rank = 0;
-        for (unsigned s = 0; s < nNear; s++) {
+        for (unsigned s = 0; s < n; s++) {
           const uint32_t e2 = S.nearD2[s], ie = S.nearIdx[s];
           rank += (e2 < d2 || (e2 == d2 && ie > id)) ? 1u : 0u;
         }
-        const int2 q = S.nearPt[tid];
-        const size_t pq = (size_t)q.y * (unsigned)g.tw + (unsigned)q.x;
-        S.off[1 + rank] = make_int2(q.x - tx, q.y - ty);
-        S.pix[1 + rank] = tword[pq];
-        S.src[1 + rank] = sourceOf[pq];
+        if (rank + 1 < L) {
+          const int2 q = S.nearPt[tid];
+          const size_t pq = (size_t)q.y * (unsigned)g.tw + (unsigned)q.x;
+          S.off[1 + rank] = make_int2(q.x - tx, q.y - ty);
+          S.pix[1 + rank] = tword[pq];
+          S.src[1 + rank] = sourceOf[pq];
+        }
       }
-      count = 1 + nNear;
+      count = 1 + min(n, L - 1);
       start = rho == 0 ? 1u : offset_index(g, S.firstIdx, rho);
       __syncthreads();
     }
@@ -803,7 +872,7 @@ __device__ __forceinline__ void load_tables(const SynthBuffers &b, VisitLDS<Word
   }
   if (threadIdx.x == 0) {
     S.tileHas = b.tileHas;
-    S.ctxRings = b.ctxRings;
+    S.ctxRadius = b.ctxRadius;
     S.firstIdx = b.firstIdx;
   }
   __syncthreads();

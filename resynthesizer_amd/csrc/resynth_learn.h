@@ -42,9 +42,11 @@ hipError_t params_to_bf16(hipStream_t s, const float *w, uint16_t *wb);
 // Transposed bf16 copies of the three weight matrices ([out x in] each, kParamCount-sized
 // scratch minus biases), for the fused embedding kernel.
 hipError_t transpose_weights(hipStream_t s, const float *w, uint16_t *wt);
-// Fused embedding of n rows (bf16 x [n x kIn]) into normalised z (bf16 and/or fp32).
+// Fused embedding of n rows (bf16 x [n x kIn]) into normalised z (bf16 and/or fp32).  The
+// persistent grid takes every compute unit (each of its workgroups fills one), or
+// cuShare permille of them, which leaves the rest to concurrent work on other streams.
 hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, const uint16_t *x, int n, uint16_t *zb,
-                           float *z);
+                           float *z, int cuShare = 0);
 hipError_t gather_rows_bf16(hipStream_t s, const uint16_t *src, int cols, const uint32_t *idx, int n, uint16_t *dst);
 hipError_t gather_spread_rows_bf16(hipStream_t s, const uint16_t *src, int cols, uint32_t nSrc, int n, uint16_t *dst);
 hipError_t hash_indices(hipStream_t s, uint32_t *idx, int n, uint32_t range, uint64_t key);

@@ -1065,10 +1065,12 @@ static int compute_units()
 }
 
 hipError_t mlp_embed_fused(hipStream_t s, const float *w, const uint16_t *wt, const uint16_t *x, int n, uint16_t *zb,
-                           float *z)
+                           float *z, int cuShare)
 {
   if (n <= 0) return hipSuccess;
-  const int blocks = std::min((n + kRowsE - 1) / kRowsE, compute_units());
+  const int cus = compute_units();
+  const int share = cuShare > 0 && cuShare < 1000 ? std::max(1, cus * cuShare / 1000) : cus;
+  const int blocks = std::min((n + kRowsE - 1) / kRowsE, share);
   hipLaunchKernelGGL(k_mlp_embed, dim3(blocks), dim3(256), 0, s, x, n, wt, w, zb, z);
   return hipGetLastError();
 }

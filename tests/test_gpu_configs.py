@@ -86,7 +86,7 @@ def test_config3_all_requests_in_one_call(serial):
 # ---------------------------------------------------------------- config 5
 
 LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
-               ramp_permille=300, first_chunk=256)
+               ramp_permille=300)
 
 
 @pytest.fixture(scope="module")
