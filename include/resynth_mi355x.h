@@ -415,6 +415,10 @@ int resynth_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, co
  * ranges [-w+1, w-1] x [-h+1, h-1]: writes (2w-1)(2h-1) (x, y) pairs to out.  Returns 0, or
  * -1 on invalid arguments.  For tests. */
 int resynth_sorted_offsets(int w, int h, int32_t *out);
+/* The prefix of those sorted offsets with x^2 + y^2 <= maxD2, the part of the table an
+ * engine run builds when no scan can reach farther: writes *count (x, y) pairs to out (room
+ * for (2w-1)(2h-1) pairs suffices).  Returns 0, or -1 on invalid arguments.  For tests. */
+int resynth_sorted_offsets_within(int w, int h, unsigned long long maxD2, int32_t *out, unsigned long long *count);
 /* The learned index's query routing, on host arrays, through the same device sort and kernel
  * the engine uses.  queries: bf16 [nq x 64]; supers: [nq] super-cluster per query (a
  * negative one counts as 0); subCentroids: bf16 [S*Q x 64], super-cluster s owning rows

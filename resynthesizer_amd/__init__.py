@@ -204,6 +204,8 @@ SIGNATURES = {
     "resynth_gemm_bf16": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "resynth_sorted_offsets": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "resynth_sorted_offsets_within": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, ctypes.c_void_p,
+                                                     ctypes.POINTER(ctypes.c_ulonglong)]),
     "resynth_learn_route": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_uint, ctypes.c_uint, ctypes.c_int, ctypes.c_void_p]),
     "resynth_learn_fine_top16": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p, ctypes.c_int,
@@ -530,12 +532,18 @@ def gemm_bf16(a_bits, b_bits, ta=False, tb=False):
     return C
 
 
-def sorted_offsets(w, h):
-    """The engine's device-built sorted offset table for a w x h offset range, (N, 2) int32."""
+def sorted_offsets(w, h, max_d2=None):
+    """The engine's device-built sorted offset table for a w x h offset range, (N, 2) int32;
+    with max_d2, its prefix of offsets with x^2 + y^2 <= max_d2."""
     out = np.zeros(((2 * w - 1) * (2 * h - 1), 2), np.int32)
-    if lib().resynth_sorted_offsets(w, h, out.ctypes.data):
-        raise ValueError("resynth_sorted_offsets rejected its arguments")
-    return out
+    if max_d2 is None:
+        if lib().resynth_sorted_offsets(w, h, out.ctypes.data):
+            raise ValueError("resynth_sorted_offsets rejected its arguments")
+        return out
+    n = ctypes.c_ulonglong(0)
+    if lib().resynth_sorted_offsets_within(w, h, int(max_d2), out.ctypes.data, ctypes.byref(n)):
+        raise ValueError("resynth_sorted_offsets_within rejected its arguments")
+    return out[:n.value]
 
 
 def route(queries_bits, supers, sub_centroids_bits, S, Q, P):

@@ -363,7 +363,8 @@ struct TmpBuf {
 // Device and pinned buffers of one engine run (grow-only, reused across calls).
 struct Workspace {
   Buf tword, cword, tflag, cflag, sourceOf, prober, targetsRM, targets, corpusPts, offsets, offTemp, lastBest,
-      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, ctxRadius, ringTemp, firstIdx, wctl,
+      tables, counters, compactTemp, perm, meanSum, image, mask, mask2, tpix, cpix, tileHas, tileFull, ctxRadius, ringTemp,
+      firstIdx, wctl,
       bandStart;
   // learned proposals: corpus keys, their embeddings and inverted file, query scratch
   Buf keyFlags, keyPts, keyTemp, keyX, keyH1, keyH2, keyE, keyZ, cent, keyCl, bCount, bStart, bCursor,
@@ -665,20 +666,21 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   HIP_TRY(launch_compact(s, cflag, cw, ch, corpusPts, dCounts + 1, ctemp + ctb));
   uint32_t *bandStart = (uint32_t *)W.bandStart.get((kBands + 1) * sizeof(uint32_t));
   HIP_TRY(launch_band_starts(s, corpusPts, dCounts + 1, ch, bandStart));
+  // how far a neighbour scan can reach (k_tile_context), read back with the counts
+  const int tilesW = (tw + kTile - 1) / kTile, tilesH = (th + kTile - 1) / kTile;
+  const size_t nTiles = (size_t)tilesW * tilesH;
+  uint8_t *tileFull = (uint8_t *)W.tileFull.get(2 * nTiles + 64);
+  int32_t *ringTemp = (int32_t *)W.ringTemp.get(nTiles * sizeof(int32_t) + 64);
+  int32_t *ctxRadius = (int32_t *)W.ctxRadius.get(nTiles * sizeof(int32_t) + 64);
+  HIP_TRY(launch_reach_radius(s, job.wide, tword, tflag, tw, th, tileFull, tileFull + nTiles, ringTemp, ctxRadius,
+                              dCounts + 2));
   unsigned *hCounts = (unsigned *)W.pinCounts.get(64);
-  HIP_TRY(hipMemcpyAsync(hCounts, dCounts, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(hCounts, dCounts, 3 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
   hipEvent_t countsReady = g_dev.countsReady;
   HIP_TRY(hipEventRecord(countsReady, s));
 
   // Work that does not depend on the counts goes to the device now and overlaps the
-  // host's target ordering: the sorted offsets (lib/engine.c:465-497), the metric tables
-  // and the state maps.
-  const int ow = std::min(cw, tw), oh = std::min(ch, th);
-  const size_t nO = (size_t)(2 * ow - 1) * (size_t)(2 * oh - 1);
-  job.st.nOffsets = (unsigned)nO;
-  int2 *offsets = (int2 *)W.offsets.get(nO * sizeof(int2));
-  size_t otb = offsets_temp_bytes(ow, oh);
-  HIP_TRY(launch_sorted_offsets(s, ow, oh, offsets, W.offTemp.get(otb), otb));
+  // host's wait: the metric tables and the state maps.
   Tables *hT = (Tables *)W.pinTables.get(sizeof(Tables));
   quantize_tables(P.sensitivityToOutliers, P.mapWeight, *hT);
   Tables *dT = (Tables *)W.tables.get(sizeof(Tables));
@@ -691,22 +693,36 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   HIP_TRY(launch_fill_u32(s, prober, 0xFFFFFFFFu, nCpix));
   HIP_TRY(launch_fill_u32(s, lastBest, 0xFFFFFFFFu, nTpix));
   HIP_TRY(launch_fill_u32(s, (uint32_t *)cnt, 0u, sizeof(Counters) / 4));
-  // occupancy of 8x8 tiles and the start of each distance in the sorted offsets, used to
-  // skip provably empty offsets in pass-0 scans
-  const int tilesW = (tw + kTile - 1) / kTile, tilesH = (th + kTile - 1) / kTile;
-  const size_t nTiles = (size_t)tilesW * tilesH;
+  // occupancy of 8x8 tiles, used to skip provably empty offsets in pass-0 scans, and the
+  // context radius of the context-only occupancy, before any visit updates it
   uint8_t *tileHas = (uint8_t *)W.tileHas.get((nTiles + 3) / 4 * 4);
   HIP_TRY(launch_fill_u32(s, (uint32_t *)tileHas, 0u, (nTiles + 3) / 4));
   HIP_TRY(launch_tile_init(s, job.wide, tword, tw, th, tileHas));
-  // context radius of the context-only occupancy, before any visit updates tileHas
-  int32_t *ctxRadius = (int32_t *)W.ctxRadius.get(nTiles * sizeof(int32_t) + 64);
-  HIP_TRY(launch_ctx_radius(s, tileHas, tilesW, tilesH, (int32_t *)W.ringTemp.get(nTiles * sizeof(int32_t) + 64),
-                            ctxRadius));
-  const unsigned rhoMax = (unsigned)std::ceil(std::sqrt((double)(ow - 1) * (ow - 1) + (double)(oh - 1) * (oh - 1))) + 1;
-  uint32_t *firstIdx = (uint32_t *)W.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
-  HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
+  HIP_TRY(launch_ctx_radius(s, tileHas, tilesW, tilesH, ringTemp, ctxRadius));
 
   HIP_TRY(hipEventSynchronize(countsReady));
+  // The sorted offsets (lib/engine.c:465-497), as far as a scan can reach: every target's
+  // scan gathers its patch before passing an offset farther than reach + 21 from it
+  // (k_tile_context), so the table's prefix up to that distance is all any visit reads.
+  // Wrapped neighbourhoods (tiling) and a corpus narrower or shorter than the target (a
+  // table that does not span the image) take the whole table.
+  const int ow = std::min(cw, tw), oh = std::min(ch, th);
+  const unsigned reach = hCounts[2];
+  uint64_t maxD2 = ~0ull;
+  if (!P.isMakeSeamlesslyTileableHorizontally && !P.isMakeSeamlesslyTileableVertically && ow == tw && oh == th &&
+      reach < (1u << 30))
+    maxD2 = (uint64_t)(reach + 21) * (reach + 21);
+  const OffsetPlan plan = plan_offsets(ow, oh, maxD2);
+  const size_t nO = plan.nO;
+  job.st.nOffsets = (unsigned)nO;
+  int2 *offsets = (int2 *)W.offsets.get(nO * sizeof(int2) + 64);
+  const size_t otb = offsets_temp_bytes(plan);
+  HIP_TRY(launch_sorted_offsets(s, plan, offsets, W.offTemp.get(otb), otb));
+  // the start of each distance in the sorted offsets, up to the largest one the table holds
+  const unsigned rhoFar = (unsigned)std::ceil(std::sqrt((double)(ow - 1) * (ow - 1) + (double)(oh - 1) * (oh - 1))) + 1;
+  const unsigned rhoMax = plan.whole ? rhoFar : std::min(rhoFar, (unsigned)std::sqrt((double)plan.lim2));
+  uint32_t *firstIdx = (uint32_t *)W.firstIdx.get(((size_t)rhoMax + 1) * sizeof(uint32_t));
+  HIP_TRY(launch_first_idx(s, ow, oh, rhoMax, firstIdx));
   const unsigned nT = hCounts[0], nC = hCounts[1];
   job.st.nTargets = nT;
   job.st.nCorpus = nC;
@@ -902,8 +918,8 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   uint16_t *x = (uint16_t *)W.keyX.get((size_t)kKeyBlock * kIn * 2);
   for (unsigned b0 = 0; b0 < nK; b0 += kKeyBlock) {
     const int nb = (int)std::min<unsigned>(kKeyBlock, nK - b0);
-    HIP_TRY(learn_features(s, job.wide, pr.b.cword, g.cw, g.ch, 0, 0, g.nColors, pr.b.offsets, L.keyPts + b0, nb, true,
-                           true, 0, x));
+    HIP_TRY(learn_features(s, job.wide, pr.b.cword, g.cw, g.ch, 0, 0, g.nColors, pr.b.offsets, g.nO, L.keyPts + b0, nb,
+                           true, true, 0, x));
     HIP_TRY(mlp_embed_fused(s, w, wt, x, nb, L.keyZ + (size_t)b0 * kEmb, nullptr, index_cu_share()));
   }
   const unsigned Q = 128;
@@ -986,7 +1002,7 @@ void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, 
   uint16_t *z = (uint16_t *)W.qZ.get((size_t)n * kEmb * 2 + 64);
   int32_t *qcl = (int32_t *)W.qCl.get((size_t)n * 2 * 4 + 64);
   HIP_TRY(learn_features(s, job.wide, pr.b.tword, g.tw, g.th, g.tileH, g.tileV, g.nColors, pr.b.offsets,
-                         pr.b.targets + k0, n, false, false, 0, x));
+                         g.nO, pr.b.targets + k0, n, false, false, 0, x));
   HIP_TRY(mlp_embed_fused(s, model_weights(job.opt.model), model_weights_t(job.opt.model, s), x, n, z, nullptr));
   int32_t *qs = (int32_t *)W.qSuper.get((size_t)n * 4 + 64);
   HIP_TRY(learn_topk_cols(s, z, n, L.superCent, (int)L.S, 1, qs));
@@ -1278,7 +1294,7 @@ int refine_engine(Job &job, Prepared &pr)
     Counters hc;
     unsigned long long *dSum = (unsigned long long *)W.meanSum.get(16);
     HIP_TRY(launch_fill_u32(s, (uint32_t *)dSum, 0u, 4));
-    HIP_TRY(launch_mean_error(s, lastBest, targets, nT, tw, dSum));
+    HIP_TRY(launch_mean_error(s, lastBest, pr.targetsRM ? pr.targetsRM : targets, nT, tw, dSum));
     unsigned long long hs[2];
     HIP_TRY(hipMemcpyAsync(hs, dSum, 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
@@ -1745,7 +1761,7 @@ int run_batch(ResynthRequest *reqs, unsigned n, bool device, const ResynthOption
     Workspace &W = *job.W;
     unsigned long long *dSum = (unsigned long long *)W.meanSum.get(16);
     HIP_TRY(launch_fill_u32(s, (uint32_t *)dSum, 0u, 4));
-    HIP_TRY(launch_mean_error(s, pr.b.lastBest, pr.b.targets, pr.nT, pr.g.tw, dSum));
+    HIP_TRY(launch_mean_error(s, pr.b.lastBest, pr.targetsRM ? pr.targetsRM : pr.b.targets, pr.nT, pr.g.tw, dSum));
     HIP_TRY(hipMemcpyAsync(&sums[2 * i], dSum, 16, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(&hc[i], pr.b.counters, sizeof(Counters), hipMemcpyDeviceToHost, s));
     const Source &S = job.src;
@@ -2129,10 +2145,10 @@ int resynth_model_log_pairs(ResynthModel *model, ImageBuffer *image, ImageBuffer
     uint16_t *pq, *pk;
     model_reserve_pairs(model, n, &pq, &pk);
     HIP_TRY(learn_features(s, job.wide, W.tword.p, tw, th, p.isMakeSeamlesslyTileableHorizontally,
-                           p.isMakeSeamlesslyTileableVertically, job.nColors, (const int2 *)W.offsets.p, pts, (int)n,
-                           false, false, key + 1, pq));
+                           p.isMakeSeamlesslyTileableVertically, job.nColors, (const int2 *)W.offsets.p, job.st.nOffsets,
+                           pts, (int)n, false, false, key + 1, pq));
     HIP_TRY(learn_features(s, job.wide, W.cword.p, job.src.cw, job.src.ch, 0, 0, job.nColors,
-                           (const int2 *)W.offsets.p, src, (int)n, true, false, 0, pk));
+                           (const int2 *)W.offsets.p, job.st.nOffsets, src, (int)n, true, false, 0, pk));
     HIP_TRY(hipStreamSynchronize(s));
     model_commit_pairs(model, n);
   };
@@ -2277,19 +2293,26 @@ int resynth_learn_fine_top16(const uint16_t *queries, unsigned int nq, const int
 
 int resynth_sorted_offsets(int w, int h, int32_t *out)
 {
-  if (w < 1 || h < 1 || !out || (size_t)w * (size_t)h > (1u << 28)) return -1;
+  unsigned long long n = 0;
+  return resynth_sorted_offsets_within(w, h, ~0ull, out, &n);
+}
+
+int resynth_sorted_offsets_within(int w, int h, unsigned long long maxD2, int32_t *out, unsigned long long *count)
+{
+  if (w < 1 || h < 1 || !out || !count || (size_t)w * (size_t)h > (1u << 28)) return -1;
   return guarded([&]() -> int {
     std::lock_guard<std::mutex> lock(g_mutex);
     if (!ensure_device()) return RESYNTH_ERROR_DEVICE;
     hipStream_t s = g_dev.stream;
-    const size_t nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
-    const size_t tb = offsets_temp_bytes(w, h);
+    const OffsetPlan plan = plan_offsets(w, h, maxD2);
+    const size_t tb = offsets_temp_bytes(plan);
     TmpBuf dout, dtemp;
-    HIP_TRY(dout.alloc(nO * sizeof(int2)));
+    HIP_TRY(dout.alloc(plan.nO * sizeof(int2)));
     HIP_TRY(dtemp.alloc(tb));
-    HIP_TRY(launch_sorted_offsets(s, w, h, (int2 *)dout.p, dtemp.p, tb));
-    HIP_TRY(hipMemcpyAsync(out, dout.p, nO * sizeof(int2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(launch_sorted_offsets(s, plan, (int2 *)dout.p, dtemp.p, tb));
+    HIP_TRY(hipMemcpyAsync(out, dout.p, plan.nO * sizeof(int2), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    *count = plan.nO;
     return 0;
   });
 }

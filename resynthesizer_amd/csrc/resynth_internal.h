@@ -145,14 +145,27 @@ hipError_t launch_adapt_maps(hipStream_t s, bool wide, const unsigned char *pix,
 size_t compact_temp_bytes(size_t n);
 hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int2 *out, unsigned *count,
                           void *temp);
-size_t offsets_temp_bytes(int w, int h);
+// How much of the sorted offsets to build: the prefix with x^2 + y^2 <= lim2 (plan_offsets).
+struct OffsetPlan {
+  int w = 0, h = 0;        // offsets span [-w+1, w-1] x [-h+1, h-1]
+  int qw = 0, qh = 0;      // quadrant corner sorted
+  uint32_t lim2 = 0;       // largest x^2 + y^2 kept
+  size_t nQ = 0;           // quadrant entries sorted (qw * qh)
+  size_t nKeep = 0;        // of them inside the disc
+  size_t nO = 0;           // offsets in the table
+  bool whole = true;       // the whole table
+};
+OffsetPlan plan_offsets(int w, int h, uint64_t maxD2);
+size_t offsets_temp_bytes(const OffsetPlan &p);
 size_t sort_pairs_temp_bytes(size_t n);
 hipError_t launch_sort_pairs(hipStream_t s, const uint32_t *kIn, uint32_t *kOut, const uint32_t *vIn, uint32_t *vOut,
                              size_t n, unsigned bits, void *temp, size_t tempBytes);
-hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes);
+hipError_t launch_sorted_offsets(hipStream_t s, const OffsetPlan &p, int2 *out, void *temp, size_t tempBytes);
 hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 hipError_t launch_band_starts(hipStream_t s, const int2 *corpusPts, const unsigned *nC, int ch, uint32_t *out);
 hipError_t launch_ctx_radius(hipStream_t s, const uint8_t *mark, int W, int H, int32_t *scratch, int32_t *radius);
+hipError_t launch_reach_radius(hipStream_t s, bool wide, const void *tword, const uint8_t *tflag, int tw, int th,
+                               uint8_t *full, uint8_t *hasTarget, int32_t *scratch, int32_t *rhoFull, uint32_t *reach);
 hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas);
 hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx);
 hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);

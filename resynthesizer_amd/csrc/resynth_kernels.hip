@@ -11,6 +11,7 @@
 #include "resynth_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -297,6 +298,67 @@ hipError_t launch_ctx_radius(hipStream_t s, const uint8_t *mark, int W, int H, i
 {
   hipLaunchKernelGGL(k_ring_rows, dim3((W * H + 255) / 256), dim3(256), 0, s, mark, W, H, scratch);
   hipLaunchKernelGGL(k_ctx_radius, dim3((W * H + 255) / 256), dim3(256), 0, s, scratch, W, H, radius);
+  return hipGetLastError();
+}
+
+// How far any target's neighbour scan can reach, from the context at the start of a run.
+// A tile is full when all kTile^2 of its pixels lie in the image and hold a value (context
+// pixels: they keep their values for the whole run).  For a pixel p of a tile whose box lies
+// rho (k_ctx_radius over the full tiles) from the nearest full tile's box, every pixel of
+// that full tile lies within rho + 1 + 2 (kTile - 1) sqrt 2 < rho + 21 of p: it holds
+// kTile^2 = 64 valued pixels, at least the patch's L - 1 <= 63 neighbours, so p's scan
+// ends before any offset farther than rho + 21.  reach = the largest rho over the tiles
+// that hold a target (0 when none does; 2^30 when no tile is full).
+template <typename Word>
+__global__ void k_tile_context(const Word *tword, const uint8_t *tflag, int tw, int th, int tilesW, int tilesH,
+                               uint8_t *full, uint8_t *hasTarget)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= tilesW * tilesH) return;
+  const int x0 = (t % tilesW) * kTile, y0 = (t / tilesW) * kTile;
+  bool all = x0 + kTile <= tw && y0 + kTile <= th, any = false;
+  for (int dy = 0; dy < kTile && y0 + dy < th; dy++)
+    for (int dx = 0; dx < kTile && x0 + dx < tw; dx++) {
+      const size_t p = (size_t)(y0 + dy) * (unsigned)tw + (unsigned)(x0 + dx);
+      all = all && (tword[p] & 1u);
+      any = any || tflag[p];
+    }
+  full[t] = all ? 1 : 0;
+  hasTarget[t] = any ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_reach_max(const int32_t *rho, const uint8_t *hasTarget, unsigned n,
+                                                   uint32_t *reach)
+{
+  __shared__ uint32_t part[4];
+  uint32_t m = 0;
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (hasTarget[i]) m = max(m, (uint32_t)rho[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o));
+  if (lane_id() == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = max(max(part[0], part[1]), max(part[2], part[3]));
+    if (m) atomicMax(reach, m);
+  }
+}
+
+hipError_t launch_reach_radius(hipStream_t s, bool wide, const void *tword, const uint8_t *tflag, int tw, int th,
+                               uint8_t *full, uint8_t *hasTarget, int32_t *scratch, int32_t *rhoFull, uint32_t *reach)
+{
+  const int tilesW = (tw + kTile - 1) / kTile, tilesH = (th + kTile - 1) / kTile;
+  const unsigned nTiles = (unsigned)(tilesW * tilesH), blocks = (nTiles + 255) / 256;
+  hipError_t e = launch_fill_u32(s, reach, 0u, 1);
+  if (e != hipSuccess || !nTiles) return e;
+  if (wide)
+    hipLaunchKernelGGL((k_tile_context<uint64_t>), dim3(blocks), dim3(256), 0, s, (const uint64_t *)tword, tflag, tw,
+                       th, tilesW, tilesH, full, hasTarget);
+  else
+    hipLaunchKernelGGL((k_tile_context<uint32_t>), dim3(blocks), dim3(256), 0, s, (const uint32_t *)tword, tflag, tw,
+                       th, tilesW, tilesH, full, hasTarget);
+  e = launch_ctx_radius(s, full, tilesW, tilesH, scratch, rhoFull);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reach_max, dim3(std::min(blocks, 1024u)), dim3(256), 0, s, rhoFull, hasTarget, nTiles, reach);
   return hipGetLastError();
 }
 
@@ -2327,15 +2389,18 @@ hipError_t launch_compact(hipStream_t s, const uint8_t *flags, int w, int h, int
 //   the y = 0 row, (a, 0) then (-a, 0);
 //   y = -b rows in the reverse order, each (a, -b) then (-a, -b).
 
-// Quadrant entry q = b*w + a, generated in descending q so that a stable sort by the key
-// leaves ties in descending q, i.e. descending b.  The sorted payload is (b << shift) | a
-// (shift = bits of w - 1; it fits, as w*h < 2^31), so the later kernels need no division.
-__global__ void k_quad_keys(int w, unsigned shift, uint32_t *keys, uint32_t *vals, size_t nQ)
+// Quadrant entry q = b*qw + a of the quadrant's [0, qw) x [0, qh) corner, generated in
+// descending q so that a stable sort by the key leaves ties in descending q, i.e. descending
+// b.  The sorted payload is (b << shift) | a (shift = bits of qw - 1; it fits, as w*h <
+// 2^31), so the later kernels need no division.  Keys above lim2 (a table cut at x^2 + y^2
+// <= lim2) become lim2 + 1 and sort after every kept entry.
+__global__ void k_quad_keys(int qw, unsigned shift, uint32_t lim2, uint32_t *keys, uint32_t *vals, size_t nQ)
 {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < nQ; e += (size_t)gridDim.x * blockDim.x) {
     const uint32_t q = (uint32_t)(nQ - 1 - e);
-    const uint32_t a = q % (unsigned)w, b = q / (unsigned)w;
-    keys[e] = a * a + b * b;
+    const uint32_t a = q % (unsigned)qw, b = q / (unsigned)qw;
+    const uint32_t d2 = a * a + b * b;
+    keys[e] = d2 > lim2 ? lim2 + 1u : d2;
     vals[e] = (b << shift) | a;
   }
 }
@@ -2392,9 +2457,51 @@ __global__ void k_quad_expand(unsigned shift, const uint32_t *keys, const uint32
   }
 }
 
-static unsigned key_bits(int w, int h)
+static uint64_t isqrt_u64(uint64_t v)
 {
-  uint64_t mx = (uint64_t)(w - 1) * (w - 1) + (uint64_t)(h - 1) * (h - 1);
+  uint64_t r = (uint64_t)std::sqrt((double)v);
+  while (r * r > v) r--;
+  while ((r + 1) * (r + 1) <= v) r++;
+  return r;
+}
+
+// The sorted offsets of [-w+1, w-1] x [-h+1, h-1] with x^2 + y^2 <= maxD2: the table's first
+// nO entries (the sort key is x^2 + y^2), built from the quadrant's [0, qw) x [0, qh) corner
+// and its nKeep entries inside the disc.  maxD2 at or above (w-1)^2 + (h-1)^2 is the whole
+// table.
+OffsetPlan plan_offsets(int w, int h, uint64_t maxD2)
+{
+  OffsetPlan p;
+  p.w = w;
+  p.h = h;
+  const uint64_t far2 = (uint64_t)(w - 1) * (w - 1) + (uint64_t)(h - 1) * (h - 1);
+  if (maxD2 >= far2) {
+    p.qw = w;
+    p.qh = h;
+    p.lim2 = (uint32_t)far2;
+    p.nQ = p.nKeep = (size_t)w * h;
+    p.nO = (size_t)(2 * w - 1) * (size_t)(2 * h - 1);
+    p.whole = true;
+    return p;
+  }
+  const uint64_t R = isqrt_u64(maxD2);
+  p.qw = (int)std::min<uint64_t>((uint64_t)w, R + 1);
+  p.qh = (int)std::min<uint64_t>((uint64_t)h, R + 1);
+  p.lim2 = (uint32_t)maxD2;
+  p.nQ = (size_t)p.qw * p.qh;
+  p.nKeep = 0;
+  p.nO = 0;
+  for (int a = 0; a < p.qw; a++) {
+    const uint64_t bmax = std::min<uint64_t>((uint64_t)(h - 1), isqrt_u64(maxD2 - (uint64_t)a * a));
+    p.nKeep += bmax + 1;
+    p.nO += (a ? 2 : 1) * (2 * bmax + 1);
+  }
+  p.whole = false;
+  return p;
+}
+
+static unsigned bits_of(uint64_t mx)
+{
   unsigned b = 1;
   while (b < 32 && (mx >> b)) b++;
   return b;
@@ -2413,19 +2520,18 @@ static size_t quad_scan_bytes(size_t nQ)
 // state in the work area and need it aligned, whatever w*h is.
 static size_t quad_array_bytes(size_t nQ) { return (nQ * sizeof(uint32_t) + 255) & ~(size_t)255; }
 
-size_t offsets_temp_bytes(int w, int h)
+size_t offsets_temp_bytes(const OffsetPlan &p)
 {
-  const size_t nQ = (size_t)w * (size_t)h;
   size_t sortBytes = 0;
   (void)rocprim::radix_sort_pairs<rocprim::default_config>(nullptr, sortBytes, (uint32_t *)nullptr,
                                                            (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                                           (uint32_t *)nullptr, nQ, 0, 32);
-  return 5 * quad_array_bytes(nQ) + std::max(sortBytes, quad_scan_bytes(nQ)) + 1024;
+                                                           (uint32_t *)nullptr, p.nQ, 0, 32);
+  return 5 * quad_array_bytes(p.nQ) + std::max(sortBytes, quad_scan_bytes(p.nKeep)) + 1024;
 }
 
-hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *temp, size_t tempBytes)
+hipError_t launch_sorted_offsets(hipStream_t s, const OffsetPlan &p, int2 *out, void *temp, size_t tempBytes)
 {
-  const size_t nQ = (size_t)w * (size_t)h, A = quad_array_bytes(nQ);
+  const size_t nQ = p.nQ, A = quad_array_bytes(nQ);
   char *base = (char *)temp;
   uint32_t *k0 = (uint32_t *)base, *v0 = (uint32_t *)(base + A), *k1 = (uint32_t *)(base + 2 * A),
            *v1 = (uint32_t *)(base + 3 * A), *first = (uint32_t *)(base + 4 * A);
@@ -2434,15 +2540,19 @@ hipError_t launch_sorted_offsets(hipStream_t s, int w, int h, int2 *out, void *t
   unsigned blocks = (unsigned)((nQ + 255) / 256);
   if (blocks > 65536) blocks = 65536;
   unsigned shift = 1;
-  while (shift < 31 && ((unsigned)(w - 1) >> shift)) shift++;   // bits of w - 1 (at least 1)
-  hipLaunchKernelGGL(k_quad_keys, dim3(blocks), dim3(256), 0, s, w, shift, k0, v0, nQ);
+  while (shift < 31 && ((unsigned)(p.qw - 1) >> shift)) shift++;   // bits of qw - 1 (at least 1)
+  hipLaunchKernelGGL(k_quad_keys, dim3(blocks), dim3(256), 0, s, p.qw, shift, p.lim2, k0, v0, nQ);
   hipError_t e = rocprim::radix_sort_pairs<rocprim::default_config>(work, workBytes, k0, k1, v0, v1, nQ, 0,
-                                                                    key_bits(w, h), s);
+                                                                    bits_of(p.whole ? p.lim2 : (uint64_t)p.lim2 + 1), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_quad_members, dim3(blocks), dim3(256), 0, s, shift, v1, k0, nQ);
-  e = rocprim::exclusive_scan(work, workBytes, k0, first, 0u, nQ, rocprim::plus<uint32_t>(), s);
+  // the kept entries are the sorted prefix [0, nKeep): whole rings, as a ring's entries share one key
+  const size_t nK = p.nKeep;
+  unsigned kblocks = (unsigned)std::min<size_t>((nK + 255) / 256, 65536);
+  if (!kblocks) return hipGetLastError();
+  hipLaunchKernelGGL(k_quad_members, dim3(kblocks), dim3(256), 0, s, shift, v1, k0, nK);
+  e = rocprim::exclusive_scan(work, workBytes, k0, first, 0u, nK, rocprim::plus<uint32_t>(), s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_quad_expand, dim3(blocks), dim3(256), 0, s, shift, k1, v1, first, nQ, out);
+  hipLaunchKernelGGL(k_quad_expand, dim3(kblocks), dim3(256), 0, s, shift, k1, v1, first, nK, out);
   return hipGetLastError();
 }
 
@@ -2637,9 +2747,13 @@ hipError_t launch_writeback_pixmap(hipStream_t s, bool wide, const void *tword, 
   return hipGetLastError();
 }
 
-__global__ void k_mean_error(const uint32_t *lastBest, const int2 *targets, unsigned nT, int w,
-                             unsigned long long *sumCount)
+// Sum and count of the final match errors over the targets.  Callers pass the targets in
+// row-major order, so that the loads of a wavefront fall on a few rows; each workgroup
+// reduces in LDS and adds once (one pair of 64-bit atomics per workgroup, not per wavefront).
+__global__ __launch_bounds__(256) void k_mean_error(const uint32_t *lastBest, const int2 *targets, unsigned nT, int w,
+                                                    unsigned long long *sumCount)
 {
+  __shared__ unsigned long long part[2][4];
   unsigned long long s = 0, c = 0;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < nT; i += gridDim.x * blockDim.x) {
     int2 t = targets[i];
@@ -2653,17 +2767,31 @@ __global__ void k_mean_error(const uint32_t *lastBest, const int2 *targets, unsi
     s += __shfl_xor(s, o);
     c += __shfl_xor(c, o);
   }
-  if (lane_id() == 0 && c) {
-    atomicAdd(&sumCount[0], s);
-    atomicAdd(&sumCount[1], c);
+  const unsigned wave = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    part[0][wave] = s;
+    part[1][wave] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nw = blockDim.x >> 6;
+    unsigned long long bs = 0, bc = 0;
+    for (unsigned k = 0; k < nw; k++) {
+      bs += part[0][k];
+      bc += part[1][k];
+    }
+    if (bc) {
+      atomicAdd(&sumCount[0], bs);
+      atomicAdd(&sumCount[1], bc);
+    }
   }
 }
 
 hipError_t launch_mean_error(hipStream_t s, const uint32_t *lastBest, const int2 *targets, unsigned nT, int w,
                              unsigned long long *sumCount)
 {
-  unsigned blocks = (nT + 255) / 256;
-  if (blocks > 1024) blocks = 1024;
+  unsigned blocks = (nT + 1023) / 1024;
+  if (blocks > 512) blocks = 512;
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_mean_error, dim3(blocks), dim3(256), 0, s, lastBest, targets, nT, w, sumCount);
   return hipGetLastError();

@@ -72,8 +72,8 @@ hipStream_t model_stream(const ResynthModel *M);
 // (half-resolution key grid).  dropKey != 0: each neighbour of a target point is dropped
 // with a per-point probability drawn from the hash of (dropKey, point).
 hipError_t learn_features(hipStream_t s, bool wide, const void *words, int w, int h, int tileH, int tileV, int nColors,
-                          const int2 *offsets, const int2 *pts, int n, bool isCorpus, bool scale2, uint64_t dropKey,
-                          uint16_t *out);
+                          const int2 *offsets, unsigned nOffsets, const int2 *pts, int n, bool isCorpus, bool scale2,
+                          uint64_t dropKey, uint16_t *out);
 // Flags of the half-resolution key grid: 1 where (2x, 2y) is a corpus point.
 hipError_t learn_key_flags(hipStream_t s, bool wide, const void *cword, int cw, int ch, uint8_t *flags);
 // Top-P (P = 1 or 2) columns by dot product for every row: rows [M x 64] and cols [N x 64]

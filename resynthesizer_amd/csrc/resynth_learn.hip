@@ -477,14 +477,14 @@ namespace rs {
 
 template <typename Word>
 __global__ __launch_bounds__(256) void k_features(const Word *words, int w, int h, int tileH, int tileV, int nColors,
-                                                  const int2 *offsets, const int2 *pts, int n, int isCorpus,
-                                                  int scale2, uint64_t dropKey, uint16_t *out)
+                                                  const int2 *offsets, unsigned nOffsets, const int2 *pts, int n,
+                                                  int isCorpus, int scale2, uint64_t dropKey, uint16_t *out)
 {
   const int i = blockIdx.x * 8 + (threadIdx.x >> 5);
   const int j = threadIdx.x & 31;
   if (i >= n) return;
   uint16_t v[4] = {0, 0, 0, 0};
-  if (j < kNbr && j > 0) {
+  if (j < kNbr && j > 0 && (unsigned)j < nOffsets) {
     int2 p = pts[i];
     if (scale2) {
       p.x *= 2;
@@ -520,17 +520,17 @@ __global__ __launch_bounds__(256) void k_features(const Word *words, int w, int 
 }
 
 hipError_t learn_features(hipStream_t s, bool wide, const void *words, int w, int h, int tileH, int tileV, int nColors,
-                          const int2 *offsets, const int2 *pts, int n, bool isCorpus, bool scale2, uint64_t dropKey,
-                          uint16_t *out)
+                          const int2 *offsets, unsigned nOffsets, const int2 *pts, int n, bool isCorpus, bool scale2,
+                          uint64_t dropKey, uint16_t *out)
 {
   if (n <= 0) return hipSuccess;
   dim3 grid((n + 7) / 8);
   if (wide)
     hipLaunchKernelGGL(k_features<uint64_t>, grid, dim3(256), 0, s, (const uint64_t *)words, w, h, tileH, tileV,
-                       nColors, offsets, pts, n, (int)isCorpus, (int)scale2, dropKey, out);
+                       nColors, offsets, nOffsets, pts, n, (int)isCorpus, (int)scale2, dropKey, out);
   else
     hipLaunchKernelGGL(k_features<uint32_t>, grid, dim3(256), 0, s, (const uint32_t *)words, w, h, tileH, tileV,
-                       nColors, offsets, pts, n, (int)isCorpus, (int)scale2, dropKey, out);
+                       nColors, offsets, nOffsets, pts, n, (int)isCorpus, (int)scale2, dropKey, out);
   return hipGetLastError();
 }
 

@@ -41,6 +41,8 @@ def test_config2_serial_equals_the_oracle_at_full_size(serial):
     assert r.passes == g0["passes"] and list(r.betters)[:r.passes] == g0["betters"]
     assert hashlib.sha256(out.tobytes()).hexdigest() == g0["sha256"]
     assert abs(r.meanError - g0["meanError"]) <= 1e-9 * g0["meanError"]
+    # the run built only the part of the sorted offsets its scans can reach
+    assert r.nOffsets < (2 * 2048 - 1) ** 2 // 10, r.nOffsets
 
 
 def test_config2_batched_within_two_percent_of_serial(serial):

@@ -52,6 +52,22 @@ def test_device_sorted_offsets_closed_form(wh):
     assert np.array_equal(R.sorted_offsets(w, h), pts[order])
 
 
+@pytest.mark.parametrize("whd", [(1, 1, 0), (7, 5, 0), (7, 5, 1), (7, 5, 2), (7, 5, 8), (33, 17, 100), (33, 17, 1300),
+                                 (33, 17, 1279), (5, 300, 100), (300, 5, 2500), (257, 190, 4097), (1023, 1001, 40000)])
+def test_device_sorted_offsets_prefix_within_a_distance(whd):
+    """The table cut at x^2 + y^2 <= maxD2, as an engine run builds it when no scan can reach
+    farther, is exactly the whole table's prefix of offsets within that distance (cuts
+    narrower than the range in x or y, between two distances, and at or past the farthest
+    offset, which is the whole table)."""
+    w, h, d2 = whd
+    full = R.sorted_offsets(w, h)
+    dist = full[:, 0].astype(np.int64) ** 2 + full[:, 1].astype(np.int64) ** 2
+    n = int((dist <= d2).sum())
+    assert np.all(dist[:n] <= d2) and np.all(dist[n:] > d2)
+    got = R.sorted_offsets(w, h, max_d2=d2)
+    assert got.shape == (n, 2) and np.array_equal(got, full[:n])
+
+
 # ---------------------------------------------------------------- frozen-state scorer
 
 @pytest.mark.parametrize("layout", ["rgb", "gray", "rgb_maps3", "gray_map1", "rgba"])
