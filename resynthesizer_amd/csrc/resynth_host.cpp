@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -374,7 +375,7 @@ struct Workspace {
   // serial mode: raw PRNG outputs and the stream state
   Buf streamBuf, streamState;
   // target ordering of types 2..8
-  Buf orderRays, orderTemp, orderKeys, orderKeys2, orderVals;
+  Buf orderRays, orderTemp;
   PinnedBuf pinCounts, pinTables, pinPerm, pinStream;
 };
 
@@ -749,19 +750,14 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   // them.  The allocation-free entry draws them inline.
   // The batched mode is judged by its mean match error, not by the reference's visiting
   // order: its random order (types 0/1) is a permutation drawn on the device from the probe
-  // seed, which keeps the host shuffle (about 3 ns per target) off the critical path.
-  pr.dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
+  // seed (k_random_order), which keeps the host shuffle (about 3 ns per target) off the
+  // critical path.
   pr.targetsRM = targetsRM;
   if (!serialMode && type <= 1) {
-    uint32_t *k0 = (uint32_t *)W.orderKeys.get((size_t)nT * sizeof(uint32_t) + 64);
-    uint32_t *k1 = (uint32_t *)W.orderKeys2.get((size_t)nT * sizeof(uint32_t) + 64);
-    uint32_t *v0 = (uint32_t *)W.orderVals.get((size_t)nT * sizeof(uint32_t) + 64);
-    const size_t tb = sort_pairs_temp_bytes(nT);
-    HIP_TRY(launch_random_order_keys(s, (job.opt.seed ? job.opt.seed : kEngineSeed) ^ 0x5EED0DE5u, nT, k0, v0));
-    HIP_TRY(launch_sort_pairs(s, k0, k1, v0, pr.dPerm, nT, 32, W.orderTemp.get(tb + 64), tb));
-    HIP_TRY(launch_gather_targets(s, targetsRM, pr.dPerm, targets, nT));
+    HIP_TRY(launch_random_order(s, (job.opt.seed ? job.opt.seed : kEngineSeed) ^ 0x5EED0DE5u, nT, targetsRM, targets));
     pr.orderPending = false;
   } else {
+    pr.dPerm = (uint32_t *)W.perm.get((size_t)nT * sizeof(uint32_t) + 64);
     pr.hPerm = (uint32_t *)W.pinPerm.get((size_t)nT * sizeof(uint32_t) + 64);
     if (type >= 2) {
       static uint32_t hRays[9];
@@ -858,6 +854,12 @@ constexpr int kKeyBlock = 1 << 20;   // corpus keys embedded per block (256 B of
 struct LearnedIndex {
   bool ok = false;
   bool pending = false;   // built on the side stream; the library stream has not waited yet
+  std::thread builder;    // enqueues the build on the side stream (learned_build)
+  std::exception_ptr builderError;
+  ~LearnedIndex()
+  {
+    if (builder.joinable()) builder.join();
+  }
   unsigned nK = 0, C = 0, S = 0, Q = 0;
   int2 *keyPts = nullptr, *keyPtsS = nullptr;      // keyPtsS: in cluster order
   uint16_t *keyZ = nullptr, *keyZs = nullptr;      // keyZs: embeddings in cluster order
@@ -888,7 +890,7 @@ int index_cu_share()
 
 // Two-level inverted file over the corpus keys: S super-centroids (every (nK/S)-th key),
 // Q = 128 sub-centroids inside each, about 240 keys per cluster.
-void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
+void learned_build(Job &job, const Prepared &pr, LearnedIndex &L, hipEvent_t builtEvent)
 {
   hipStream_t s = g_dev.stream;
   Workspace &W = *job.W;
@@ -910,18 +912,14 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   // From here the build reads only the corpus words, which synthesis never writes, and
   // buffers of its own: it runs on the side stream, overlapping pass 0 (whose early chunks
   // leave most compute units idle), and the first proposal waits for indexReady.  The
-  // synchronization above has completed everything it depends on.
+  // synchronization above has completed everything it depends on.  Its buffers are sized
+  // here; a helper thread enqueues its kernels, so that the host goes on to enqueue pass 0
+  // at once rather than after the build's some fifty launches.
   s = g_dev.side;
   const float *w = model_weights(M);
   const uint16_t *wt = model_weights_t(M, s);
   L.keyZ = (uint16_t *)W.keyZ.get((size_t)nK * kEmb * 2 + 64);
   uint16_t *x = (uint16_t *)W.keyX.get((size_t)kKeyBlock * kIn * 2);
-  for (unsigned b0 = 0; b0 < nK; b0 += kKeyBlock) {
-    const int nb = (int)std::min<unsigned>(kKeyBlock, nK - b0);
-    HIP_TRY(learn_features(s, job.wide, pr.b.cword, g.cw, g.ch, 0, 0, g.nColors, pr.b.offsets, g.nO, L.keyPts + b0, nb,
-                           true, true, 0, x));
-    HIP_TRY(mlp_embed_fused(s, w, wt, x, nb, L.keyZ + (size_t)b0 * kEmb, nullptr, index_cu_share()));
-  }
   const unsigned Q = 128;
   const unsigned S = std::max(1u, std::min(128u, nK / (Q * 240u)));
   L.S = S;
@@ -930,38 +928,64 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L)
   // super-centroids: keys c * nK / S, c < S (gathered on the device: a copy from pageable
   // host memory would hold the host until the side stream reached it)
   L.superCent = (uint16_t *)W.superCent.get((size_t)S * kEmb * 2 + 64);
-  HIP_TRY(gather_spread_rows_bf16(s, L.keyZ, kEmb, nK, (int)S, L.superCent));
   int32_t *sid = (int32_t *)W.superId.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_topk_cols(s, L.keyZ, (int)nK, L.superCent, (int)S, 1, sid));
   // keys sorted by super-cluster, then by cluster (radix sorts: no atomics on hot counters)
   uint32_t *k0 = (uint32_t *)W.sortK0.get((size_t)nK * 4 + 64), *v0 = (uint32_t *)W.sortV0.get((size_t)nK * 4 + 64);
   uint32_t *k1 = (uint32_t *)W.sortK1.get((size_t)nK * 4 + 64), *v1 = (uint32_t *)W.sortV1.get((size_t)nK * 4 + 64);
   const size_t stb = sort_pairs_temp_bytes(nK);
   void *stemp = W.sortTemp.get(stb);
-  HIP_TRY(learn_sort_keys(s, sid, 1, (int)nK, k0, v0));
-  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, nK, bits_for(S), stemp, stb));
   uint32_t *startS = (uint32_t *)W.sStart.get(((size_t)S + 65) * 4);
-  HIP_TRY(learn_bucket_starts(s, k1, (int)nK, (int)S, startS));
   // sub-centroids and final clusters
   L.subCent = (uint16_t *)W.subCent.get((size_t)L.C * kEmb * 2 + 64);
   int32_t *cl = (int32_t *)W.keyCl.get((size_t)nK * 4 + 64);
-  HIP_TRY(learn_sub_index(s, startS, v1, (int)S, (int)Q, (int)nK, L.keyZ, L.subCent,
-                          (uint32_t *)W.tileStart.get(((size_t)S + 65) * 4), cl));
-  HIP_TRY(learn_sort_keys(s, cl, 1, (int)nK, k0, v0));
-  HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, nK, bits_for(L.C), stemp, stb));
+  uint32_t *tileStart = (uint32_t *)W.tileStart.get(((size_t)S + 65) * 4);
   L.start = (uint32_t *)W.bStart.get(((size_t)L.C + 65) * 4);
-  HIP_TRY(learn_bucket_starts(s, k1, (int)nK, (int)L.C, L.start));
   L.keyZs = (uint16_t *)W.keyZs.get((size_t)nK * kEmb * 2 + 64);
   L.keyPtsS = (int2 *)W.keyPtsS.get((size_t)nK * sizeof(int2) + 64);
-  HIP_TRY(learn_permute_keys(s, v1, (int)nK, L.keyZ, L.keyPts, L.keyZs, L.keyPtsS));
-  HIP_TRY(hipEventRecord(g_dev.indexReady, s));
+  const bool wide = job.wide;
+  const void *cword = pr.b.cword;
+  const int2 *offsets = pr.b.offsets;
+  const int share = index_cu_share();
   L.ok = true;
   L.pending = true;
+  LearnedIndex *Lp = &L;
+  L.builder = std::thread([=]() {
+    try {
+      for (unsigned b0 = 0; b0 < nK; b0 += kKeyBlock) {
+        const int nb = (int)std::min<unsigned>(kKeyBlock, nK - b0);
+        HIP_TRY(learn_features(s, wide, cword, g.cw, g.ch, 0, 0, g.nColors, offsets, g.nO, Lp->keyPts + b0, nb, true,
+                               true, 0, x));
+        HIP_TRY(mlp_embed_fused(s, w, wt, x, nb, Lp->keyZ + (size_t)b0 * kEmb, nullptr, share));
+      }
+      HIP_TRY(gather_spread_rows_bf16(s, Lp->keyZ, kEmb, nK, (int)S, Lp->superCent));
+      HIP_TRY(learn_topk_cols(s, Lp->keyZ, (int)nK, Lp->superCent, (int)S, 1, sid));
+      HIP_TRY(learn_sort_keys(s, sid, 1, (int)nK, k0, v0));
+      HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, nK, bits_for(S), stemp, stb));
+      HIP_TRY(learn_bucket_starts(s, k1, (int)nK, (int)S, startS));
+      HIP_TRY(learn_sub_index(s, startS, v1, (int)S, (int)Q, (int)nK, Lp->keyZ, Lp->subCent, tileStart, cl));
+      HIP_TRY(learn_sort_keys(s, cl, 1, (int)nK, k0, v0));
+      HIP_TRY(launch_sort_pairs(s, k0, k1, v0, v1, nK, bits_for(Lp->C), stemp, stb));
+      HIP_TRY(learn_bucket_starts(s, k1, (int)nK, (int)Lp->C, Lp->start));
+      HIP_TRY(learn_permute_keys(s, v1, (int)nK, Lp->keyZ, Lp->keyPts, Lp->keyZs, Lp->keyPtsS));
+      HIP_TRY(hipEventRecord(g_dev.indexReady, s));
+      if (builtEvent) HIP_TRY(hipEventRecord(builtEvent, s));
+    } catch (...) {
+      Lp->builderError = std::current_exception();
+    }
+  });
 }
 
-// Makes the library stream wait for an index still building on the side stream.
+// Makes the library stream wait for an index still building on the side stream (after its
+// helper thread has enqueued the build, or rethrows what stopped it).
 void learned_wait(LearnedIndex &L)
 {
+  if (L.builder.joinable()) L.builder.join();
+  if (L.builderError) {
+    std::exception_ptr e = L.builderError;
+    L.builderError = nullptr;
+    L.pending = false;
+    std::rethrow_exception(e);
+  }
   if (!L.pending) return;
   HIP_TRY(hipStreamWaitEvent(g_dev.stream, g_dev.indexReady, 0));
   L.pending = false;
@@ -1128,8 +1152,8 @@ int refine_engine(Job &job, Prepared &pr)
       HIP_TRY(hipEventCreate(&indexEvents.second));
       HIP_TRY(hipEventRecord(indexEvents.first, s));
     }
-    learned_build(job, pr, learned);
-    if (job.opt.timing) HIP_TRY(hipEventRecord(indexEvents.second, learned.pending ? g_dev.side : s));
+    learned_build(job, pr, learned, indexEvents.second);
+    if (job.opt.timing && !learned.pending) HIP_TRY(hipEventRecord(indexEvents.second, s));
     job.st.learnedKeys = learned.nK;
     job.st.learnedClusters = learned.ok ? learned.C : 0;
     job.st.learnedIndexMs = now_ms() - tl;   // host time; the device time replaces it when timed

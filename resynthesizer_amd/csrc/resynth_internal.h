@@ -169,7 +169,7 @@ hipError_t launch_reach_radius(hipStream_t s, bool wide, const void *tword, cons
 hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas);
 hipError_t launch_first_idx(hipStream_t s, int w, int h, unsigned rhoMax, uint32_t *firstIdx);
 hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *perm, int2 *out, unsigned n);
-hipError_t launch_random_order_keys(hipStream_t s, uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals);
+hipError_t launch_random_order(hipStream_t s, uint32_t seed, unsigned n, const int2 *rm, int2 *out);
 // target ordering of match-context types 2..8 (resynth_order.hip)
 size_t order_temp_bytes(unsigned n);
 hipError_t launch_order_sort(hipStream_t s, const int2 *rm, unsigned n, int type, const uint32_t *axisRays,

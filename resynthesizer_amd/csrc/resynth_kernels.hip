@@ -2672,25 +2672,45 @@ hipError_t launch_gather_targets(hipStream_t s, const int2 *rm, const uint32_t *
   return hipGetLastError();
 }
 
-// Sort keys of the batched mode's random target order: a counter-based hash of (seed, i)
-// per row-major target i; sorted (stably, all 32 bits) with the indices as values they give
-// a uniformly random permutation, drawn on the device.
-__global__ void k_random_order_keys(uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
+// The batched mode's random target order: out[i] = rm[perm(i)] for a keyed pseudo-random
+// permutation of [0, n), drawn on the device in one pass.  perm is a 4-round Feistel
+// network on 2*hb-bit words (2^(2 hb) >= n, at most 4n), each round function a 64-bit
+// mixer of (seed, round, half); it is a bijection of the words, and cycle-walking (apply it
+// again while the result is n or more) restricts it to a bijection of [0, n) -- the walk
+// from i returns below n within its own cycle, about 2-4 steps on average.
+__device__ __forceinline__ uint32_t feistel_round(uint32_t seed, uint32_t round, uint32_t half)
+{
+  uint64_t z = (((uint64_t)seed << 32) | (((uint64_t)round << 28) ^ half)) + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return (uint32_t)((z ^ (z >> 31)) >> 32);
+}
+
+__global__ void k_random_order(uint32_t seed, unsigned n, unsigned hb, const int2 *rm, int2 *out)
 {
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t z = ((uint64_t)seed << 32 | i) + 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  keys[i] = (uint32_t)(z >> 32);
-  vals[i] = i;
+  const uint32_t mask = (1u << hb) - 1u;
+  uint32_t x = i;
+  do {
+    uint32_t L = x >> hb, R = x & mask;
+#pragma unroll
+    for (uint32_t r = 0; r < 4; r++) {
+      const uint32_t t = L ^ (feistel_round(seed, r, R) & mask);
+      L = R;
+      R = t;
+    }
+    x = (L << hb) | R;
+  } while (x >= n);
+  out[i] = rm[x];
 }
 
-hipError_t launch_random_order_keys(hipStream_t s, uint32_t seed, unsigned n, uint32_t *keys, uint32_t *vals)
+hipError_t launch_random_order(hipStream_t s, uint32_t seed, unsigned n, const int2 *rm, int2 *out)
 {
   if (!n) return hipSuccess;
-  hipLaunchKernelGGL(k_random_order_keys, dim3((n + 255) / 256), dim3(256), 0, s, seed, n, keys, vals);
+  unsigned hb = 1;
+  while ((1ull << (2 * hb)) < (unsigned long long)n) hb++;
+  hipLaunchKernelGGL(k_random_order, dim3((n + 255) / 256), dim3(256), 0, s, seed, n, hb, rm, out);
   return hipGetLastError();
 }
 
