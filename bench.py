@@ -119,9 +119,11 @@ def bench_config3(args, np, torch, R, W):
 # Learned-mode settings of the config-5 line: proposals of every target computed once at
 # the start of pass 1 from the pass-0 result (1 inverted-file cluster searched, top 16),
 # 1 random probe beside them, 16 random probes in pass 0, whose chunks grow by 30% of the
-# visited targets, 2 passes.
+# visited targets up to 262 144 targets (a quarter of the hole; the library's default cap
+# is nT/32), 2 passes.
+LEARNED_BATCH = 262144
 LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
-               ramp_permille=300)
+               ramp_permille=300, batch=LEARNED_BATCH)
 # The same with 32 random probes beside the proposals (the probe count of the configured
 # setting).
 LEARNED32 = dict(LEARNED, learned_probes=32)
@@ -217,29 +219,31 @@ def bench_config5(args, np, torch, R, W):
     d_img = torch.empty_like(d_src)
     d_mask = torch.from_numpy(mask).cuda()
 
-    def run(probes, learned, seed=0, timing=False):
+    def run(probes, learned, seed=0, timing=False, plain=None):
         d_img.copy_(d_src)
         torch.cuda.synchronize()
-        kw = dict(learned, model=model) if learned is not None else {}
+        kw = dict(learned, model=model) if learned is not None else dict(plain or {})
         r = R.image_synth_device(d_img.data_ptr(), w, h, w * c, d_mask.data_ptr(), w, R.T_RGB,
                                  R.default_params(maxProbeCount=probes), mode=R.MODE_BATCHED, seed=seed,
                                  timing=timing, **kw)
         assert r.err == 0, r.err
         return r
 
-    def timed(probes, learned, steps, warmup):
+    def timed(probes, learned, steps, warmup, plain=None):
         for _ in range(warmup):
-            run(probes, learned)
+            run(probes, learned, plain=plain)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         errs = []
         for k in range(steps):
-            errs.append(run(probes, learned, seed=k + 1).meanError)
+            errs.append(run(probes, learned, seed=k + 1, plain=plain).meanError)
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / steps, float(np.mean(errs))
 
     sec_l, err_l = timed(200, LEARNED, args.steps, args.warmup)
     sec_p, err_p = timed(200, None, max(2, args.steps // 2), 1)
+    # the plain method with the learned line's chunk cap (faster, at a higher error): reported
+    sec_pc, err_pc = timed(200, None, 2, 1, plain=dict(batch=LEARNED_BATCH))
     rl = run(200, LEARNED, timing=True)
     rp = run(200, None, timing=True)
     n_t = rl.nTargets
@@ -290,6 +294,8 @@ def bench_config5(args, np, torch, R, W):
                    "learned_mean_match_error": round(err_l, 1),
                    "plain200_px_per_s": round(n_t / sec_p, 1), "plain200_ms": round(sec_p * 1e3, 2),
                    "plain200_mean_match_error": round(err_p, 1),
+                   "plain200_at_learned_chunk_cap": {"batch": LEARNED_BATCH, "ms": round(sec_pc * 1e3, 2),
+                                                     "mean_match_error": round(err_pc, 1)},
                    "speedup_vs_plain200": round(sec_p / sec_l, 3),
                    "error_ratio_learned_vs_plain200": round(err_l / err_p, 4),
                    "speedup_at_equal_error": round(sec_p * 1e3 / best_eq[0], 3),

@@ -87,8 +87,9 @@ def test_config3_all_requests_in_one_call(serial):
 
 # ---------------------------------------------------------------- config 5
 
+# the benchmark's learned setting (chunks capped at 262 144 targets)
 LEARNED = dict(learned_refresh=0b10, learned_clusters=1, learned_probes=1, learned_probes_later=16, max_passes=2,
-               ramp_permille=300)
+               ramp_permille=300, batch=262144)
 
 
 @pytest.fixture(scope="module")
