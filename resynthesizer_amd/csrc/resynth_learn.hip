@@ -902,30 +902,39 @@ __device__ __forceinline__ uint32_t pack_bf16(float lo, float hi)
 }
 
 // One layer on 64 rows: out[64 x N] = act(in[64 x K] W + b); this wave computes output
-// features [wave*N/4, (wave+1)*N/4), CB = N/64 blocks of 16, from its slice of the weights
-// held in registers (fw[k-step][feature block]).  The weights are the MFMA A operand and
-// the rows the B operand, so a lane ends up with 4 consecutive features of one row and
-// writes them with one 8-byte (16-byte for the fp32 last layer) LDS store.
-template <int K, int N, bool RELU>
-__device__ __forceinline__ void mlp_layer(const uint16_t (*in)[kLdsW], const bf16x8 (&fw)[K / 32][N / 64],
-                                          const float *bias, uint16_t (*out)[kLdsW], float (*outF)[kEmb + 4])
+// features [wave*N/4, (wave+1)*N/4), CB = N/64 blocks of 16, from its slice of the weights:
+// weight(ks, b, feature, kq) returns a lane's A operand for k-step ks and feature block b,
+// from registers or from the layer's transposed weights in LDS.  The weights are the MFMA A
+// operand and the rows the B operand, so a lane ends up with 4 consecutive features of one
+// row and writes them with one 8-byte (16-byte for the fp32 last layer) LDS store.  The
+// row fragments are read from LDS kPipe steps ahead of the MFMAs that use them, so that LDS
+// latency hides behind the matrix cores (one wave per SIMD has no other wave to switch to).
+constexpr int kPipe = 3;
+
+template <int K, int N, bool RELU, typename Weight>
+__device__ __forceinline__ void mlp_layer(const uint16_t (*in)[kLdsW], Weight weight, const float *bias,
+                                          uint16_t (*out)[kLdsW], float (*outF)[kEmb + 4])
 {
-  constexpr int CB = N / 64, KS = K / 32;
+  constexpr int CB = N / 64, KS = K / 32, STEPS = KS * 4;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int c0 = wave * (N / 4);
   const int rr = lane & 15, kq = (lane >> 4) * 8, f4 = (lane >> 4) * 4;
   f32x4 acc[4][CB];
   for (int a = 0; a < 4; a++)
     for (int b = 0; b < CB; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // step st = ks * 4 + a: rows a*16.., k-step ks
+  bf16x8 fx[kPipe];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) {
+  for (int st = 0; st < kPipe; st++) fx[st] = *(const bf16x8 *)&in[(st & 3) * 16 + rr][(st >> 2) * 32 + kq];
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-      const bf16x8 fx = *(const bf16x8 *)&in[a * 16 + rr][ks * 32 + kq];
+  for (int st = 0; st < STEPS; st++) {
+    const int ks = st >> 2, a = st & 3;
+    const bf16x8 cur = fx[st % kPipe];
+    if (st + kPipe < STEPS)
+      fx[st % kPipe] = *(const bf16x8 *)&in[((st + kPipe) & 3) * 16 + rr][((st + kPipe) >> 2) * 32 + kq];
 #pragma unroll
-      for (int b = 0; b < CB; b++)
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[ks][b], fx, acc[a][b], 0, 0, 0);
-    }
+    for (int b = 0; b < CB; b++)
+      acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(weight(ks, b, c0 + b * 16 + rr, kq), cur, acc[a][b], 0, 0, 0);
   }
   // acc[a][b][r]: feature c0 + b*16 + f4 + r of row a*16 + rr
 #pragma unroll
@@ -962,24 +971,30 @@ __device__ __forceinline__ void load_layer(const uint16_t *wt, bf16x8 (&fw)[K / 
 }
 
 // k_mlp_embed: persistent, one 4-wave workgroup per compute unit.  Each wave loads its
-// column slices of all three layers' weights into registers once (224 VGPRs of bf16
-// B operands) and then takes 64-row blocks in turn: the rows go through the three layers
-// and the normalisation with the activations in LDS (two ping-pong buffers), and the next
-// block's input rows are fetched into registers while this one computes.  Re-reading the
-// 229 KB of weights from L2 for every 64 rows left the matrix cores idle 88% of the time.
+// column slices of the first two layers' weights into registers once (192 VGPRs of bf16
+// operands; the third layer's 32 KB sit in LDS, which leaves registers to read the row
+// fragments ahead) and then takes 64-row blocks in turn: the rows go through the three
+// layers and the normalisation with the activations in LDS (two ping-pong buffers), and the
+// next block's input rows are fetched into registers while this one computes.  Re-reading
+// the 229 KB of weights from L2 for every 64 rows left the matrix cores idle 88% of the time.
 __global__ __launch_bounds__(256, 1) void k_mlp_embed(const uint16_t *x, int n, const uint16_t *wt, const float *w,
                                                       uint16_t *zb, float *z)
 {
   __shared__ __attribute__((aligned(16))) uint16_t bufA[kRowsE][kLdsW];
   __shared__ __attribute__((aligned(16))) uint16_t bufB[kRowsE][kLdsW];
+  __shared__ __attribute__((aligned(16))) uint16_t w3s[kEmb][kLdsW];
   const size_t OB1 = (size_t)kIn * kH1, OW2 = OB1 + kH1, OB2 = OW2 + (size_t)kH1 * kH2, OW3 = OB2 + kH2,
                OB3 = OW3 + (size_t)kH2 * kEmb;
   const uint16_t *wt1 = wt, *wt2 = wt + (size_t)kIn * kH1, *wt3 = wt2 + (size_t)kH1 * kH2;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  bf16x8 w1[kIn / 32][kH1 / 64], w2[kH1 / 32][kH2 / 64], w3[kH2 / 32][kEmb / 64];
+  bf16x8 w1[kIn / 32][kH1 / 64], w2[kH1 / 32][kH2 / 64];
   load_layer<kIn, kH1>(wt1, w1);
   load_layer<kH1, kH2>(wt2, w2);
-  load_layer<kH2, kEmb>(wt3, w3);
+  // third layer [kEmb x kH2] transposed, 16 bytes a thread per step
+  for (int i = tid; i < kEmb * kH2 / 8; i += 256) {
+    const int r = i / (kH2 / 8), c = (i % (kH2 / 8)) * 8;
+    *(uint4 *)&w3s[r][c] = *(const uint4 *)(wt3 + (size_t)r * kH2 + c);
+  }
   const int nBlocks = (n + kRowsE - 1) / kRowsE;
   // input rows of a block: 64 x 128 bf16 = 1024 uint4, four per thread
   uint4 xin[4];
@@ -1003,11 +1018,13 @@ __global__ __launch_bounds__(256, 1) void k_mlp_embed(const uint16_t *x, int n, 
   float (*eF)[kEmb + 4] = (float (*)[kEmb + 4])&bufB[0][0];   // 64 x 68 floats fit in bufB
   for (; blk < nBlocks; blk += gridDim.x) {
     fetch(blk + (int)gridDim.x);
-    mlp_layer<kIn, kH1, true>(bufA, w1, w + OB1, bufB, nullptr);
+    mlp_layer<kIn, kH1, true>(bufA, [&](int ks, int b, int, int) { return w1[ks][b]; }, w + OB1, bufB, nullptr);
     __syncthreads();
-    mlp_layer<kH1, kH2, true>(bufB, w2, w + OB2, bufA, nullptr);
+    mlp_layer<kH1, kH2, true>(bufB, [&](int ks, int b, int, int) { return w2[ks][b]; }, w + OB2, bufA, nullptr);
     __syncthreads();
-    mlp_layer<kH2, kEmb, false>(bufA, w3, w + OB3, nullptr, eF);
+    mlp_layer<kH2, kEmb, false>(
+        bufA, [&](int ks, int, int f, int kq) { return *(const bf16x8 *)&w3s[f][ks * 32 + kq]; }, w + OB3, nullptr,
+        eF);
     __syncthreads();
     stage();   // bufA is free: the next block's rows go in while eF is normalised
     // normalise: wave w takes rows w*16 .. w*16+15 all at once, four lanes per row with 16

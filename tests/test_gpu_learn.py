@@ -50,30 +50,12 @@ def test_query_routing_matches_brute_force(P):
         gs = scores[got[i] - s * Q]
         assert np.allclose(gs, scores[order[:P]], atol=1e-5), (i, gs, scores[order[:P]])
 
-@pytest.mark.parametrize("P", [1, 2])
-def test_fine_search_returns_exact_top16(P):
-    """The index's fine search (16 x 16 MFMA score tiles, per-lane sorted lists, a merge per
-    query) against brute force: for every query, the 16 keys of its clusters with the
-    largest dot products, best first, padded with (-1, -1) when its clusters hold fewer than
-    16 keys.  Cluster sizes cover empty, partial key tiles and multi-tile clusters; tiles
-    of 16 sorted queries straddle cluster boundaries."""
-    rng = np.random.RandomState(20 + P)
-    sizes = [0, 1, 5, 15, 16, 17, 31, 33, 240, 700, 64, 3] * 3
-    nc = len(sizes)
-    start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
-    nk = int(start[-1])
-    kb = _bf16(rng, (nk, 64))
-    kf = MO.from_bits(kb)
-    nq = 523
-    qb = _bf16(rng, (nq, 64))
-    qf = MO.from_bits(qb)
-    pts = np.stack([np.arange(nk) % 1000, np.arange(nk) // 1000], 1).astype(np.int32)
-    cl = np.stack([rng.permutation(nc)[:P] for _ in range(nq)]).astype(np.int32)
-    cl[rng.rand(nq) < 0.05, 0] = -1
-    if P == 2:
-        cl[rng.rand(nq) < 0.05, 1] = -1
+def _check_fine_top16(qb, kb, cl, start, pts):
+    """Every query's 16 best keys of its clusters against brute force, best first, padded
+    with (-1, -1) when its clusters hold fewer than 16 keys; ties may come in any order."""
+    qf, kf = MO.from_bits(qb), MO.from_bits(kb)
     out = R.fine_top16(qb, cl, kb, start, pts)
-    for i in range(nq):
+    for i in range(len(qb)):
         cand = np.concatenate([np.arange(start[c], start[c + 1]) for c in cl[i] if c >= 0] + [np.zeros(0, int)])
         cand = cand.astype(np.int64)
         scores = kf[cand] @ qf[i]
@@ -88,6 +70,44 @@ def test_fine_search_returns_exact_top16(P):
         if want:
             kth = np.sort(scores)[::-1][want - 1]
             assert s.min() >= kth - 1e-5, (i, s.min(), kth)
+
+
+def _clusters(rng, sizes, nq, P):
+    nc = len(sizes)
+    start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint32)
+    cl = np.stack([rng.permutation(nc)[:P] for _ in range(nq)]).astype(np.int32)
+    cl[rng.rand(nq) < 0.05, 0] = -1
+    if P == 2:
+        cl[rng.rand(nq) < 0.05, 1] = -1
+    nk = int(start[-1])
+    pts = np.stack([np.arange(nk) % 1000, np.arange(nk) // 1000], 1).astype(np.int32)
+    return start, cl, pts
+
+
+@pytest.mark.parametrize("P", [1, 2])
+def test_fine_search_returns_exact_top16(P):
+    """The index's fine search (16 x 16 MFMA score tiles, per-lane sorted lists, a merge per
+    query) against brute force.  Cluster
+    sizes cover empty, partial key tiles and multi-tile clusters; tiles of 16 sorted queries
+    straddle cluster boundaries."""
+    rng = np.random.RandomState(20 + P)
+    sizes = [0, 1, 5, 15, 16, 17, 31, 33, 240, 700, 64, 3] * 3
+    start, cl, pts = _clusters(rng, sizes, 523, P)
+    _check_fine_top16(_bf16(rng, (523, 64)), _bf16(rng, (int(start[-1]), 64)), cl, start, pts)
+
+
+@pytest.mark.parametrize("distinct", [1, 3, 40])
+def test_fine_search_exact_under_ties(distinct):
+    """Keys drawn from a few distinct vectors tie in their products (with one, every key of a
+    cluster ties): the top 16 are still 16 distinct keys of the query's clusters, best first,
+    none below the 16th best product."""
+    rng = np.random.RandomState(50 + distinct)
+    sizes = [240, 700, 16, 90, 33, 300] * 2
+    start, cl, pts = _clusters(rng, sizes, 301, 2)
+    base = _bf16(rng, (distinct, 64))
+    kb = base[rng.randint(0, distinct, size=int(start[-1]))]
+    _check_fine_top16(_bf16(rng, (301, 64)), kb, cl, start, pts)
+
 
 def _features(n, seed):
     rng = np.random.RandomState(seed)
