@@ -210,7 +210,10 @@ typedef struct {
 } ResynthOptions;
 
 typedef struct {
-  unsigned int nTargets, nCorpus, nOffsets;
+  unsigned int nTargets, nCorpus;
+  unsigned int nOffsets;              /* sorted offsets built: the prefix every neighbour scan of the run
+                                         can reach (all (2w-1)(2h-1) when wrapping, or when the corpus is
+                                         smaller than the target) */
   unsigned int passes;
   unsigned int passSize[6];
   unsigned int betters[6];
