@@ -644,7 +644,7 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   const bool serialMode = job.opt.mode == RESYNTH_MODE_SERIAL;
 
   void *tword = W.tword.get(nTpix * wb);
-  void *cword = W.cword.get(std::max(nCpix, nTpix) * wb);
+  void *cword = W.cword.get(std::max(cword_words(cw, ch), nTpix) * wb);
   uint8_t *tflag = (uint8_t *)W.tflag.get(nTpix + 64);
   uint8_t *cflag = (uint8_t *)W.cflag.get(std::max(nCpix, nTpix) + 64);
   if (S.simple) {
@@ -687,11 +687,12 @@ int prepare_engine(Job &job, Prepared &pr, bool deferOrder = false)
   Tables *dT = (Tables *)W.tables.get(sizeof(Tables));
   HIP_TRY(hipMemcpyAsync(dT, hT, sizeof(Tables), hipMemcpyHostToDevice, s));
   int2 *sourceOf = (int2 *)W.sourceOf.get(nTpix * sizeof(int2));
-  uint32_t *prober = (uint32_t *)W.prober.get(nCpix * sizeof(uint32_t));
+  // the recent-prober map is indexed like the corpus words (tiled)
+  uint32_t *prober = (uint32_t *)W.prober.get(cword_words(cw, ch) * sizeof(uint32_t));
   uint32_t *lastBest = (uint32_t *)W.lastBest.get(nTpix * sizeof(uint32_t));
   Counters *cnt = (Counters *)W.counters.get(sizeof(Counters));
   HIP_TRY(launch_fill_u32(s, (uint32_t *)sourceOf, 0xFFFFFFFFu, nTpix * 2));
-  HIP_TRY(launch_fill_u32(s, prober, 0xFFFFFFFFu, nCpix));
+  HIP_TRY(launch_fill_u32(s, prober, 0xFFFFFFFFu, cword_words(cw, ch)));
   HIP_TRY(launch_fill_u32(s, lastBest, 0xFFFFFFFFu, nTpix));
   HIP_TRY(launch_fill_u32(s, (uint32_t *)cnt, 0u, sizeof(Counters) / 4));
   // occupancy of 8x8 tiles, used to skip provably empty offsets in pass-0 scans, and the
@@ -2227,11 +2228,11 @@ int resynth_score_frozen(const unsigned char *corpus, unsigned int cw, unsigned 
     std::vector<unsigned char> nb(nnb * depth);
     for (size_t i = 0; i < nnb; i++) std::memcpy(&nb[i * depth], nbPixel + i * 8, depth);
     HIP_TRY(hipMemcpyAsync(dNbPix, nb.data(), nb.size(), hipMemcpyHostToDevice, s));
-    void *dC = g_dev.ws.cword.get(ncp * wb + 64);
+    void *dC = g_dev.ws.cword.get(cword_words((int)cw, (int)ch) * wb + 64);
     void *dNbW = g_dev.ws.tword.get(nnb * wb + 64);
-    HIP_TRY(launch_pack_words(s, wide, dPix, (int)ncp, depth, indices->colorEndBip, indices->map_start_bip,
+    HIP_TRY(launch_pack_words(s, wide, dPix, (int)ncp, (int)cw, depth, indices->colorEndBip, indices->map_start_bip,
                               indices->map_end_bip, 1, dC));
-    HIP_TRY(launch_pack_words(s, wide, dNbPix, (int)nnb, depth, indices->colorEndBip, indices->map_start_bip,
+    HIP_TRY(launch_pack_words(s, wide, dNbPix, (int)nnb, 1, depth, indices->colorEndBip, indices->map_start_bip,
                               indices->map_end_bip, 0, dNbW));
     Tables ht;
     std::memcpy(ht.T, T, sizeof ht.T);

@@ -36,6 +36,13 @@ __device__ __forceinline__ uint32_t chan(Word w, int c)
   return (uint32_t)(w >> (8 * (1 + c))) & 0xFFu;
 }
 
+// Word index of corpus pixel (x, y) in the tiled corpus layout (cword_words).
+__device__ __forceinline__ unsigned cidx(int cw, unsigned x, unsigned y)
+{
+  const unsigned tilesPerRow = ((unsigned)cw + kCTileW - 1) / kCTileW;
+  return ((__umul24(y >> 2, tilesPerRow) + (x >> 3)) << 5) | ((y & 3u) << 3) | (x & 7u);
+}
+
 // Score contribution of one in-corpus neighbour (lib/synthesize.h:328-355): colours
 // except for neighbour 0 (the target itself), maps for every neighbour.
 template <typename Word>
@@ -69,7 +76,7 @@ __device__ __forceinline__ void score_group(const Geometry &g, const Word *__res
     const int qx = cx + o.x, qy = cy + o.y;
     in[u] = live && (unsigned)qx < (unsigned)g.cw && (unsigned)qy < (unsigned)g.ch;
     // unconditional load, word 0 standing in for an outside neighbour (see score_group_rgb)
-    const unsigned idx = in[u] ? __umul24((unsigned)qy, (unsigned)g.cw) + (unsigned)qx : 0u;
+    const unsigned idx = in[u] ? cidx(g.cw, (unsigned)qx, (unsigned)qy) : 0u;
     q[u] = cword[idx];
   }
 #pragma unroll
@@ -107,7 +114,7 @@ __device__ __forceinline__ void score_group_rgb(const Geometry &g, const Word *_
     // Every lane loads (word 0 stands in for an outside neighbour, whose term is the
     // penalty), so the loads need no branch around them; the index is 32-bit (corpus
     // sides below 2^24 pixels, fewer than 2^31 words).
-    const unsigned idx = in[u] ? __umul24((unsigned)qy, (unsigned)g.cw) + (unsigned)qx : 0u;
+    const unsigned idx = in[u] ? cidx(g.cw, (unsigned)qx, (unsigned)qy) : 0u;
     q[u] = cword[idx];
   }
 #pragma unroll
@@ -447,7 +454,7 @@ __device__ __forceinline__ void coherence_phase(const Geometry &g, const Word *_
     cx = src.x - o.x;
     cy = src.y - o.y;
     const bool in = lane < count && src.x != -1 && (unsigned)cx < (unsigned)g.cw && (unsigned)cy < (unsigned)g.ch;
-    const unsigned ci = in ? __umul24((unsigned)cy, (unsigned)g.cw) + (unsigned)cx : 0u;
+    const unsigned ci = in ? cidx(g.cw, (unsigned)cx, (unsigned)cy) : 0u;
     const Word q = cword[ci];
     const uint32_t pr = prober[ci];
     valid = in && (q & 0xFEu) == 0xFEu;
@@ -901,7 +908,7 @@ __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word
   if (SERIAL && S.aborted) return;
 
   // commit (lib/synthesize.h:620-639)
-  if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
+  if (record) b.prober[cidx(g.cw, (unsigned)cx, (unsigned)cy)] = k;
   if (tid == 0) {
     size_t tp = (size_t)t.y * (unsigned)g.tw + (unsigned)t.x;
     Word old = tword[tp];
@@ -911,7 +918,7 @@ __device__ void visit_wg(const Geometry &g, const SynthBuffers &b, VisitLDS<Word
       if (cur.x != S.winX || cur.y != S.winY) {
         Word cmask = 0;
         for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
-        Word src = cword[(size_t)S.winY * (unsigned)g.cw + (unsigned)S.winX];
+        Word src = cword[cidx(g.cw, (unsigned)S.winX, (unsigned)S.winY)];
         nw = (old & ~cmask) | (src & cmask) | (Word)1;
         b.sourceOf[tp] = make_int2(S.winX, S.winY);
         betTot++;
@@ -1013,7 +1020,7 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
   int cx, cy;
   coherence_phase<Word, NC>(g, cword, b.prober, k, count, sOff, sPix, sSrc, sT, sTm, best, winX, winY, perfect,
                             scored, record, cx, cy, evals);
-  if (record) b.prober[(size_t)cy * (unsigned)g.cw + (unsigned)cx] = k;
+  if (record) b.prober[cidx(g.cw, (unsigned)cx, (unsigned)cy)] = k;
   // learned proposals: up to 16 corpus points, scored in rank order like probes (groups of
   // 8 loads would save round trips but take 8 more VGPRs, one wave per SIMD less)
   if (b.props && !perfect) {
@@ -1062,7 +1069,7 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
       c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
       // the candidate's own word (neighbour 0, loaded with the first group) carries the
       // corpus-point flag
-      const Word w0 = cword[p];
+      const Word w0 = cword[cidx(g.cw, (unsigned)c.x, (unsigned)c.y)];
       score_candidate<Word, G0 + 1, GL, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, false);
       valid = (w0 & 0xFFu) == 0xFFu;
     } else {
@@ -1099,7 +1106,7 @@ __device__ __forceinline__ void visit_wave(const Geometry &g, const SynthBuffers
       if (cur.x != winX || cur.y != winY) {
         Word cmask = 0;
         for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
-        const Word src = cword[(size_t)winY * (unsigned)g.cw + (unsigned)winX];
+        const Word src = cword[cidx(g.cw, (unsigned)winX, (unsigned)winY)];
         nw = (old & ~cmask) | (src & cmask) | (Word)1;
         const uint64_t packed = (uint64_t)(uint32_t)winX | ((uint64_t)(uint32_t)winY << 32);
         put<PUBLISH>((uint64_t *)&b.sourceOf[tp], packed);
@@ -1268,7 +1275,7 @@ __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers 
     const bool mine = go && i < count;
     const int2 c = sSrc[mine ? i : 0u];
     const bool in = mine && c.x != INT_MIN && (unsigned)c.x < (unsigned)g.cw && (unsigned)c.y < (unsigned)g.ch;
-    const unsigned ci = in ? __umul24((unsigned)c.y, (unsigned)g.cw) + (unsigned)c.x : 0u;
+    const unsigned ci = in ? cidx(g.cw, (unsigned)c.x, (unsigned)c.y) : 0u;
     const Word q = cword[ci];
     const uint32_t pr = b.prober[ci];
     bool valid = in && (q & 0xFEu) == 0xFEu && pr != k;
@@ -1359,7 +1366,8 @@ __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers 
       if (byPixel) {
         const uint64_t p = pix0 + (((h >> 32) * nPix) >> 32);
         c = make_int2((int)(p % (unsigned)g.cw), (int)(p / (unsigned)g.cw));
-        valid = (cword[p] & 0xFFu) == 0xFFu;   // a corpus point: selected and opaque
+        // a corpus point: selected and opaque
+        valid = (cword[cidx(g.cw, (unsigned)c.x, (unsigned)c.y)] & 0xFFu) == 0xFFu;
       } else {
         c = b.corpusPts[probe_index(g, b, h)];
         valid = true;
@@ -1401,7 +1409,7 @@ __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers 
       if (cur.x != winX || cur.y != winY) {
         Word cmask = 0;
         for (int ch = 0; ch < g.nColors; ch++) cmask |= (Word)0xFF << (8 * (1 + ch));
-        const Word src = cword[(size_t)winY * (unsigned)g.cw + (unsigned)winX];
+        const Word src = cword[cidx(g.cw, (unsigned)winX, (unsigned)winY)];
         nw = (old & ~cmask) | (src & cmask) | (Word)1;
         const uint64_t packed = (uint64_t)(uint32_t)winX | ((uint64_t)(uint32_t)winY << 32);
         put<PUBLISH>((uint64_t *)&b.sourceOf[tp], packed);
@@ -2177,7 +2185,7 @@ __global__ void k_adapt_simple(const unsigned char *__restrict__ image, size_t r
     unsigned char cm = mask2 ? mask2[y * mask2RowBytes + x] : (unsigned char)~m;
     bool hasValue = useContext && m == 0 && opaque;
     tword[p] = chans | (Word)(hasValue ? 1u : 0u);
-    cword[p] = chans | (Word)(cm == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
+    cword[cidx(w, x, y)] = chans | (Word)(cm == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
     tflag[p] = m != 0;
     cflag[p] = (cm == 0xFF) && opaque;
   }
@@ -2225,7 +2233,8 @@ __global__ void k_adapt_maps(const unsigned char *__restrict__ pix, int w, int h
       word[p] = chans | (Word)(hasValue ? 1u : 0u);
       flag[p] = m != 0;
     } else {
-      word[p] = chans | (Word)(m == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
+      word[cidx(w, (unsigned)(p % (unsigned)w), (unsigned)(p / (unsigned)w))] =
+          chans | (Word)(m == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
       flag[p] = (m == 0xFF) && opaque;
     }
   }
@@ -2249,8 +2258,9 @@ hipError_t launch_adapt_maps(hipStream_t s, bool wide, const unsigned char *pix,
 }
 
 // Packs engine-layout pixels (frozen-state scorer input) into words.
+// Corpus words (corpusFlags) go to the tiled corpus layout of a w-wide corpus.
 template <typename Word>
-__global__ void k_pack_words(const unsigned char *pix, int n, int depth, int colorEnd, int mapStart, int mapEnd,
+__global__ void k_pack_words(const unsigned char *pix, int n, int w, int depth, int colorEnd, int mapStart, int mapEnd,
                              int corpusFlags, Word *out)
 {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2260,19 +2270,21 @@ __global__ void k_pack_words(const unsigned char *pix, int n, int depth, int col
   int c = 0;
   for (int j = 1; j < colorEnd; j++, c++) chans |= (Word)px[j] << (8 * (1 + c));
   for (int j = mapStart; j < mapEnd; j++, c++) chans |= (Word)px[j] << (8 * (1 + c));
-  out[p] = chans | (Word)(corpusFlags ? (px[0] == 0xFF ? 0xFFu : 0u) : (px[0] & 1u));
+  const Word v = chans | (Word)(corpusFlags ? (px[0] == 0xFF ? 0xFFu : 0u) : (px[0] & 1u));
+  if (corpusFlags) out[cidx(w, (unsigned)p % (unsigned)w, (unsigned)p / (unsigned)w)] = v;
+  else out[p] = v;
 }
 
-hipError_t launch_pack_words(hipStream_t s, bool wide, const unsigned char *pix, int n, int depth, int colorEnd,
-                             int mapStart, int mapEnd, int corpusFlags, void *out)
+hipError_t launch_pack_words(hipStream_t s, bool wide, const unsigned char *pix, int n, int w, int depth,
+                             int colorEnd, int mapStart, int mapEnd, int corpusFlags, void *out)
 {
   unsigned blocks = (unsigned)((n + 255) / 256);
   if (blocks == 0) return hipSuccess;
   if (wide)
-    hipLaunchKernelGGL((k_pack_words<uint64_t>), dim3(blocks), dim3(256), 0, s, pix, n, depth, colorEnd, mapStart,
+    hipLaunchKernelGGL((k_pack_words<uint64_t>), dim3(blocks), dim3(256), 0, s, pix, n, w, depth, colorEnd, mapStart,
                        mapEnd, corpusFlags, (uint64_t *)out);
   else
-    hipLaunchKernelGGL((k_pack_words<uint32_t>), dim3(blocks), dim3(256), 0, s, pix, n, depth, colorEnd, mapStart,
+    hipLaunchKernelGGL((k_pack_words<uint32_t>), dim3(blocks), dim3(256), 0, s, pix, n, w, depth, colorEnd, mapStart,
                        mapEnd, corpusFlags, (uint32_t *)out);
   return hipGetLastError();
 }

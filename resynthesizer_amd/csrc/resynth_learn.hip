@@ -29,6 +29,14 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// Word index of corpus pixel (x, y): corpus words are stored in tiles of 8 x 4 pixels, the
+// tiles of a row of tiles side by side (the engine's corpus layout).
+__device__ __forceinline__ size_t corpus_word_index(int cw, unsigned x, unsigned y)
+{
+  const unsigned tilesPerRow = ((unsigned)cw + 7u) / 8u;
+  return ((size_t)((y >> 2) * tilesPerRow + (x >> 3)) << 5) | ((y & 3u) << 3) | (x & 7u);
+}
+
 // ------------------------------------------------------------------ GEMM
 //
 // C[M x N] = op(A)[M x K] * op(B)[K x N], A and B bf16, FP32 accumulation on
@@ -498,7 +506,9 @@ __global__ __launch_bounds__(256) void k_features(const Word *words, int w, int 
     if (y < 0) { if (tileV) y += h; else in = false; }
     else if (y >= h) { if (tileV) y -= h; else in = false; }
     if (in && x >= 0 && x < w && y >= 0 && y < h) {
-      const Word wd = words[(size_t)y * (unsigned)w + (unsigned)x];
+      const size_t at =
+          isCorpus ? corpus_word_index(w, (unsigned)x, (unsigned)y) : (size_t)y * (unsigned)w + (unsigned)x;
+      const Word wd = words[at];
       bool present = isCorpus ? ((wd & 0xFEu) == 0xFEu) : ((wd & 1u) != 0);
       if (present && dropKey) {
         // keep each neighbour with a per-point probability in [0.2, 1)
@@ -541,7 +551,7 @@ __global__ void k_key_flags(const Word *cword, int cw, int ch, uint8_t *flags)
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)w2 * h2) return;
   const int x = (int)(i % w2) * 2, y = (int)(i / w2) * 2;
-  flags[i] = ((cword[(size_t)y * cw + x] & 0xFFu) == 0xFFu) ? 1 : 0;
+  flags[i] = ((cword[corpus_word_index(cw, (unsigned)x, (unsigned)y)] & 0xFFu) == 0xFFu) ? 1 : 0;
 }
 
 hipError_t learn_key_flags(hipStream_t s, bool wide, const void *cword, int cw, int ch, uint8_t *flags)
