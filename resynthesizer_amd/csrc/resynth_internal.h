@@ -48,21 +48,15 @@ struct Geometry {
 constexpr int kBands = 8;   // one band of corpus rows per XCD
 
 // Corpus words are stored in tiles of 8 x 4 pixels (32 words: one 128-byte line of 32-bit
-// words), the tiles of a row of tiles side by side and the rows of tiles one after another.
-// A candidate's patch of the ~30 nearest neighbours (about 7 x 7 pixels) then touches about
-// 4 lines, where rows of pixels would take 7 to 8.  The buffer covers the corpus rounded up
-// to whole tiles (cword_words); the words past the corpus are never read.
+// words): the corpus is cut into rows of four pixel rows, and within one such row each
+// pixel's four words (one per pixel row) lie side by side.  A candidate's patch of the ~30
+// nearest neighbours (about 7 x 7 pixels) then touches about 4 lines, where rows of pixels
+// would take 7 to 8.  The buffer covers the corpus rounded up to whole tiles
+// (cword_words); the words past the corpus are never read.
 constexpr int kCTileW = 8, kCTileH = 4;
 __host__ __device__ inline size_t cword_words(int cw, int ch)
 {
   return (size_t)((cw + kCTileW - 1) / kCTileW) * kCTileW * (size_t)((ch + kCTileH - 1) / kCTileH) * kCTileH;
-}
-// Word index of corpus pixel (x, y) (host side; the kernels use cidx, the same with a 24-bit
-// multiply).
-inline unsigned cidx_host(int cw, unsigned x, unsigned y)
-{
-  const unsigned tilesPerRow = ((unsigned)cw + kCTileW - 1) / kCTileW;
-  return (((y >> 2) * tilesPerRow + (x >> 3)) << 5) | ((y & 3u) << 3) | (x & 7u);
 }
 
 constexpr int kTile = 8;

@@ -36,11 +36,13 @@ __device__ __forceinline__ uint32_t chan(Word w, int c)
   return (uint32_t)(w >> (8 * (1 + c))) & 0xFFu;
 }
 
-// Word index of corpus pixel (x, y) in the tiled corpus layout (cword_words).
+// Word index of corpus pixel (x, y) in the tiled corpus layout (cword_words): rows of four
+// pixel rows, each pixel's four words (one per row) side by side, so that the 32 words of
+// a 128-byte line hold an 8 x 4 tile.  One 24-bit multiply-add and two bit operations.
 __device__ __forceinline__ unsigned cidx(int cw, unsigned x, unsigned y)
 {
-  const unsigned tilesPerRow = ((unsigned)cw + kCTileW - 1) / kCTileW;
-  return ((__umul24(y >> 2, tilesPerRow) + (x >> 3)) << 5) | ((y & 3u) << 3) | (x & 7u);
+  const unsigned rowOf4 = (((unsigned)cw + kCTileW - 1) & ~(unsigned)(kCTileW - 1)) * kCTileH;
+  return __umul24(y >> 2, rowOf4) + ((x << 2) | (y & 3u));
 }
 
 // Score contribution of one in-corpus neighbour (lib/synthesize.h:328-355): colours

@@ -29,12 +29,12 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Word index of corpus pixel (x, y): corpus words are stored in tiles of 8 x 4 pixels, the
-// tiles of a row of tiles side by side (the engine's corpus layout).
+// Word index of corpus pixel (x, y) in the engine's corpus layout: rows of four pixel rows,
+// each pixel's four words side by side (8 x 4-pixel tiles per 128-byte line).
 __device__ __forceinline__ size_t corpus_word_index(int cw, unsigned x, unsigned y)
 {
-  const unsigned tilesPerRow = ((unsigned)cw + 7u) / 8u;
-  return ((size_t)((y >> 2) * tilesPerRow + (x >> 3)) << 5) | ((y & 3u) << 3) | (x & 7u);
+  const size_t rowOf4 = (size_t)(((unsigned)cw + 7u) & ~7u) * 4u;
+  return (size_t)(y >> 2) * rowOf4 + ((x << 2) | (y & 3u));
 }
 
 // ------------------------------------------------------------------ GEMM
