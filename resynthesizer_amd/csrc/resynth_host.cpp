@@ -386,6 +386,7 @@ struct DeviceState {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;                     // learned index builds, beside pass 0
   hipEvent_t indexReady = nullptr;                // recorded on `side` when an index is built
+  hipEvent_t weightsReady = nullptr;              // recorded on `side` after the network's weights are laid out
   Workspace ws;                                   // single-request entries
   std::vector<std::unique_ptr<Workspace>> pool;   // one per request of a multi-request call
   Buf manyDesc, manyCtl, manyGlobal;
@@ -412,6 +413,7 @@ bool ensure_device()
       hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&g_dev.side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&g_dev.indexReady, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g_dev.weightsReady, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&g_dev.countsReady, hipEventDisableTiming) != hipSuccess) {
     std::fprintf(stderr, "resynth_mi355x: the HIP device could not be set up\n");
     return false;
@@ -912,13 +914,15 @@ void learned_build(Job &job, const Prepared &pr, LearnedIndex &L, hipEvent_t bui
   if (nK < 64) return;
   // From here the build reads only the corpus words, which synthesis never writes, and
   // buffers of its own: it runs on the side stream, overlapping pass 0 (whose early chunks
-  // leave most compute units idle), and the first proposal waits for indexReady.  The
+  // leave most compute units idle), and the first proposal's search waits for indexReady
+  // (its queries' embedding only for the network's weights, weightsReady).  The
   // synchronization above has completed everything it depends on.  Its buffers are sized
   // here; a helper thread enqueues its kernels, so that the host goes on to enqueue pass 0
   // at once rather than after the build's some fifty launches.
   s = g_dev.side;
   const float *w = model_weights(M);
   const uint16_t *wt = model_weights_t(M, s);
+  HIP_TRY(hipEventRecord(g_dev.weightsReady, s));
   L.keyZ = (uint16_t *)W.keyZ.get((size_t)nK * kEmb * 2 + 64);
   uint16_t *x = (uint16_t *)W.keyX.get((size_t)kKeyBlock * kIn * 2);
   const unsigned Q = 128;
@@ -1001,12 +1005,12 @@ void learned_wait(LearnedIndex &L)
 constexpr unsigned kQueryBlock = 1u << 20;
 constexpr unsigned kLearnMinChunk = 256;  // per-chunk proposals for chunks of at least this size
 
-void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
+void learned_propose_block(Job &job, const Prepared &pr, LearnedIndex &L, unsigned k0, unsigned k1, int P,
                            int2 *props);
 
 // Proposals for targets k0..k1-1, 16 each, written to out (or to the workspace when out is
 // null); returns where they are.
-const int2 *learned_propose(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
+const int2 *learned_propose(Job &job, const Prepared &pr, LearnedIndex &L, unsigned k0, unsigned k1, int P,
                             int2 *out = nullptr)
 {
   Workspace &W = *job.W;
@@ -1016,7 +1020,7 @@ const int2 *learned_propose(Job &job, const Prepared &pr, const LearnedIndex &L,
   return out;
 }
 
-void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, unsigned k0, unsigned k1, int P,
+void learned_propose_block(Job &job, const Prepared &pr, LearnedIndex &L, unsigned k0, unsigned k1, int P,
                            int2 *props)
 {
   hipStream_t s = g_dev.stream;
@@ -1028,7 +1032,11 @@ void learned_propose_block(Job &job, const Prepared &pr, const LearnedIndex &L, 
   int32_t *qcl = (int32_t *)W.qCl.get((size_t)n * 2 * 4 + 64);
   HIP_TRY(learn_features(s, job.wide, pr.b.tword, g.tw, g.th, g.tileH, g.tileV, g.nColors, pr.b.offsets,
                          g.nO, pr.b.targets + k0, n, false, false, 0, x));
+  // The queries' features and embedding need only the targets' state and the network, so
+  // they run while the index build finishes on the side stream; the search waits for it.
+  HIP_TRY(hipStreamWaitEvent(s, g_dev.weightsReady, 0));
   HIP_TRY(mlp_embed_fused(s, model_weights(job.opt.model), model_weights_t(job.opt.model, s), x, n, z, nullptr));
+  learned_wait(L);
   int32_t *qs = (int32_t *)W.qSuper.get((size_t)n * 4 + 64);
   HIP_TRY(learn_topk_cols(s, z, n, L.superCent, (int)L.S, 1, qs));
   // Both searches visit the queries sorted by the cluster they search, so that a tile of 16
@@ -1177,7 +1185,6 @@ int refine_engine(Job &job, Prepared &pr)
         HIP_TRY(hipEventRecord(pa, s));
       }
       propsAll = (int2 *)W.propsAll.get((size_t)nT * 16 * sizeof(int2) + 64);
-      learned_wait(learned);
       learned_propose(job, pr, learned, 0, nT, job.opt.learnedClusters == 1 ? 1 : 2, propsAll);
       if (job.opt.timing) {
         HIP_TRY(hipEventRecord(pb, s));
@@ -1266,7 +1273,6 @@ int refine_engine(Job &job, Prepared &pr)
               HIP_TRY(hipEventCreate(&pb));
               HIP_TRY(hipEventRecord(pa, s));
             }
-            learned_wait(learned);
             bl.props = learned_propose(job, pr, learned, k, k1, job.opt.learnedClusters == 1 ? 1 : 2);
             if (job.opt.timing) {
               HIP_TRY(hipEventRecord(pb, s));
