@@ -444,7 +444,8 @@ def main():
                     help="only run a plain device copy of MIB MiB three times (calibrates HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        print("bench.py: the engine runs on one GPU (the sharded path is a design only); --gpus %d is not "
+              "supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
