@@ -376,7 +376,7 @@ struct Workspace {
   Buf streamBuf, streamState;
   // target ordering of types 2..8
   Buf orderRays, orderTemp;
-  PinnedBuf pinCounts, pinTables, pinPerm, pinStream;
+  PinnedBuf pinCounts, pinTables, pinPerm, pinStream, pinPass;
 };
 
 struct DeviceState {
@@ -387,6 +387,7 @@ struct DeviceState {
   hipStream_t side = nullptr;                     // learned index builds, beside pass 0
   hipEvent_t indexReady = nullptr;                // recorded on `side` when an index is built
   hipEvent_t weightsReady = nullptr;              // recorded on `side` after the network's weights are laid out
+  hipEvent_t passCounts = nullptr;                // a pass's counters on the host
   Workspace ws;                                   // single-request entries
   std::vector<std::unique_ptr<Workspace>> pool;   // one per request of a multi-request call
   Buf manyDesc, manyCtl, manyGlobal;
@@ -414,6 +415,7 @@ bool ensure_device()
       hipStreamCreateWithFlags(&g_dev.side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&g_dev.indexReady, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&g_dev.weightsReady, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g_dev.passCounts, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&g_dev.countsReady, hipEventDisableTiming) != hipSuccess) {
     std::fprintf(stderr, "resynth_mi355x: the HIP device could not be set up\n");
     return false;
@@ -1171,26 +1173,34 @@ int refine_engine(Job &job, Prepared &pr)
   unsigned pass;
   unsigned passesRun = 0;
   unsigned bettersBefore = 0;
+  bool lastBettersPending = false;
+  // pinned, so that the counter copies below do not hold the host
+  Counters *hcPass = (Counters *)W.pinPass.get(2 * sizeof(Counters) + 64);
+  unsigned long long *hSum = (unsigned long long *)(hcPass + 1);
+  // proposals of every target from the state at the start of pass p
+  unsigned proposedFor = ~0u;
+  auto refreshes = [&](unsigned p) { return learned.ok && p < 6 && ((job.opt.learnedRefresh >> p) & 1u); };
+  auto propose_pass = [&](unsigned p) {
+    hipEvent_t pa = nullptr, pb = nullptr;
+    if (job.opt.timing) {
+      HIP_TRY(hipEventCreate(&pa));
+      HIP_TRY(hipEventCreate(&pb));
+      HIP_TRY(hipEventRecord(pa, s));
+    }
+    propsAll = (int2 *)W.propsAll.get((size_t)nT * 16 * sizeof(int2) + 64);
+    learned_propose(job, pr, learned, 0, nT, job.opt.learnedClusters == 1 ? 1 : 2, propsAll);
+    if (job.opt.timing) {
+      HIP_TRY(hipEventRecord(pb, s));
+      proposeEvents.push_back({pa, pb});
+    }
+    proposedFor = p;
+  };
   for (pass = 0; pass < 6; pass++) {
     const unsigned end = sizes[pass];
     // every target has a value after pass 0; scans can skip empty rings only in pass 0,
     // and wrapped neighbourhoods (tiling) are not covered by the tile walk
     g.skipEmpty = (pass == 0 && !g.tileH && !g.tileV) ? 1 : 0;
-    if (learned.ok && (job.opt.learnedRefresh >> pass) & 1u) {
-      // proposals of every target from the state at the start of this pass
-      hipEvent_t pa = nullptr, pb = nullptr;
-      if (job.opt.timing) {
-        HIP_TRY(hipEventCreate(&pa));
-        HIP_TRY(hipEventCreate(&pb));
-        HIP_TRY(hipEventRecord(pa, s));
-      }
-      propsAll = (int2 *)W.propsAll.get((size_t)nT * 16 * sizeof(int2) + 64);
-      learned_propose(job, pr, learned, 0, nT, job.opt.learnedClusters == 1 ? 1 : 2, propsAll);
-      if (job.opt.timing) {
-        HIP_TRY(hipEventRecord(pb, s));
-        proposeEvents.push_back({pa, pb});
-      }
-    }
+    if (refreshes(pass) && proposedFor != pass) propose_pass(pass);
     // lib/synthesize.h:493-497: progress + cancel poll before target k when k & 4095 == 0
     unsigned k = 0;
     bool stop = false;
@@ -1293,11 +1303,25 @@ int refine_engine(Job &job, Prepared &pr)
       }
       k = k1;
     }
-    Counters hc;
+    // A pass after which the loop ends whatever the stop rule says (the pass cap, or the
+    // sixth pass) leaves its counters to the statistics' read below: one host round trip less.
+    const bool capped = pass == 5 || (!serial && job.opt.maxPasses && pass + 1 >= job.opt.maxPasses);
+    if (capped && !windowLaunches) {
+      job.st.passSize[pass] = end;
+      passesRun = pass + 1;
+      job.st.visits += k;
+      lastBettersPending = true;
+      break;
+    }
     uint32_t expired = 0;
-    HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hcPass, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     if (windowLaunches) HIP_TRY(hipMemcpyAsync(&expired, b.ctl + WCTL_EXPIRED, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipEventRecord(g_dev.passCounts, s));
+    // The next pass's proposals read only this pass's result: they go in before the host
+    // waits for the stop rule's counters (and are dropped if the rule ends the run).
+    if (refreshes(pass + 1)) propose_pass(pass + 1);
+    HIP_TRY(hipEventSynchronize(g_dev.passCounts));
+    const Counters &hc = *hcPass;
     if (expired) windowExpired++;
     unsigned bettersNow = 0;
     for (int i = 0; i < CNT_SLOTS; i++) bettersNow += hc.betters[i];
@@ -1322,14 +1346,19 @@ int refine_engine(Job &job, Prepared &pr)
 
   // statistics
   {
-    Counters hc;
     unsigned long long *dSum = (unsigned long long *)W.meanSum.get(16);
     HIP_TRY(launch_fill_u32(s, (uint32_t *)dSum, 0u, 4));
     HIP_TRY(launch_mean_error(s, lastBest, pr.targetsRM ? pr.targetsRM : targets, nT, tw, dSum));
-    unsigned long long hs[2];
+    unsigned long long *hs = hSum;
     HIP_TRY(hipMemcpyAsync(hs, dSum, 16, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&hc, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hcPass, cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const Counters &hc = *hcPass;
+    if (lastBettersPending) {
+      unsigned bettersNow = 0;
+      for (int i = 0; i < CNT_SLOTS; i++) bettersNow += hc.betters[i];
+      job.st.betters[passesRun - 1] = bettersNow - bettersBefore;
+    }
     job.st.meanError = hs[1] ? (double)hs[0] / (double)hs[1] : 0.0;
     for (int i = 0; i < CNT_SLOTS; i++) {
       job.st.candidates += hc.candidates[i];

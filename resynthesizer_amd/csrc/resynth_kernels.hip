@@ -2166,9 +2166,21 @@ hipError_t launch_synth_serial(hipStream_t s, bool wide, const Geometry &g, cons
 
 // ------------------------------------------------------------------ adaptation
 
-// SimpleAPI adaptation (lib/adaptSimple.h:226-307 + lib/engine.c:338-431): one thread per
-// pixel builds the target word (hasValue = use_context && unselected && opaque), the
-// corpus word (mask byte 0xFF <=> selected in the corpus mask) and the point flags.
+// SimpleAPI adaptation (lib/adaptSimple.h:226-307 + lib/engine.c:338-431): the target word
+// (hasValue = use_context && unselected && opaque), the corpus word (mask byte 0xFF <=>
+// selected in the corpus mask) and the point flags of every pixel.  A thread takes a column
+// of four rows: their corpus words are adjacent in the tiled layout and go out in one store
+// (rows past the image get zero words, in the layout's padding).
+__device__ __forceinline__ void store4(uint32_t *dst, const uint32_t (&v)[4])
+{
+  *(uint4 *)dst = make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void store4(uint64_t *dst, const uint64_t (&v)[4])
+{
+  ((ulonglong2 *)dst)[0] = make_ulonglong2(v[0], v[1]);
+  ((ulonglong2 *)dst)[1] = make_ulonglong2(v[2], v[3]);
+}
+
 template <typename Word>
 __global__ void k_adapt_simple(const unsigned char *__restrict__ image, size_t rowBytes, unsigned bpp,
                                const unsigned char *__restrict__ mask, size_t maskRowBytes,
@@ -2176,21 +2188,29 @@ __global__ void k_adapt_simple(const unsigned char *__restrict__ image, size_t r
                                int nColors, int alphaIdx, int useContext, Word *tword, Word *cword, uint8_t *tflag,
                                uint8_t *cflag)
 {
-  size_t n = (size_t)w * h;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-    unsigned x = (unsigned)(p % (unsigned)w), y = (unsigned)(p / (unsigned)w);
-    const unsigned char *px = image + y * rowBytes + (size_t)x * bpp;
-    Word chans = 0;
-    for (int c = 0; c < nColors; c++) chans |= (Word)px[c] << (8 * (1 + c));
-    bool opaque = alphaIdx < 0 ? true : px[alphaIdx] != 0;
-    unsigned char m = mask[y * maskRowBytes + x];
-    unsigned char cm = mask2 ? mask2[y * mask2RowBytes + x] : (unsigned char)~m;
-    bool hasValue = useContext && m == 0 && opaque;
-    tword[p] = chans | (Word)(hasValue ? 1u : 0u);
-    cword[cidx(w, x, y)] = chans | (Word)(cm == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
-    tflag[p] = m != 0;
-    cflag[p] = (cm == 0xFF) && opaque;
-  }
+  const unsigned quads = ((unsigned)h + 3) / 4;
+  for (unsigned q = blockIdx.y; q < quads; q += gridDim.y)
+    for (unsigned x = blockIdx.x * blockDim.x + threadIdx.x; x < (unsigned)w; x += gridDim.x * blockDim.x) {
+      Word c4[4];
+      for (int r = 0; r < 4; r++) {
+        const unsigned y = q * 4 + r;
+        c4[r] = 0;
+        if (y >= (unsigned)h) continue;
+        const size_t p = (size_t)y * (unsigned)w + x;
+        const unsigned char *px = image + y * rowBytes + (size_t)x * bpp;
+        Word chans = 0;
+        for (int c = 0; c < nColors; c++) chans |= (Word)px[c] << (8 * (1 + c));
+        const bool opaque = alphaIdx < 0 ? true : px[alphaIdx] != 0;
+        const unsigned char m = mask[y * maskRowBytes + x];
+        const unsigned char cm = mask2 ? mask2[y * mask2RowBytes + x] : (unsigned char)~m;
+        const bool hasValue = useContext && m == 0 && opaque;
+        tword[p] = chans | (Word)(hasValue ? 1u : 0u);
+        c4[r] = chans | (Word)(cm == 0xFF ? (opaque ? 0xFFu : 0xFEu) : 0u);
+        tflag[p] = m != 0;
+        cflag[p] = (cm == 0xFF) && opaque;
+      }
+      store4(cword + cidx(w, x, q * 4), c4);
+    }
 }
 
 hipError_t launch_adapt_simple(hipStream_t s, bool wide, const unsigned char *image, size_t rowBytes, unsigned bpp,
@@ -2198,10 +2218,8 @@ hipError_t launch_adapt_simple(hipStream_t s, bool wide, const unsigned char *im
                                size_t mask2RowBytes, int w, int h, int nColors, int alphaIdx, int useContext,
                                void *tword, void *cword, uint8_t *tflag, uint8_t *cflag)
 {
-  size_t n = (size_t)w * h;
-  unsigned blocks = (unsigned)((n + 255) / 256);
-  if (blocks > 65536) blocks = 65536;
-  if (blocks == 0) blocks = 1;
+  const dim3 blocks(std::max(1u, std::min(64u, ((unsigned)w + 255) / 256)),
+                    std::max(1u, std::min(65535u, ((unsigned)h + 3) / 4)));
   if (wide)
     hipLaunchKernelGGL((k_adapt_simple<uint64_t>), dim3(blocks), dim3(256), 0, s, image, rowBytes, bpp, mask,
                        maskRowBytes, mask2, mask2RowBytes, w, h, nColors, alphaIdx, useContext, (uint64_t *)tword,
@@ -2299,13 +2317,28 @@ constexpr int CT_THREADS = 256;
 constexpr int CT_ITEMS = 16;
 constexpr int CT_TILE = CT_THREADS * CT_ITEMS;
 
+// A thread's CT_ITEMS (16) flags as 0/1 bytes: one 16-byte load where the run is whole and
+// aligned, byte loads at the end of the buffer.
+__device__ __forceinline__ void load_flags(const uint8_t *flags, size_t n, size_t base, uint8_t (&f)[CT_ITEMS])
+{
+  if (base + CT_ITEMS <= n && ((uintptr_t)(flags + base) & 15u) == 0) {
+    const uint4 v = *(const uint4 *)(flags + base);
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+    for (int q = 0; q < 4; q++)
+      for (int b = 0; b < 4; b++) f[q * 4 + b] = ((w4[q] >> (8 * b)) & 0xFFu) != 0;
+  } else {
+    for (int i = 0; i < CT_ITEMS; i++) f[i] = (base + i < n) ? (flags[base + i] != 0) : 0;
+  }
+}
+
 __global__ __launch_bounds__(CT_THREADS) void k_tile_counts(const uint8_t *flags, size_t n, unsigned *tileCount)
 {
   __shared__ unsigned red[CT_THREADS / 64];
   size_t base = (size_t)blockIdx.x * CT_TILE + (size_t)threadIdx.x * CT_ITEMS;
+  uint8_t f[CT_ITEMS];
+  load_flags(flags, n, base, f);
   unsigned c = 0;
-  for (int i = 0; i < CT_ITEMS; i++)
-    if (base + i < n) c += flags[base + i] != 0;
+  for (int i = 0; i < CT_ITEMS; i++) c += f[i];
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if (lane_id() == 0) red[threadIdx.x >> 6] = c;
   __syncthreads();
@@ -2344,17 +2377,18 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(unsigned *tileCount, unsign
   if (threadIdx.x == 0) *total = carry;
 }
 
+// The tile's points are gathered in LDS in order, then written out by consecutive threads
+// (a thread's own points would go out as 64 scattered runs per store).
 __global__ __launch_bounds__(CT_THREADS) void k_tile_scatter(const uint8_t *flags, size_t n, int w,
                                                              const unsigned *tileOffset, int2 *out)
 {
   __shared__ unsigned waveSum[CT_THREADS / 64];
+  __shared__ int2 stage[CT_TILE];
   size_t base = (size_t)blockIdx.x * CT_TILE + (size_t)threadIdx.x * CT_ITEMS;
   uint8_t f[CT_ITEMS];
+  load_flags(flags, n, base, f);
   unsigned c = 0;
-  for (int i = 0; i < CT_ITEMS; i++) {
-    f[i] = (base + i < n) ? (flags[base + i] != 0) : 0;
-    c += f[i];
-  }
+  for (int i = 0; i < CT_ITEMS; i++) c += f[i];
   unsigned x = c;
   for (int o = 1; o < 64; o <<= 1) {
     unsigned y = __shfl_up(x, o);
@@ -2362,14 +2396,26 @@ __global__ __launch_bounds__(CT_THREADS) void k_tile_scatter(const uint8_t *flag
   }
   if (lane_id() == 63) waveSum[threadIdx.x >> 6] = x;
   __syncthreads();
-  unsigned off = tileOffset[blockIdx.x];
-  for (int wv = 0; wv < (int)(threadIdx.x >> 6); wv++) off += waveSum[wv];
+  unsigned off = 0, total = 0;
+  for (int wv = 0; wv < CT_THREADS / 64; wv++) {
+    if (wv < (int)(threadIdx.x >> 6)) off += waveSum[wv];
+    total += waveSum[wv];
+  }
   off += x - c;
-  for (int i = 0; i < CT_ITEMS; i++)
-    if (f[i]) {
-      size_t p = base + i;
-      out[off++] = make_int2((int)(p % (unsigned)w), (int)(p / (unsigned)w));
+  if (c) {
+    // the run's first pixel, then step along the row (one division per thread)
+    unsigned px = (unsigned)(base % (unsigned)w), py = (unsigned)(base / (unsigned)w);
+    for (int i = 0; i < CT_ITEMS; i++) {
+      if (f[i]) stage[off++] = make_int2((int)px, (int)py);
+      if (++px == (unsigned)w) {
+        px = 0;
+        py++;
+      }
     }
+  }
+  __syncthreads();
+  int2 *dst = out + tileOffset[blockIdx.x];
+  for (unsigned i = threadIdx.x; i < total; i += CT_THREADS) dst[i] = stage[i];
 }
 
 size_t compact_temp_bytes(size_t n) { return (((n + CT_TILE - 1) / CT_TILE) + 64) * sizeof(unsigned); }
@@ -2628,17 +2674,16 @@ hipError_t launch_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n)
 template <typename Word>
 __global__ void k_tile_init(const Word *tword, int tw, int th, uint8_t *tileHas)
 {
-  const int tilesW = (tw + kTile - 1) / kTile;
-  size_t n = (size_t)tw * th;
-  for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
-    if (tword[p] & 1u) tileHas[(p / (unsigned)tw) / kTile * (size_t)tilesW + (p % (unsigned)tw) / kTile] = 1;
+  const unsigned tilesW = (unsigned)(tw + kTile - 1) / kTile;
+  for (unsigned y = blockIdx.y; y < (unsigned)th; y += gridDim.y)
+    for (unsigned x = blockIdx.x * blockDim.x + threadIdx.x; x < (unsigned)tw; x += gridDim.x * blockDim.x)
+      if (tword[(size_t)y * (unsigned)tw + x] & 1u) tileHas[(size_t)(y / kTile) * tilesW + x / kTile] = 1;
 }
 
 hipError_t launch_tile_init(hipStream_t s, bool wide, const void *tword, int tw, int th, uint8_t *tileHas)
 {
-  size_t n = (size_t)tw * th;
-  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
-  if (!blocks) return hipSuccess;
+  if (tw <= 0 || th <= 0) return hipSuccess;
+  const dim3 blocks(std::min(64u, ((unsigned)tw + 255) / 256), std::min(65535u, (unsigned)th));
   if (wide) hipLaunchKernelGGL((k_tile_init<uint64_t>), dim3(blocks), dim3(256), 0, s, (const uint64_t *)tword, tw, th, tileHas);
   else hipLaunchKernelGGL((k_tile_init<uint32_t>), dim3(blocks), dim3(256), 0, s, (const uint32_t *)tword, tw, th, tileHas);
   return hipGetLastError();
