@@ -326,12 +326,22 @@ __global__ void k_tile_context(const Word *tword, const uint8_t *tflag, int tw, 
   if (t >= tilesW * tilesH) return;
   const int x0 = (t % tilesW) * kTile, y0 = (t / tilesW) * kTile;
   bool all = x0 + kTile <= tw && y0 + kTile <= th, any = false;
-  for (int dy = 0; dy < kTile && y0 + dy < th; dy++)
-    for (int dx = 0; dx < kTile && x0 + dx < tw; dx++) {
-      const size_t p = (size_t)(y0 + dy) * (unsigned)tw + (unsigned)(x0 + dx);
-      all = all && (tword[p] & 1u);
-      any = any || tflag[p];
+  if (sizeof(Word) == 4 && all && (tw & 3) == 0) {
+    // a whole tile with 16-byte aligned rows: each row's 8 words and 8 flags in four loads
+    for (int dy = 0; dy < kTile; dy++) {
+      const size_t p = (size_t)(y0 + dy) * (unsigned)tw + (unsigned)x0;
+      const uint4 a = *(const uint4 *)(tword + p), b = *(const uint4 *)(tword + p + 4);
+      all = all && (a.x & a.y & a.z & a.w & b.x & b.y & b.z & b.w & 1u);
+      any = any || (*(const uint32_t *)(tflag + p) | *(const uint32_t *)(tflag + p + 4));
     }
+  } else {
+    for (int dy = 0; dy < kTile && y0 + dy < th; dy++)
+      for (int dx = 0; dx < kTile && x0 + dx < tw; dx++) {
+        const size_t p = (size_t)(y0 + dy) * (unsigned)tw + (unsigned)(x0 + dx);
+        all = all && (tword[p] & 1u);
+        any = any || tflag[p];
+      }
+  }
   full[t] = all ? 1 : 0;
   hasTarget[t] = any ? 1 : 0;
 }
