@@ -877,6 +877,20 @@ def test_segmented_visits_give_each_target_the_wave_kernels_result(fmt, probes, 
             assert np.array_equal(got[0], ref[0]) and got[1:] == ref[1:], lpt
 
 
+def test_pass_cap_keeps_the_per_pass_counts():
+    """A run capped at two passes reads its last pass's counters together with the run's
+    statistics instead of after the pass.  With one target per launch (deterministic) its two
+    passes are those of a three-pass run, whose second pass is read the ordinary way: the
+    same targets per pass, the same betters."""
+    img, mask = _case_inputs(R.T_RGB, (40, 52), 77)
+    p = R.default_params(maxProbeCount=30)
+    r2, _ = R.image_synth(img, mask, R.T_RGB, p, batch=1, max_passes=2)
+    r3, _ = R.image_synth(img, mask, R.T_RGB, p, batch=1, max_passes=3)
+    assert r2.err == 0 and r3.err == 0 and r2.passes == 2 and r3.passes >= 2
+    assert list(r2.passSize)[:2] == list(r3.passSize)[:2]
+    assert list(r2.betters)[:2] == list(r3.betters)[:2] and r2.betters[1] > 0
+
+
 @pytest.mark.parametrize("lpt", [16, 32, 64])
 def test_segmented_visits_keep_batched_quality(lpt):
     """Concurrent segmented visits keep the batched mode's invariants and its mean error
