@@ -1291,15 +1291,20 @@ __device__ __forceinline__ void visit_seg(const Geometry &g, const SynthBuffers 
     const Word q = cword[ci];
     const uint32_t pr = b.prober[ci];
     bool valid = in && (q & 0xFEu) == 0xFEu && pr != k;
-    // an earlier neighbour proposing the same point was scored (and recorded) already
-    if (valid)
-      for (unsigned j = 0; j < i; j++) {
-        const int2 e = sSrc[j];
-        if (e.x == c.x && e.y == c.y) {
-          valid = false;
-          break;
-        }
+    // an earlier neighbour proposing the same point was scored (and recorded) already.
+    // Entries [0, i) are read in pairs (entry i at most is read past them, and i < count <= 64)
+    // and compared without an early exit: one LDS round trip per two entries and no divergent
+    // exit (unrolling further costs registers and, on some instances, a wave per SIMD).
+    if (valid) {
+      bool dup = false;
+#pragma unroll 1
+      for (unsigned j = 0; j < i; j += 2) {
+        const int2 e0 = sSrc[j], e1 = sSrc[j + 1];
+        dup |= (e0.x == c.x) & (e0.y == c.y);
+        dup |= (j + 1 < i) & (e1.x == c.x) & (e1.y == c.y);
       }
+      valid = !dup;
+    }
     uint32_t F = 0, M = 0;
     bool alive = false;
     if (valid) score_candidate<Word, 8, 8, NC>(g, cword, c.x, c.y, count, sOff, sPix, sT, sTm, best, F, M, alive, skip0);
