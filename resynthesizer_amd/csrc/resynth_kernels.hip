@@ -1209,12 +1209,16 @@ __device__ unsigned gather_neighbours_seg(const Seg<LPT> &S, const Geometry &g, 
   if (g.skipEmpty && tileHas && active)
     base = scan_start(g, tileHas, firstIdx, tx, ty, S.sl, (unsigned)LPT,
                       [&S](bool v) { return S.bits(__ballot(v)) != 0ull; });
+  // a round's offsets depend only on base: each round loads the next round's, so only the
+  // pixel gather waits on memory (offNext always holds entry base + sl, or entry 0 past nO)
+  int2 offNext = offsets[base + S.sl < g.nO ? base + S.sl : 0u];
   for (;;) {
     const bool need = active && count < L && base < g.nO;
     if (__ballot(need) == 0ull) break;
     if (need) {
       const unsigned j = base + S.sl;
-      const int2 off = offsets[j < g.nO ? j : 0u];
+      const int2 off = offNext;
+      offNext = offsets[j + LPT < g.nO ? j + LPT : 0u];
       int px = tx + off.x, py = ty + off.y;
       bool in = j < g.nO;
       if (px < 0) { if (g.tileH) px += g.tw; else in = false; }
